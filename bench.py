@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the batched shape-query hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+A step is one pass of the hot path over one batch of synthetic input resident in
+HBM. Workload "shapes-box-tube": UnplacedBox (10, 20, 30) and full-phi
+UnplacedTube (5, 10, 15), 2^20 points each (uniform in 1.2x the bounding box,
+isotropic directions, float64 SOA), all six methods per shape per step.
+
+Prints ONE JSON line (rank 0) with throughput, the roofline fraction of the
+dominant kernel (timed with HIP events on the stream it runs on) and the CPU
+scalar baseline (the oracle restatement, timed on this host).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per query (float64 SOA in, result out)
+BYTES = {"Contains": 25, "Inside": 28, "DistanceToIn": 56, "DistanceToOut": 56, "SafetyToIn": 32,
+         "SafetyToOut": 32}
+METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="shapes-box-tube")
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def run_shapes(args):
+    import numpy as np
+    import torch
+
+    from vecgeom_amd import SOA3D, make_shape, workloads
+
+    n = args.n
+    shapes = []
+    for i, (kind, params) in enumerate(workloads.CONFIG0_SHAPES):
+        s = make_shape(kind, params)
+        p, d = workloads.shape_stream(s.Extent(), n, workloads.SEED_BASE + i)
+        P, D = SOA3D.from_array(p), SOA3D.from_array(d)
+        outs = {m: torch.empty(n, dtype=torch.uint8 if m == "Contains" else
+                               (torch.int32 if m == "Inside" else torch.float64), device="cuda") for m in METHODS}
+        shapes.append((kind, params, s, P, D, outs, p, d))
+
+    def step():
+        for kind, params, s, P, D, outs, _, _ in shapes:
+            for m in METHODS:
+                if m.startswith("Distance"):
+                    getattr(s, m)(P, D, out=outs[m])
+                else:
+                    getattr(s, m)(P, out=outs[m])
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    ms_per_step = ms_total / args.steps
+    queries_per_step = n * len(METHODS) * len(shapes)
+    value = queries_per_step / (ms_per_step * 1e-3)
+
+    # per-kernel average durations over their own timed loops (same stream, HIP events)
+    reps = max(20, args.steps)
+    per_kernel = []
+    for kind, params, s, P, D, outs, _, _ in shapes:
+        for m in METHODS:
+            f = (lambda s=s, m=m, P=P, D=D, o=outs[m]: getattr(s, m)(P, D, out=o)) if m.startswith("Distance") \
+                else (lambda s=s, m=m, P=P, o=outs[m]: getattr(s, m)(P, out=o))
+            f()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                f()
+            b.record(stream)
+            torch.cuda.synchronize()
+            us = a.elapsed_time(b) * 1e3 / reps
+            gbs = BYTES[m] * n / (us * 1e-6) / 1e9
+            per_kernel.append({"kernel": "%s.%s" % (kind, m), "avg_us": round(us, 3), "GBps": round(gbs, 1)})
+    dom = max(per_kernel, key=lambda k: k["avg_us"])
+    kind, m = dom["kernel"].split(".")
+    roofline = {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_query_fp64<%sData,%s> (%s)" % (kind.capitalize(), m, dom["kernel"]),
+                "bytes_per_launch": BYTES[m] * n}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(shapes, args.cpu_seconds)
+    return {
+        "metric": "shape queries/s (FP64 GPU)",
+        "value": round(value, 1),
+        "unit": "queries/s",
+        "ms_per_step": round(ms_per_step, 5),
+        "wall_s": round(wall, 4),
+        "config": {"workload": "shapes-box-tube", "shapes": [[k, p] for k, p, *_ in shapes], "points_per_shape": n,
+                   "methods": METHODS, "queries_per_step": queries_per_step},
+        "dtype": "f64",
+        "roofline": roofline,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline(shapes, budget_s):
+    """Oracle (CPU scalar FP64 restatement) on a bounded sample of the same inputs, 1 thread."""
+    import numpy as np
+
+    from oracle.pyoracle import OracleShape
+
+    sample = 1 << 18
+    done, t_used = 0, 0.0
+    while t_used < budget_s:
+        for kind, params, s, P, D, outs, p, d in shapes:
+            o = OracleShape(kind, params)
+            ps, ds = np.ascontiguousarray(p[:, :sample]), np.ascontiguousarray(d[:, :sample])
+            t0 = time.perf_counter()
+            for m in METHODS:
+                o.query(m, ps, ds if m.startswith("Distance") else None, threads=1)
+            t_used += time.perf_counter() - t0
+            done += sample * len(METHODS)
+        if t_used <= 0:
+            break
+    model, ncpu = cpu_info()
+    return {"value": round(done / t_used, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": "%d queries: first 2^18 points of each shape's stream x 6 methods, repeated for ~%.0f s; "
+                      "host %s, %d logical CPUs" % (done, budget_s, model, ncpu)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; --gpus %d is not supported "
+              "(shape queries and navigation tracks shard trivially across GPUs)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.synchronize()
+    res = run_shapes(args)
+    line = {"metric": res["metric"], "value": res["value"], "unit": res["unit"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res["dtype"],
+            "data": "synthetic", "config": res["config"], "roofline": res["roofline"],
+            "cpu_baseline": res["cpu_baseline"], "kernels": res["kernels"]}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

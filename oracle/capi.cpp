@@ -1,0 +1,93 @@
+// Oracle C ABI (ctypes-friendly) used by tests/ and by the CPU-baseline leg of bench.py.
+// TEST INFRASTRUCTURE ONLY (see common.h).
+//
+// Shape parameter lists (same order as the reference constructors):
+//   0 box         dx dy dz                                   (volumes/UnplacedBox.h)
+//   1 tube        rmin rmax dz sphi dphi                     (volumes/UnplacedTube.h:29)
+#include "shapes.h"
+#include <chrono>
+#include <thread>
+#include <vector>
+
+using namespace vgo;
+
+namespace vgo {
+Shape *MakeShape(int type, const double *a, int n)
+{
+  switch (type) {
+  case 0:
+    return n >= 3 ? new Box(a[0], a[1], a[2]) : nullptr;
+  case 1:
+    return n >= 5 ? new Tube(a[0], a[1], a[2], a[3], a[4]) : nullptr;
+  default:
+    return nullptr;
+  }
+}
+} // namespace vgo
+
+enum { OP_CONTAINS = 0, OP_INSIDE = 1, OP_DIST_IN = 2, OP_DIST_OUT = 3, OP_SAFETY_IN = 4, OP_SAFETY_OUT = 5 };
+
+static void run_range(const Shape *s, int op, const double *px, const double *py, const double *pz,
+                      const double *dx, const double *dy, const double *dz, const double *step, void *out,
+                      int64_t b, int64_t e)
+{
+  for (int64_t i = b; i < e; ++i) {
+    V3 p(px[i], py[i], pz[i]);
+    switch (op) {
+    case OP_CONTAINS:
+      ((uint8_t *)out)[i] = s->Contains(p) ? 1 : 0;
+      break;
+    case OP_INSIDE:
+      ((int32_t *)out)[i] = s->Inside(p);
+      break;
+    case OP_DIST_IN:
+      ((double *)out)[i] = s->DistanceToIn(p, V3(dx[i], dy[i], dz[i]), step ? step[i] : kInfLength);
+      break;
+    case OP_DIST_OUT:
+      ((double *)out)[i] = s->DistanceToOut(p, V3(dx[i], dy[i], dz[i]), step ? step[i] : kInfLength);
+      break;
+    case OP_SAFETY_IN:
+      ((double *)out)[i] = s->SafetyToIn(p);
+      break;
+    case OP_SAFETY_OUT:
+      ((double *)out)[i] = s->SafetyToOut(p);
+      break;
+    }
+  }
+}
+
+extern "C" {
+
+void *vgo_shape_new(int type, const double *params, int nparams) { return MakeShape(type, params, nparams); }
+void vgo_shape_free(void *s) { delete (Shape *)s; }
+
+void vgo_shape_extent(void *s, double *lo3, double *hi3)
+{
+  V3 lo, hi;
+  ((Shape *)s)->Extent(lo, hi);
+  lo3[0] = lo.x; lo3[1] = lo.y; lo3[2] = lo.z;
+  hi3[0] = hi.x; hi3[1] = hi.y; hi3[2] = hi.z;
+}
+
+// Runs op over n points with `threads` host threads (1 = the scalar reference loop).
+int vgo_shape_query(void *s, int op, const double *px, const double *py, const double *pz, const double *dx,
+                    const double *dy, const double *dz, const double *step, void *out, int64_t n, int threads)
+{
+  if (!s || op < 0 || op > 5) return -1;
+  if ((op == OP_DIST_IN || op == OP_DIST_OUT) && (!dx || !dy || !dz)) return -2;
+  if (threads <= 1) {
+    run_range((Shape *)s, op, px, py, pz, dx, dy, dz, step, out, 0, n);
+    return 0;
+  }
+  std::vector<std::thread> pool;
+  int64_t chunk = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    pool.emplace_back(run_range, (Shape *)s, op, px, py, pz, dx, dy, dz, step, out, b, e);
+  }
+  for (auto &th : pool) th.join();
+  return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,85 @@
+// Oracle shapes: scalar FP64 restatement of VecGeom's unplaced-shape kernels.
+// TEST INFRASTRUCTURE ONLY (see common.h).
+//
+// Each shape mirrors the VUnplacedVolume virtual interface
+// (VecGeom/volumes/UnplacedVolume.h:88-223) as reached through
+// CommonUnplacedVolumeImplHelper (VecGeom/volumes/UnplacedVolumeImplHelper.h:77-180):
+// the output is initialised to kInfLength and the static kernel is called.
+#pragma once
+#include "common.h"
+#include <vector>
+
+namespace vgo {
+
+class Shape {
+public:
+  virtual ~Shape() {}
+  virtual bool Contains(V3 const &p) const = 0;
+  virtual int Inside(V3 const &p) const = 0;
+  virtual double DistanceToIn(V3 const &p, V3 const &d, double step_max) const = 0;
+  virtual double DistanceToOut(V3 const &p, V3 const &d, double step_max) const = 0;
+  virtual double SafetyToIn(V3 const &p) const = 0;
+  virtual double SafetyToOut(V3 const &p) const = 0;
+  virtual void Extent(V3 &lo, V3 &hi) const = 0;
+};
+
+// Phi wedge: VecGeom/volumes/Wedge_Evolution.h, ctor source/Wedge_Evolution.cpp:17-128
+struct Wedge {
+  double dphi = 0;
+  V3 along1, along2, normal1, normal2;
+  void Set(double angle, double zeroangle);
+  bool IsOnSurfaceGeneric(V3 const &along, V3 const &normal, V3 const &p) const;
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  double SafetyToOut(V3 const &p) const;
+  double SafetyToIn(V3 const &p) const;
+};
+
+// ---------------------------------------------------------------- Box
+// VecGeom/volumes/kernel/BoxImplementation.h:76-167
+class Box : public Shape {
+public:
+  V3 h;
+  Box(double dx, double dy, double dz) : h(dx, dy, dz) {}
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override { lo = -h; hi = h; }
+};
+
+// ---------------------------------------------------------------- Tube
+// VecGeom/volumes/TubeStruct.h:15-176, VecGeom/volumes/kernel/TubeImplementation.h:389-964
+// (UniversalTube: rmin and phi treatment decided at run time, TubeTypes.h:92-144)
+class Tube : public Shape {
+public:
+  double rmin, rmax, z, sphi, dphi;
+  double rmin2, rmax2, a1x, a1y, a2x, a2y;
+  double tolIrmin2, tolOrmin2, tolIrmax2, tolOrmax2, tolIz, tolOz;
+  double tolIrmin, tolOrmin, tolIrmax, tolOrmax, maxVal;
+  Wedge wedge;
+  Tube(double rmin, double rmax, double dz, double sphi, double dphi);
+  bool hasRmin() const { return rmin > 0; }
+  bool hasPhi() const { return dphi < kTwoPi; }
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  // internals shared with other shapes
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  double DistanceToInKernel(V3 const &p, V3 const &d) const;
+  bool InSector(double x, double y, bool includeSurface) const;
+  bool CircleHit(double b, double c, V3 const &p, V3 const &d, bool largest, bool sectorCheck, double &dist) const;
+  bool PhiPlaneHit(double ax, double ay, double nx, double ny, V3 const &p, V3 const &d, bool posDir, bool toIn,
+                   double &dist) const;
+  double PhiPlaneSafetyOutside(V3 const &p) const;
+};
+
+// Factory used by the C ABI. Parameter meaning per type is listed in capi.cpp.
+Shape *MakeShape(int type, const double *params, int nparams);
+
+} // namespace vgo

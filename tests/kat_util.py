@@ -1,0 +1,45 @@
+"""Helpers for the known-answer fixtures in tests/golden/kat_*.json.
+
+ApproxEqual follows test/unit_tests/ApproxEqual.h of the reference: equal, or
+absolute 1e-6 when one side is zero, or both beyond 1e100 with equal sign
+(infinity / kInfLength), or relative 1e-6.
+"""
+import json
+import os
+
+import numpy as np
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INSIDE = {"Inside": 1, "Surface": 2, "Outside": 3}
+
+
+def approx_equal(x, y, tol=1e-6):
+    if x == y:
+        return True
+    if x * y == 0.0:
+        return abs(x - y) < tol
+    if abs(x) > 1e100 or abs(y) > 1e100:
+        return x * y > 0 and abs(x) > 1e100 and abs(y) > 1e100
+    return abs(x - y) / (abs(x) + abs(y)) < tol
+
+
+def load(name):
+    with open(os.path.join(GOLDEN, "kat_%s.json" % name)) as f:
+        return json.load(f)["records"]
+
+
+def check(rec, got):
+    e = rec["expect"]
+    if e["kind"] == "approx":
+        return approx_equal(got, e["value"])
+    if e["kind"] == "anyof":
+        return any(approx_equal(got, v) for v in e["values"])
+    if e["kind"] == "inside":
+        return int(got) == INSIDE[e["value"]]
+    if e["kind"] == "negative":
+        return got < 0
+    raise ValueError(e)
+
+
+def soa(vecs):
+    return np.ascontiguousarray(np.array(vecs, dtype=np.float64).T)

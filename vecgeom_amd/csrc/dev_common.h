@@ -1,0 +1,116 @@
+// Device-side constants and small vector helpers for the CDNA4 kernels.
+//
+// The kernels are templated on the arithmetic type R (double for the FP64 path,
+// float for the bulk of the mixed path). Tolerances follow
+// VecGeom/base/Math.h:26-53; min/max use the (a < b) ? b : a form so FP64
+// results match a scalar host evaluation bit for bit (the library is compiled
+// with -ffp-contract=off; FP32 code asks for fused multiply-add explicitly).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+namespace vgmi {
+
+#define VGMI_DEV __device__ __forceinline__
+
+constexpr double kTolerance     = 1e-9;
+constexpr double kHalfTolerance = 0.5 * kTolerance;
+constexpr double kPushTolerance = 1e-6;
+constexpr double kConeTolerance = 1e-7;
+constexpr double kInfLength     = DBL_MAX;
+constexpr double kPi            = 3.14159265358979323846;
+constexpr double kTwoPi         = 2. * kPi;
+
+constexpr int kInside  = 1;
+constexpr int kSurface = 2;
+constexpr int kOutside = 3;
+
+template <class R>
+struct Vec3 {
+  R x, y, z;
+  VGMI_DEV Vec3() {}
+  VGMI_DEV Vec3(R a, R b, R c) : x(a), y(b), z(c) {}
+  VGMI_DEV Vec3 operator+(Vec3 const &o) const { return Vec3(x + o.x, y + o.y, z + o.z); }
+  VGMI_DEV Vec3 operator-(Vec3 const &o) const { return Vec3(x - o.x, y - o.y, z - o.z); }
+  VGMI_DEV Vec3 operator*(Vec3 const &o) const { return Vec3(x * o.x, y * o.y, z * o.z); }
+  VGMI_DEV Vec3 operator*(R s) const { return Vec3(x * s, y * s, z * s); }
+  VGMI_DEV Vec3 operator-() const { return Vec3(-x, -y, -z); }
+  VGMI_DEV R Dot(Vec3 const &o) const { return x * o.x + y * o.y + z * o.z; }
+  VGMI_DEV R Mag2() const { return x * x + y * y + z * z; }
+  VGMI_DEV R Perp2() const { return x * x + y * y; }
+};
+
+template <class R>
+VGMI_DEV Vec3<R> operator*(R s, Vec3<R> const &v)
+{
+  return Vec3<R>(s * v.x, s * v.y, s * v.z);
+}
+
+template <class R>
+VGMI_DEV R vmax(R a, R b)
+{
+  return a < b ? b : a;
+}
+template <class R>
+VGMI_DEV R vmin(R a, R b)
+{
+  return b < a ? b : a;
+}
+template <class R>
+VGMI_DEV R vabs(R a)
+{
+  return __builtin_fabs(a);
+}
+VGMI_DEV double vsqrt(double a) { return __builtin_sqrt(a); }
+VGMI_DEV float vsqrt(float a) { return __builtin_sqrtf(a); }
+VGMI_DEV double vcopysign(double a, double b) { return __builtin_copysign(a, b); }
+VGMI_DEV float vcopysign(float a, float b) { return __builtin_copysignf(a, b); }
+
+template <class R>
+VGMI_DEV R Max3(Vec3<R> const &v)
+{
+  return vmax(v.x, vmax(v.y, v.z));
+}
+template <class R>
+VGMI_DEV R Min3(Vec3<R> const &v)
+{
+  return vmin(v.x, vmin(v.y, v.z));
+}
+template <class R>
+VGMI_DEV Vec3<R> Abs3(Vec3<R> const &v)
+{
+  return Vec3<R>(vabs(v.x), vabs(v.y), vabs(v.z));
+}
+
+// Guarded reciprocal argument and sign (VecGeom/base/Global.h:163-177; Sign is CopySign(1, x))
+template <class R>
+VGMI_DEV R NonZero(R x)
+{
+  return x + vcopysign(R(1.0e-30), x);
+}
+template <class R>
+VGMI_DEV R NonZeroAbs(R x)
+{
+  return vabs(x) + R(1.0e-30);
+}
+template <class R>
+VGMI_DEV R Sign(R x)
+{
+  return vcopysign(R(1), x);
+}
+
+template <class R>
+VGMI_DEV R InfLen();
+template <>
+VGMI_DEV double InfLen<double>()
+{
+  return DBL_MAX;
+}
+template <>
+VGMI_DEV float InfLen<float>()
+{
+  return FLT_MAX;
+}
+
+} // namespace vgmi

@@ -1,0 +1,72 @@
+// Flat shape-parameter records shared by the host builder and the device kernels.
+//
+// Each record holds the shape's defining parameters plus the cached members the
+// reference precomputes in its Unplaced*/…Struct constructors, so kernels never
+// recompute trigonometry. Records are POD so they can be passed by value as
+// kernel arguments (scalar registers) for single-shape launches, or stored in a
+// device table indexed by shape id for navigation and mixed batches.
+#pragma once
+#include <stdint.h>
+
+namespace vgmi {
+
+enum ShapeType : int32_t {
+  kBox = 0,
+  kTube = 1,
+  kCone = 2,
+  kPolycone = 3,
+  kPolyhedron = 4,
+  kTrapezoid = 5,
+  kSphere = 6,
+  kTorus2 = 7,
+  kOrb = 8,
+  kTessellated = 9,
+  kHype = 10,
+  kNumShapeTypes = 11
+};
+
+enum QueryOp : int32_t {
+  kOpContains = 0,
+  kOpInside = 1,
+  kOpDistIn = 2,
+  kOpDistOut = 3,
+  kOpSafetyIn = 4,
+  kOpSafetyOut = 5,
+  kNumOps = 6
+};
+
+// Box: half lengths (VecGeom/volumes/BoxStruct.h:20)
+struct BoxData {
+  double h[3];
+};
+
+// Tube (VecGeom/volumes/TubeStruct.h:15-43): defining parameters, cached
+// squares and tolerance bands, phi-sector unit vectors and wedge normals.
+struct TubeData {
+  double rmin, rmax, z, sphi, dphi;
+  double rmin2, rmax2;
+  double a1x, a1y, a2x, a2y;     // along vectors of the two phi planes
+  double n1x, n1y, n2x, n2y;     // inward wedge normals
+  double tolIrmin2, tolOrmin2, tolIrmax2, tolOrmax2;
+  double maxVal;
+  int32_t hasRmin, hasPhi;       // rmin > 0, dphi < 2 pi
+  int32_t phiGtPi;               // dphi > pi   (PhiPlaneSafety)
+  int32_t phiLePi;               // dphi <= pi  (PointInCyclicalSector)
+  int32_t wedgeLtPi;             // dphi < pi   (Wedge kernels)
+  int32_t pad_;
+};
+
+// Union of all records: the device shape table stores one per unplaced shape.
+// Variable-length data (polycone sections, tessellated facets, ...) lives in a
+// separate double pool addressed by an offset inside the record.
+struct ShapeRecord {
+  int32_t type;
+  int32_t pad_;
+  union {
+    BoxData box;
+    TubeData tube;
+    double raw[64];
+  } u;
+};
+
+} // namespace vgmi

@@ -1,0 +1,210 @@
+// Single-shape SOA query kernels and the C ABI entry points for them.
+//
+// One launch streams one shape over n points: each lane reads its point (and
+// direction) from the SOA arrays with unit-stride, fully coalesced 8-byte loads,
+// the shape record arrives by value in the kernel arguments (scalar registers,
+// no per-lane parameter traffic), and results are written back coalesced.
+// The FP64 path is the reference's scalar-backend arithmetic; the MIXED path
+// runs the same kernel in FP32 and re-evaluates every point whose FP32 result
+// is within the near-surface band in FP64, so decisions stay exact.
+#include "vgmi_internal.h"
+#include "dev_box.h"
+#include "dev_tube.h"
+
+namespace vgmi {
+
+constexpr int kBlock = 256;
+
+template <class R>
+VGMI_DEV Vec3<R> load3(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                       size_t i)
+{
+  return Vec3<R>(R(__builtin_nontemporal_load(x + i)), R(__builtin_nontemporal_load(y + i)),
+                 R(__builtin_nontemporal_load(z + i)));
+}
+
+// Static dispatch of one method on one shape record type, arithmetic type R.
+template <class R, class Data>
+struct Shape;
+
+template <class R>
+struct Shape<R, BoxData> {
+  static VGMI_DEV bool Contains(BoxData const &s, Vec3<R> const &p) { return box::Contains(s, p); }
+  static VGMI_DEV int Inside(BoxData const &s, Vec3<R> const &p) { return box::Inside(s, p); }
+  static VGMI_DEV R DistIn(BoxData const &s, Vec3<R> const &p, Vec3<R> const &d) { return box::DistanceToIn(s, p, d); }
+  static VGMI_DEV R DistOut(BoxData const &s, Vec3<R> const &p, Vec3<R> const &d) { return box::DistanceToOut(s, p, d); }
+  static VGMI_DEV R SafeIn(BoxData const &s, Vec3<R> const &p) { return box::SafetyToIn(s, p); }
+  static VGMI_DEV R SafeOut(BoxData const &s, Vec3<R> const &p) { return box::SafetyToOut(s, p); }
+};
+
+template <class R>
+struct Shape<R, TubeData> {
+  static VGMI_DEV bool Contains(TubeData const &s, Vec3<R> const &p) { return tube::Contains(s, p); }
+  static VGMI_DEV int Inside(TubeData const &s, Vec3<R> const &p) { return tube::Inside(s, p); }
+  static VGMI_DEV R DistIn(TubeData const &s, Vec3<R> const &p, Vec3<R> const &d) { return tube::DistanceToIn(s, p, d); }
+  static VGMI_DEV R DistOut(TubeData const &s, Vec3<R> const &p, Vec3<R> const &d) { return tube::DistanceToOut(s, p, d); }
+  static VGMI_DEV R SafeIn(TubeData const &s, Vec3<R> const &p) { return tube::SafetyToIn(s, p); }
+  static VGMI_DEV R SafeOut(TubeData const &s, Vec3<R> const &p) { return tube::SafetyToOut(s, p); }
+};
+
+struct QueryArgs {
+  const double *px, *py, *pz, *dx, *dy, *dz;
+  void *out;
+  size_t n;
+};
+
+template <class Data, int OP>
+__global__ __launch_bounds__(kBlock) void k_query_fp64(Data s, QueryArgs a)
+{
+  using S      = Shape<double, Data>;
+  size_t i     = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  size_t strid = size_t(gridDim.x) * kBlock;
+  for (; i < a.n; i += strid) {
+    Vec3<double> p = load3<double>(a.px, a.py, a.pz, i);
+    if constexpr (OP == kOpContains) {
+      __builtin_nontemporal_store(uint8_t(S::Contains(s, p)), (uint8_t *)a.out + i);
+    } else if constexpr (OP == kOpInside) {
+      __builtin_nontemporal_store(int32_t(S::Inside(s, p)), (int32_t *)a.out + i);
+    } else if constexpr (OP == kOpDistIn || OP == kOpDistOut) {
+      Vec3<double> d = load3<double>(a.dx, a.dy, a.dz, i);
+      double r       = OP == kOpDistIn ? S::DistIn(s, p, d) : S::DistOut(s, p, d);
+      __builtin_nontemporal_store(r, (double *)a.out + i);
+    } else if constexpr (OP == kOpSafetyIn) {
+      __builtin_nontemporal_store(S::SafeIn(s, p), (double *)a.out + i);
+    } else {
+      __builtin_nontemporal_store(S::SafeOut(s, p), (double *)a.out + i);
+    }
+  }
+}
+
+struct QueryAllArgs {
+  const double *px, *py, *pz, *dx, *dy, *dz;
+  uint8_t *contains;
+  int32_t *inside;
+  double *din, *dout, *sin, *sout;
+  size_t n;
+};
+
+template <class Data>
+__global__ __launch_bounds__(kBlock) void k_query_all_fp64(Data s, QueryAllArgs a)
+{
+  using S      = Shape<double, Data>;
+  size_t i     = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  size_t strid = size_t(gridDim.x) * kBlock;
+  for (; i < a.n; i += strid) {
+    Vec3<double> p = load3<double>(a.px, a.py, a.pz, i);
+    if (a.contains) __builtin_nontemporal_store(uint8_t(S::Contains(s, p)), a.contains + i);
+    if (a.inside) __builtin_nontemporal_store(int32_t(S::Inside(s, p)), a.inside + i);
+    if (a.sin) __builtin_nontemporal_store(S::SafeIn(s, p), a.sin + i);
+    if (a.sout) __builtin_nontemporal_store(S::SafeOut(s, p), a.sout + i);
+    if (a.din || a.dout) {
+      Vec3<double> d = load3<double>(a.dx, a.dy, a.dz, i);
+      if (a.din) __builtin_nontemporal_store(S::DistIn(s, p, d), a.din + i);
+      if (a.dout) __builtin_nontemporal_store(S::DistOut(s, p, d), a.dout + i);
+    }
+  }
+}
+
+static unsigned grid_for(size_t n)
+{
+  size_t blocks = (n + kBlock - 1) / kBlock;
+  // grid-stride beyond 64 waves per CU: enough to cover latency on 256 CUs
+  const size_t cap = size_t(256) * 64;
+  return unsigned(blocks < cap ? blocks : cap);
+}
+
+template <class Data>
+static hipError_t launch_query(Data const &s, int op, QueryArgs const &a, hipStream_t st)
+{
+  dim3 g(grid_for(a.n)), b(kBlock);
+  switch (op) {
+  case kOpContains:
+    hipLaunchKernelGGL((k_query_fp64<Data, kOpContains>), g, b, 0, st, s, a);
+    break;
+  case kOpInside:
+    hipLaunchKernelGGL((k_query_fp64<Data, kOpInside>), g, b, 0, st, s, a);
+    break;
+  case kOpDistIn:
+    hipLaunchKernelGGL((k_query_fp64<Data, kOpDistIn>), g, b, 0, st, s, a);
+    break;
+  case kOpDistOut:
+    hipLaunchKernelGGL((k_query_fp64<Data, kOpDistOut>), g, b, 0, st, s, a);
+    break;
+  case kOpSafetyIn:
+    hipLaunchKernelGGL((k_query_fp64<Data, kOpSafetyIn>), g, b, 0, st, s, a);
+    break;
+  default:
+    hipLaunchKernelGGL((k_query_fp64<Data, kOpSafetyOut>), g, b, 0, st, s, a);
+    break;
+  }
+  return hipGetLastError();
+}
+
+} // namespace vgmi
+
+using namespace vgmi;
+
+extern "C" {
+
+vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, const double *px, const double *py,
+                             const double *pz, const double *dx, const double *dy, const double *dz,
+                             const double *step_max, void *out, size_t n, void *stream)
+{
+  (void)step_max; // none of the supported shapes uses step_max (it only bounds iterative solvers)
+  if (!shape) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: null shape");
+  if (op < 0 || op >= kNumOps) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: unknown op %d", op);
+  if (precision != VGMI_FP64 && precision != VGMI_MIXED)
+    return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: unknown precision %d", precision);
+  if (n == 0) return VGMI_OK;
+  if (!px || !py || !pz || !out) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: null point/output pointer");
+  if ((op == kOpDistIn || op == kOpDistOut) && (!dx || !dy || !dz))
+    return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: DistanceToIn/Out need directions");
+  if (precision == VGMI_MIXED) return mixed_shape_query(shape, op, px, py, pz, dx, dy, dz, out, n, stream);
+  QueryArgs a{px, py, pz, dx, dy, dz, out, n};
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e;
+  switch (shape->rec.type) {
+  case kBox:
+    e = launch_query(shape->rec.u.box, op, a, st);
+    break;
+  case kTube:
+    e = launch_query(shape->rec.u.tube, op, a, st);
+    break;
+  default:
+    return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query: shape type %d has no kernel", shape->rec.type);
+  }
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_shape_query launch: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision, const double *px, const double *py,
+                                 const double *pz, const double *dx, const double *dy, const double *dz,
+                                 uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,
+                                 double *safety_in, double *safety_out, size_t n, void *stream)
+{
+  if (!shape) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query_all: null shape");
+  if (precision != VGMI_FP64)
+    return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query_all: only VGMI_FP64 is implemented");
+  if (n == 0) return VGMI_OK;
+  if (!px || !py || !pz) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query_all: null point pointer");
+  if ((dist_in || dist_out) && (!dx || !dy || !dz))
+    return set_error(VGMI_ERR_INVALID, "vgmi_shape_query_all: distances need directions");
+  QueryAllArgs a{px, py, pz, dx, dy, dz, contains, inside, dist_in, dist_out, safety_in, safety_out, n};
+  hipStream_t st = (hipStream_t)stream;
+  dim3 g(grid_for(n)), b(kBlock);
+  switch (shape->rec.type) {
+  case kBox:
+    hipLaunchKernelGGL((k_query_all_fp64<BoxData>), g, b, 0, st, shape->rec.u.box, a);
+    break;
+  case kTube:
+    hipLaunchKernelGGL((k_query_all_fp64<TubeData>), g, b, 0, st, shape->rec.u.tube, a);
+    break;
+  default:
+    return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query_all: shape type %d has no kernel", shape->rec.type);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_shape_query_all launch: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,17 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/vgmi.h"
+#include "shape_data.h"
+
+struct vgmi_shape {
+  vgmi::ShapeRecord rec;
+};
+
+namespace vgmi {
+// Records the message for vgmi_last_error() and returns `code`.
+int set_error(int code, const char *fmt, ...);
+vgmi_status mixed_shape_query(const vgmi_shape *shape, int op, const double *px, const double *py, const double *pz,
+                              const double *dx, const double *dy, const double *dz, void *out, size_t n,
+                              void *stream);
+} // namespace vgmi

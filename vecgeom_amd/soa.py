@@ -1,0 +1,34 @@
+"""SOA3D: three contiguous float64 device arrays x, y, z (VecGeom/base/SOA3D.h:35-41)."""
+import torch
+
+
+class SOA3D:
+    def __init__(self, x, y, z):
+        for a in (x, y, z):
+            if a.dtype != torch.float64 or not a.is_cuda or not a.is_contiguous():
+                raise ValueError("SOA3D components must be contiguous float64 device tensors")
+        if not (x.numel() == y.numel() == z.numel()):
+            raise ValueError("SOA3D components differ in length")
+        self.x, self.y, self.z = x, y, z
+
+    @classmethod
+    def from_array(cls, a, device="cuda"):
+        """a: array-like of shape (3, n)."""
+        t = torch.as_tensor(a, dtype=torch.float64)
+        return cls(*(t[i].contiguous().to(device) for i in range(3)))
+
+    @classmethod
+    def empty(cls, n, device="cuda"):
+        return cls(*(torch.empty(n, dtype=torch.float64, device=device) for _ in range(3)))
+
+    def size(self):
+        return self.x.numel()
+
+    def __len__(self):
+        return self.x.numel()
+
+    def ptrs(self):
+        return self.x.data_ptr(), self.y.data_ptr(), self.z.data_ptr()
+
+    def cpu(self):
+        return torch.stack([self.x, self.y, self.z]).cpu().numpy()
