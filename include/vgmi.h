@@ -106,6 +106,90 @@ VGMI_API vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision
                                  uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,
                                  double *safety_in, double *safety_out, size_t n, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Geometry and BVH navigation.
+ *
+ * A geometry is described by plain arrays (the same data a VecGeom user feeds
+ * to the Unplaced* constructors, LogicalVolume and LogicalVolume::PlaceDaughter):
+ * placement k becomes placed volume id k, the world placed volume gets id
+ * nplaced. vgmi_geom_build closes it (ABBoxes, per-volume BVHs, NavIndex table;
+ * GeoManager::CloseGeometry source/GeoManager.cpp:192-241, BVHManager::Init
+ * source/BVHManager.cpp:24-30, NavIndexTable::CreateTable
+ * management/NavIndexTable.h:128-143) and uploads the flat tables to the
+ * current HIP device (CudaManager::Synchronize source/CudaManager.cpp:101-332).
+ *
+ * Navigation states are NavStateIndex values (navigation/NavStateIndex.h:34-41):
+ * a uint32 index into the NavIndex table, 0 = outside the world; the "last
+ * exited" state is a second uint32 and the boundary flag a uint8.
+ * ------------------------------------------------------------------------ */
+typedef struct vgmi_geom_desc {
+  int nshapes;
+  const int *shape_type;         /* vgmi_shape_type per shape */
+  const int *shape_param_offset; /* nshapes + 1 offsets into shape_params */
+  const double *shape_params;
+  int nlogical;
+  const int *logical_shape;      /* shape index per logical volume */
+  int nplaced;
+  const int *placed_mother;      /* mother logical volume per placement */
+  const int *placed_logical;     /* placed logical volume per placement */
+  const double *placed_transform;/* 12 doubles per placement: tx ty tz r0..r8 (row major) */
+  int world_logical;
+} vgmi_geom_desc;
+
+typedef struct vgmi_geom vgmi_geom;
+
+#define VGMI_GEOM_HOST_ONLY 1 /* build the host tables only (no device upload) */
+
+VGMI_API vgmi_status vgmi_geom_build(const vgmi_geom_desc *desc, int flags, vgmi_geom **out);
+VGMI_API vgmi_status vgmi_geom_destroy(vgmi_geom *geom);
+/* counts[0..5] = placed volumes, logical volumes, NavIndex words, max depth, world placed id, device bytes */
+VGMI_API vgmi_status vgmi_geom_info(const vgmi_geom *geom, int64_t counts[6]);
+/* Host copy of the NavIndex table (uint32 words); returns the word count in *nwords. */
+VGMI_API vgmi_status vgmi_geom_navindex(const vgmi_geom *geom, uint32_t *out, int64_t cap, int64_t *nwords);
+/* BVH arrays of one logical volume (counts = depth, nodes, daughters); any output may be NULL. */
+VGMI_API vgmi_status vgmi_geom_bvh(const vgmi_geom *geom, int logical, int counts[3], int *nchild, int *offset,
+                                   int *primid, double *nodes6, double *aabbs6);
+
+/*
+ * Locate global points from the world down (GlobalLocator::LocateGlobalPoint(world, p,
+ * state, true), navigation/GlobalLocator.h:35-82, with BVHLevelLocator at every level,
+ * navigation/BVHLevelLocator.h:44-95). nav_out[i] = 0 for points outside the world.
+ */
+VGMI_API vgmi_status vgmi_nav_locate(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                                     const double *pz, uint32_t *nav_out, size_t n, void *stream);
+
+/*
+ * One navigation step per track: VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState
+ * (relocate = 1) or ComputeStep (relocate = 0) (navigation/VNavigator.h:436-540).
+ * Inputs are global points/directions, per-track step limits and in-states;
+ * outputs are the step, the out-state (nav, last exited, boundary flag).
+ * last_in may be NULL (no last exited volume).
+ */
+VGMI_API vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                                   const double *pz, const double *dx, const double *dy, const double *dz,
+                                   const double *step_limit, const uint32_t *nav_in, const uint32_t *last_in,
+                                   uint32_t *nav_out, uint32_t *last_out, uint8_t *boundary_out, double *step_out,
+                                   size_t n, int relocate, void *stream);
+
+/* Isotropic safety: VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
+ * (navigation/VSafetyEstimator.h:98-107, navigation/BVHSafetyEstimator.h:38-46). */
+VGMI_API vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                                     const double *pz, const uint32_t *nav_in, double *safety_out, size_t n,
+                                     void *stream);
+
+/*
+ * Transport loop fused on the device: for each track, up to nsteps times
+ * { step = ComputeStepAndPropagatedState(p, d, limit, state); p += step * d; state = out },
+ * stopping when the track leaves the world. p, nav and last are updated in place;
+ * steps_taken (may be NULL) receives the per-track number of steps.
+ * No single reference counterpart: the loop a transport engine runs around
+ * VNavigator::ComputeStepAndPropagatedState.
+ */
+VGMI_API vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px, double *py, double *pz,
+                                        const double *dx, const double *dy, const double *dz, double step_limit,
+                                        uint32_t *nav, uint32_t *last, int32_t *steps_taken, size_t n, int nsteps,
+                                        void *stream);
+
 #ifdef __cplusplus
 }
 #endif

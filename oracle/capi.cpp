@@ -4,6 +4,7 @@
 // Shape parameter lists (same order as the reference constructors):
 //   0 box         dx dy dz                                   (volumes/UnplacedBox.h)
 //   1 tube        rmin rmax dz sphi dphi                     (volumes/UnplacedTube.h:29)
+//   5 trapezoid   dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2 (volumes/UnplacedTrapezoid.h:56)
 #include "shapes.h"
 #include <chrono>
 #include <thread>
@@ -19,6 +20,8 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 3 ? new Box(a[0], a[1], a[2]) : nullptr;
   case 1:
     return n >= 5 ? new Tube(a[0], a[1], a[2], a[3], a[4]) : nullptr;
+  case 5:
+    return n >= 11 ? new Trapezoid(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]) : nullptr;
   default:
     return nullptr;
   }

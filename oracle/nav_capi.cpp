@@ -1,0 +1,186 @@
+// Oracle navigation C ABI for the tests and the CPU-baseline leg of the benchmark.
+// TEST INFRASTRUCTURE ONLY (see common.h).
+//
+// Geometry description (shared with the product builder):
+//   shapes     : type[i], parameters params[off[i] .. off[i+1])
+//   logicals   : shape index per logical volume
+//   placements : mother logical, daughter logical, 12 doubles (tx ty tz r0..r8) each;
+//                placed-volume id k for placement k, the world placed volume gets id nplaced
+#include "nav.h"
+#include <thread>
+
+using namespace vgo;
+
+namespace {
+template <class F>
+void parallel_for(int64_t n, int threads, F f)
+{
+  if (threads <= 1) {
+    f(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  int64_t chunk = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    int64_t b = t * chunk, e = std::min<int64_t>(n, b + chunk);
+    if (b >= e) break;
+    pool.emplace_back(f, b, e);
+  }
+  for (auto &th : pool) th.join();
+}
+} // namespace
+
+extern "C" {
+
+void *vgo_geom_new(int nshapes, const int *stype, const int *soff, const double *sparams, int nlv, const int *lvshape,
+                   int npl, const int *plmother, const int *pllv, const double *pltr, int worldlv, int use_bvh)
+{
+  auto *g   = new Geometry();
+  g->useBVH = use_bvh != 0;
+  for (int i = 0; i < nshapes; ++i) {
+    Shape *s = MakeShape(stype[i], sparams + soff[i], soff[i + 1] - soff[i]);
+    if (!s) {
+      delete g;
+      return nullptr;
+    }
+    g->shapes.emplace_back(s);
+  }
+  g->lvs.resize(nlv);
+  for (int i = 0; i < nlv; ++i) g->lvs[i].shape = lvshape[i];
+  g->pvols.resize(npl + 1);
+  for (int k = 0; k < npl; ++k) {
+    PVol &p = g->pvols[k];
+    p.lv    = pllv[k];
+    for (int i = 0; i < 3; ++i) p.tr.t[i] = pltr[12 * k + i];
+    for (int i = 0; i < 9; ++i) p.tr.r[i] = pltr[12 * k + 3 + i];
+    p.tr.SetProperties();
+    LVol &mom = g->lvs[plmother[k]];
+    p.childId = int(mom.daughters.size());
+    mom.daughters.push_back(k);
+  }
+  g->pvols[npl].lv = worldlv;
+  g->worldPvol     = npl;
+  g->Close();
+  return g;
+}
+
+void vgo_geom_free(void *g) { delete (Geometry *)g; }
+
+int64_t vgo_geom_navindex(void *g, uint32_t *out, int64_t cap)
+{
+  auto &t = ((Geometry *)g)->navind;
+  if (out)
+    for (int64_t i = 0; i < std::min<int64_t>(cap, int64_t(t.size())); ++i) out[i] = t[i];
+  return int64_t(t.size());
+}
+
+int vgo_geom_maxdepth(void *g) { return ((Geometry *)g)->maxDepth; }
+
+// BVH arrays of one logical volume: counts[0]=depth, counts[1]=nodes, counts[2]=n daughters.
+int vgo_geom_bvh(void *gp, int lv, int *counts, int *nchild, int *offset, int *primid, double *nodes6, double *aabbs6)
+{
+  Geometry *g = (Geometry *)gp;
+  if (lv < 0 || lv >= int(g->lvs.size()) || !g->lvs[lv].bvh) return -1;
+  BVHo const &b = *g->lvs[lv].bvh;
+  counts[0]     = b.depth;
+  counts[1]     = int(b.nodes.size());
+  counts[2]     = int(b.aabbs.size());
+  if (nchild)
+    for (size_t i = 0; i < b.nodes.size(); ++i) {
+      nchild[i] = b.nChild[i];
+      offset[i] = b.offset[i];
+      for (int k = 0; k < 3; ++k) nodes6[6 * i + k] = b.nodes[i].lo[k], nodes6[6 * i + 3 + k] = b.nodes[i].hi[k];
+    }
+  if (primid)
+    for (size_t i = 0; i < b.aabbs.size(); ++i) {
+      primid[i] = b.primId[i];
+      for (int k = 0; k < 3; ++k) aabbs6[6 * i + k] = b.aabbs[i].lo[k], aabbs6[6 * i + 3 + k] = b.aabbs[i].hi[k];
+    }
+  return 0;
+}
+
+int vgo_nav_locate(void *gp, const double *px, const double *py, const double *pz, uint32_t *nav_out, int64_t n,
+                   int threads)
+{
+  Geometry *g = (Geometry *)gp;
+  parallel_for(n, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) nav_out[i] = g->LocateGlobalPoint(g->worldPvol, V3(px[i], py[i], pz[i]), 0, true);
+  });
+  return 0;
+}
+
+int vgo_nav_step(void *gp, const double *px, const double *py, const double *pz, const double *dx, const double *dy,
+                 const double *dz, const double *limit, const uint32_t *nav_in, const uint32_t *last_in,
+                 uint32_t *nav_out, uint32_t *last_out, uint8_t *boundary_out, double *step_out, int64_t n,
+                 int relocate, int threads)
+{
+  Geometry *g = (Geometry *)gp;
+  parallel_for(n, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      NavState in, out;
+      in.nav  = nav_in[i];
+      in.last = last_in ? last_in[i] : 0;
+      if (in.nav == 0) {
+        nav_out[i] = 0, step_out[i] = 0;
+        if (last_out) last_out[i] = in.last;
+        if (boundary_out) boundary_out[i] = 0;
+        continue;
+      }
+      step_out[i] = g->ComputeStep(V3(px[i], py[i], pz[i]), V3(dx[i], dy[i], dz[i]), limit[i], in, out, relocate);
+      nav_out[i]  = out.nav;
+      if (last_out) last_out[i] = out.last;
+      if (boundary_out) boundary_out[i] = out.boundary;
+    }
+  });
+  return 0;
+}
+
+int vgo_nav_safety(void *gp, const double *px, const double *py, const double *pz, const uint32_t *nav_in,
+                   double *out, int64_t n, int threads)
+{
+  Geometry *g = (Geometry *)gp;
+  parallel_for(n, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      NavState in;
+      in.nav = nav_in[i];
+      out[i] = in.nav ? g->ComputeSafety(V3(px[i], py[i], pz[i]), in) : 0.;
+    }
+  });
+  return 0;
+}
+
+// Transport loop: nsteps x (ComputeStepAndPropagatedState, p += step * d) per track; tracks whose
+// state leaves the world stop. Updates p, nav, last in place; returns the number of steps taken.
+int64_t vgo_nav_propagate(void *gp, double *px, double *py, double *pz, const double *dx, const double *dy,
+                          const double *dz, double limit, uint32_t *nav, uint32_t *last, int64_t n, int nsteps,
+                          int threads)
+{
+  Geometry *g = (Geometry *)gp;
+  std::vector<int64_t> counts(std::max(1, threads), 0);
+  int64_t chunk = (n + std::max(1, threads) - 1) / std::max(1, threads);
+  parallel_for(n, threads, [&](int64_t b, int64_t e) {
+    int64_t taken = 0;
+    for (int64_t i = b; i < e; ++i) {
+      NavState st;
+      st.nav  = nav[i];
+      st.last = last[i];
+      V3 p(px[i], py[i], pz[i]), d(dx[i], dy[i], dz[i]);
+      for (int s = 0; s < nsteps && st.nav != 0; ++s) {
+        NavState out;
+        double step = g->ComputeStep(p, d, limit, st, out, true);
+        p           = p + step * d;
+        st          = out;
+        ++taken;
+      }
+      px[i] = p.x, py[i] = p.y, pz[i] = p.z;
+      nav[i]  = st.nav;
+      last[i] = st.last;
+    }
+    counts[chunk > 0 ? b / chunk : 0] = taken;
+  });
+  int64_t tot = 0;
+  for (auto c : counts) tot += c;
+  return tot;
+}
+
+} // extern "C"

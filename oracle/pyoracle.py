@@ -82,3 +82,117 @@ class OracleShape:
         if rc != 0:
             raise RuntimeError("vgo_shape_query failed: %d" % rc)
         return out
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleGeometry:
+    """Oracle geometry + BVH navigator (use_bvh=False: brute-force daughter loops)."""
+
+    def __init__(self, desc, use_bvh=True):
+        L = lib()
+        if not getattr(L, "_geom_declared", False):
+            vp, i64, ci = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+            L.vgo_geom_new.restype = vp
+            L.vgo_geom_new.argtypes = [ci, vp, vp, vp, ci, vp, ci, vp, vp, vp, ci, ci]
+            L.vgo_geom_free.argtypes = [vp]
+            L.vgo_geom_navindex.restype = i64
+            L.vgo_geom_navindex.argtypes = [vp, vp, i64]
+            L.vgo_geom_maxdepth.restype = ci
+            L.vgo_geom_maxdepth.argtypes = [vp]
+            L.vgo_geom_bvh.restype = ci
+            L.vgo_geom_bvh.argtypes = [vp, ci] + [vp] * 6
+            L.vgo_nav_locate.restype = ci
+            L.vgo_nav_locate.argtypes = [vp] + [vp] * 4 + [i64, ci]
+            L.vgo_nav_step.restype = ci
+            L.vgo_nav_step.argtypes = [vp] + [vp] * 13 + [i64, ci, ci]
+            L.vgo_nav_safety.restype = ci
+            L.vgo_nav_safety.argtypes = [vp] + [vp] * 5 + [i64, ci]
+            L.vgo_nav_propagate.restype = i64
+            L.vgo_nav_propagate.argtypes = [vp] + [vp] * 6 + [ctypes.c_double, vp, vp, i64, ci, ci]
+            L._geom_declared = True
+        self.desc = desc
+        self._arrs = desc.arrays()
+        stype, soff, sparams, lvshape, plm, pll, plt = self._arrs
+        self._h = L.vgo_geom_new(len(stype), _np_ptr(stype), _np_ptr(soff), _np_ptr(sparams), len(lvshape),
+                                 _np_ptr(lvshape), len(plm), _np_ptr(plm), _np_ptr(pll), _np_ptr(plt), desc.world,
+                                 1 if use_bvh else 0)
+        if not self._h:
+            raise ValueError("oracle rejected the geometry description")
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _LIB is not None:
+            _LIB.vgo_geom_free(self._h)
+            self._h = None
+
+    def navindex(self):
+        n = lib().vgo_geom_navindex(self._h, None, 0)
+        out = np.zeros(n, dtype=np.uint32)
+        lib().vgo_geom_navindex(self._h, _np_ptr(out), n)
+        return out
+
+    def max_depth(self):
+        return lib().vgo_geom_maxdepth(self._h)
+
+    def bvh(self, lv):
+        counts = np.zeros(3, dtype=np.int32)
+        if lib().vgo_geom_bvh(self._h, lv, _np_ptr(counts), None, None, None, None, None) != 0:
+            return None
+        depth, nn, nd = (int(x) for x in counts)
+        nchild = np.zeros(nn, np.int32)
+        offset = np.zeros(nn, np.int32)
+        prim = np.zeros(nd, np.int32)
+        nodes = np.zeros(6 * nn)
+        aabbs = np.zeros(6 * nd)
+        lib().vgo_geom_bvh(self._h, lv, _np_ptr(counts), _np_ptr(nchild), _np_ptr(offset), _np_ptr(prim),
+                           _np_ptr(nodes), _np_ptr(aabbs))
+        return {"depth": depth, "nchild": nchild, "offset": offset, "primid": prim, "nodes": nodes.reshape(-1, 6),
+                "aabbs": aabbs.reshape(-1, 6)}
+
+    def locate(self, p, threads=1):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        out = np.zeros(n, np.uint32)
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        lib().vgo_nav_locate(self._h, _np_ptr(px), _np_ptr(py), _np_ptr(pz), _np_ptr(out), n, threads)
+        return out
+
+    def step(self, p, d, limit, nav, last=None, relocate=True, threads=1):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        d = np.ascontiguousarray(d, dtype=np.float64)
+        n = p.shape[1]
+        lim = np.ascontiguousarray(np.broadcast_to(np.asarray(limit, dtype=np.float64), (n,)))
+        nav = np.ascontiguousarray(nav, dtype=np.uint32)
+        last = np.zeros(n, np.uint32) if last is None else np.ascontiguousarray(last, dtype=np.uint32)
+        nav_out = np.zeros(n, np.uint32)
+        last_out = np.zeros(n, np.uint32)
+        bnd = np.zeros(n, np.uint8)
+        step = np.zeros(n)
+        comps = [np.ascontiguousarray(a[i]) for a in (p, d) for i in range(3)]
+        lib().vgo_nav_step(self._h, *[_np_ptr(c) for c in comps], _np_ptr(lim), _np_ptr(nav), _np_ptr(last),
+                           _np_ptr(nav_out), _np_ptr(last_out), _np_ptr(bnd), _np_ptr(step), n,
+                           1 if relocate else 0, threads)
+        return step, nav_out, last_out, bnd
+
+    def safety(self, p, nav, threads=1):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        nav = np.ascontiguousarray(nav, dtype=np.uint32)
+        out = np.zeros(n)
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        lib().vgo_nav_safety(self._h, _np_ptr(px), _np_ptr(py), _np_ptr(pz), _np_ptr(nav), _np_ptr(out), n, threads)
+        return out
+
+    def propagate(self, p, d, limit, nav, last, nsteps, threads=1):
+        """In-place transport loop; returns (p, nav, last, steps_taken)."""
+        p = np.array(p, dtype=np.float64, order="C", copy=True)
+        nav = np.array(nav, dtype=np.uint32, copy=True)
+        last = np.array(last, dtype=np.uint32, copy=True)
+        n = p.shape[1]
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        dx, dy, dz = (np.ascontiguousarray(d[i], dtype=np.float64) for i in range(3))
+        taken = lib().vgo_nav_propagate(self._h, _np_ptr(px), _np_ptr(py), _np_ptr(pz), _np_ptr(dx), _np_ptr(dy),
+                                        _np_ptr(dz), float(limit), _np_ptr(nav), _np_ptr(last), n, nsteps, threads)
+        return np.stack([px, py, pz]), nav, last, int(taken)

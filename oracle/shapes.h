@@ -83,3 +83,24 @@ public:
 Shape *MakeShape(int type, const double *params, int nparams);
 
 } // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Trapezoid
+// VecGeom/volumes/TrapezoidStruct.h, source/UnplacedTrapezoid.cpp:447-675 (planes from corners),
+// VecGeom/volumes/kernel/TrapezoidImplementation.h:98-400 with PlaneShell<4> (VecGeom/base/PlaneShell.h:35-270)
+class Trapezoid : public Shape {
+public:
+  double dz, theta, phi, dy1, dx1, dx2, tanAlpha1, dy2, dx3, dx4, tanAlpha2, tthetaCphi, tthetaSphi;
+  double A[4], B[4], C[4], D[4];
+  Trapezoid(double dz, double theta, double phi, double dy1, double dx1, double dx2, double alpha1, double dy2,
+            double dx3, double dx4, double alpha2);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+};
+} // namespace vgo

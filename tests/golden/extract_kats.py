@@ -269,9 +269,10 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
 
 
 SHAPES = {
-    # fixture name: (file, shape class pattern, number of ctor params)
-    "box": ("TestBox.cpp", r"Box_t"),
-    "tube": ("TestTube.cpp", r"Tube_t"),
+    # fixture name: (file, shape class pattern, accepted ctor parameter counts)
+    "box": ("TestBox.cpp", r"Box_t", (3,)),
+    "tube": ("TestTube.cpp", r"Tube_t", (5,)),
+    "trapezoid": ("TestTrap.cpp", r"Trap_t", (11,)),
 }
 
 if __name__ == "__main__":
@@ -279,8 +280,9 @@ if __name__ == "__main__":
     out_dir = os.path.dirname(os.path.abspath(__file__))
     only = sys.argv[2:] or list(SHAPES)
     for name in only:
-        fn, pat = SHAPES[name][:2]
+        fn, pat, counts = SHAPES[name]
         recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat)
+        recs = [r for r in recs if len(r["shape"]) in counts]
         with open(os.path.join(out_dir, "kat_%s.json" % name), "w") as f:
             json.dump({"source": "test/unit_tests/%s" % fn, "records": recs}, f, indent=0)
         print("%-12s %4d records" % (name, len(recs)))

@@ -8,7 +8,7 @@ import pytest
 from oracle.pyoracle import OracleShape
 from tests import kat_util
 
-SHAPES = ["box", "tube"]
+SHAPES = ["box", "tube", "trapezoid"]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

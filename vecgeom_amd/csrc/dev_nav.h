@@ -1,0 +1,415 @@
+// Device-side BVH navigation (FP64), one lane per track.
+//
+//  - NavStateIndex arithmetic and cached global matrices (VecGeom/navigation/NavStateIndex.h:186-265)
+//  - placed-volume transforms (VecGeom/base/Transformation3D.h:661-830, general case)
+//  - BVH traversal CheckDaughterIntersections / LevelLocate / ComputeSafety (VecGeom/base/BVH.h:96-285)
+//  - VNavigatorHelper step with PrepareOutState and Relocate (VecGeom/navigation/VNavigator.h:160-540,1066-1091)
+//  - GlobalLocator (VecGeom/navigation/GlobalLocator.h:35-200)
+//
+// The traversal stack lives in LDS, one column per lane (stride = block size), so
+// pushes/pops are bank-conflict free and no scratch memory is used.
+#pragma once
+#include "dev_box.h"
+#include "dev_trapezoid.h"
+#include "dev_tube.h"
+#include "geom_data.h"
+
+namespace vgmi {
+namespace nav {
+
+constexpr int kNavBlock = 128;
+constexpr int kStackDepth = 34; // BVH depth <= 32 -> at most 33 pending nodes
+
+using V3 = Vec3<double>;
+
+struct Stack {
+  uint32_t *base; // LDS column of this lane
+  VGMI_DEV uint32_t &at(int i) { return base[i * kNavBlock]; }
+};
+
+struct Mat {
+  double t[3], r[9];
+};
+
+VGMI_DEV V3 rotate(const double *r, V3 const &m)
+{
+  double x = m.x * r[0], y = m.x * r[1], z = m.x * r[2];
+  x += m.y * r[3];
+  y += m.y * r[4];
+  z += m.y * r[5];
+  x += m.z * r[6];
+  y += m.z * r[7];
+  z += m.z * r[8];
+  return V3(x, y, z);
+}
+
+VGMI_DEV V3 to_local(const double *t, const double *r, V3 const &m)
+{
+  return rotate(r, V3(m.x - t[0], m.y - t[1], m.z - t[2]));
+}
+
+VGMI_DEV V3 to_master(const double *t, const double *r, V3 const &l)
+{
+  double x = t[0], y = t[1], z = t[2];
+  x += l.x * r[0];
+  x += l.y * r[1];
+  x += l.z * r[2];
+  y += l.x * r[3];
+  y += l.y * r[4];
+  y += l.z * r[5];
+  z += l.x * r[6];
+  z += l.y * r[7];
+  z += l.z * r[8];
+  return V3(x, y, z);
+}
+
+VGMI_DEV int top(GeomView const &g, uint32_t nav) { return nav > 0 ? int(g.navind[nav + 1]) : -1; }
+VGMI_DEV uint32_t pop(GeomView const &g, uint32_t nav) { return nav > 0 ? g.navind[nav] : 0u; }
+VGMI_DEV uint32_t push(GeomView const &g, uint32_t nav, int pvol)
+{
+  return nav > 0 ? g.navind[nav + 3 + g.pvols[pvol].childId] : 1u;
+}
+
+// Global matrix of the state's top volume, from the cached table entry.
+VGMI_DEV void top_matrix(GeomView const &g, uint32_t nav, Mat &m)
+{
+  m.t[0] = m.t[1] = m.t[2] = 0.;
+  for (int i = 0; i < 9; ++i) m.r[i] = (i % 4 == 0) ? 1. : 0.;
+  if (nav == 0) return;
+  const uint32_t w    = g.navind[nav + 2];
+  const uint32_t hasm = (w >> 8) & 0xffu;
+  if ((hasm & 0x03u) == 0) return;
+  const uint32_t nd = w >> 16;
+  uint32_t ti       = nav + 3 + nd + ((nd + 1) & 1);
+  ti += ti & 1u; // 8-byte alignment of the matrix block
+  const double *a = reinterpret_cast<const double *>(g.navind + ti);
+  if (hasm & 0x02u)
+    for (int i = 0; i < 3; ++i) m.t[i] = a[i];
+  if (hasm & 0x01u)
+    for (int i = 0; i < 9; ++i) m.r[i] = a[3 + i];
+}
+
+// ---- shape dispatch on a device shape record
+VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d)
+{
+  switch (s.type) {
+  case kBox:
+    return box::DistanceToIn(s.u.box, p, d);
+  case kTube:
+    return tube::DistanceToIn(s.u.tube, p, d);
+  case kTrapezoid:
+    return trap::DistanceToIn(s.u.trap, p, d);
+  default:
+    return kInfLength;
+  }
+}
+VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d)
+{
+  switch (s.type) {
+  case kBox:
+    return box::DistanceToOut(s.u.box, p, d);
+  case kTube:
+    return tube::DistanceToOut(s.u.tube, p, d);
+  case kTrapezoid:
+    return trap::DistanceToOut(s.u.trap, p, d);
+  default:
+    return -1.;
+  }
+}
+VGMI_DEV bool contains(ShapeRecord const &s, V3 const &p)
+{
+  switch (s.type) {
+  case kBox:
+    return box::Contains(s.u.box, p);
+  case kTube:
+    return tube::Contains(s.u.tube, p);
+  case kTrapezoid:
+    return trap::Contains(s.u.trap, p);
+  default:
+    return false;
+  }
+}
+VGMI_DEV double safety_in(ShapeRecord const &s, V3 const &p)
+{
+  switch (s.type) {
+  case kBox:
+    return box::SafetyToIn(s.u.box, p);
+  case kTube:
+    return tube::SafetyToIn(s.u.tube, p);
+  case kTrapezoid:
+    return trap::SafetyToIn(s.u.trap, p);
+  default:
+    return kInfLength;
+  }
+}
+VGMI_DEV double safety_out(ShapeRecord const &s, V3 const &p)
+{
+  switch (s.type) {
+  case kBox:
+    return box::SafetyToOut(s.u.box, p);
+  case kTube:
+    return tube::SafetyToOut(s.u.tube, p);
+  case kTrapezoid:
+    return trap::SafetyToOut(s.u.trap, p);
+  default:
+    return 0.;
+  }
+}
+
+VGMI_DEV ShapeRecord const &shape_of_pvol(GeomView const &g, int pvol) { return g.shapes[g.lvs[g.pvols[pvol].lv].shape]; }
+
+// AABB::ComputeIntersectionInvDir (VecGeom/base/AABB.h:136-154)
+VGMI_DEV void slab(AABBRec const &b, V3 const &p, V3 const &inv, double &tmin, double &tmax)
+{
+  double a0 = (b.lo[0] - p.x) * inv.x, a1 = (b.lo[1] - p.y) * inv.y, a2 = (b.lo[2] - p.z) * inv.z;
+  double b0 = (b.hi[0] - p.x) * inv.x, b1 = (b.hi[1] - p.y) * inv.y, b2 = (b.hi[2] - p.z) * inv.z;
+  if (a0 > b0) {
+    double t = a0;
+    a0       = b0;
+    b0       = t;
+  }
+  if (a1 > b1) {
+    double t = a1;
+    a1       = b1;
+    b1       = t;
+  }
+  if (a2 > b2) {
+    double t = a2;
+    a2       = b2;
+    b2       = t;
+  }
+  tmin = vmax(a0, vmax(a1, a2));
+  tmax = vmin(b0, vmin(b1, b2)) * (1. + 2 * DBL_EPSILON);
+}
+
+VGMI_DEV bool box_contains(AABBRec const &b, V3 const &p)
+{
+  return p.x >= b.lo[0] && p.x <= b.hi[0] && p.y >= b.lo[1] && p.y <= b.hi[1] && p.z >= b.lo[2] && p.z <= b.hi[2];
+}
+
+VGMI_DEV double box_safety(AABBRec const &b, V3 const &p)
+{
+  double c0 = 0.5 * (b.hi[0] + b.lo[0]), c1 = 0.5 * (b.hi[1] + b.lo[1]), c2 = 0.5 * (b.hi[2] + b.lo[2]);
+  double s0 = vabs(p.x - c0) - 0.5 * (b.hi[0] - b.lo[0]);
+  double s1 = vabs(p.y - c1) - 0.5 * (b.hi[1] - b.lo[1]);
+  double s2 = vabs(p.z - c2) - 0.5 * (b.hi[2] - b.lo[2]);
+  return vmax(s0, vmax(s1, s2));
+}
+
+// BVH::CheckDaughterIntersections
+VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V3 const &ld, double &step, int last,
+                              int &hit, Stack stk)
+{
+  const V3 inv(1.0 / NonZero(ld.x), 1.0 / NonZero(ld.y), 1.0 / NonZero(ld.z));
+  int sp       = 1;
+  stk.at(0)    = 0;
+  do {
+    const uint32_t id = stk.at(--sp);
+    const int nc      = g.nchild[L.nodeBase + id];
+    if (nc >= 0) {
+      const int off = g.offset[L.nodeBase + id];
+      for (int i = 0; i < nc; ++i) {
+        const int prim = g.prim[L.primBase + off + i];
+        double tmin, tmax;
+        slab(g.dAABB[L.dOffset + prim], lp, inv, tmin, tmax);
+        if (tmin <= tmax && tmax >= 0.0 && tmin < step) {
+          const int pv      = g.daughters[L.dOffset + prim];
+          PVolRec const &P  = g.pvols[pv];
+          const double dist = dist_in(shape_of_pvol(g, pv), to_local(P.t, P.r, lp), rotate(P.r, ld));
+          if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
+        }
+      }
+    } else {
+      const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      double tminL, tmaxL, tminR, tmaxR;
+      slab(g.nodes[L.nodeBase + l], lp, inv, tminL, tmaxL);
+      slab(g.nodes[L.nodeBase + r], lp, inv, tminR, tmaxR);
+      const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step;
+      const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step;
+      if (tminR < tminL) {
+        if (tr) stk.at(sp++) = r;
+        if (tl) stk.at(sp++) = l;
+      } else {
+        if (tl) stk.at(sp++) = l;
+        if (tr) stk.at(sp++) = r;
+      }
+    }
+  } while (sp > 0);
+}
+
+// BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint)
+VGMI_DEV bool level_locate(GeomView const &g, LVRec const &L, int excl, V3 const &lp, int &pvol, V3 &dlp, Stack stk)
+{
+  int sp    = 1;
+  stk.at(0) = 0;
+  do {
+    const uint32_t id = stk.at(--sp);
+    const int nc      = g.nchild[L.nodeBase + id];
+    if (nc >= 0) {
+      const int off = g.offset[L.nodeBase + id];
+      for (int i = 0; i < nc; ++i) {
+        const int prim = g.prim[L.primBase + off + i];
+        if (box_contains(g.dAABB[L.dOffset + prim], lp)) {
+          const int pv = g.daughters[L.dOffset + prim];
+          if (pv == excl) continue;
+          PVolRec const &P = g.pvols[pv];
+          const V3 loc     = to_local(P.t, P.r, lp);
+          if (contains(shape_of_pvol(g, pv), loc)) {
+            pvol = pv;
+            dlp  = loc;
+            return true;
+          }
+        }
+      }
+    } else {
+      const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      if (box_contains(g.nodes[L.nodeBase + l], lp)) stk.at(sp++) = l;
+      if (box_contains(g.nodes[L.nodeBase + r], lp)) stk.at(sp++) = r;
+    }
+  } while (sp > 0);
+  return false;
+}
+
+// BVH::ComputeSafety
+VGMI_DEV double safety_to_daughters(GeomView const &g, LVRec const &L, V3 const &lp, double safety, Stack stk)
+{
+  int sp    = 1;
+  stk.at(0) = 0;
+  do {
+    const uint32_t id = stk.at(--sp);
+    const int nc      = g.nchild[L.nodeBase + id];
+    if (nc >= 0) {
+      const int off = g.offset[L.nodeBase + id];
+      for (int i = 0; i < nc; ++i) {
+        const int prim = g.prim[L.primBase + off + i];
+        if (box_safety(g.dAABB[L.dOffset + prim], lp) < safety) {
+          const int pv     = g.daughters[L.dOffset + prim];
+          PVolRec const &P = g.pvols[pv];
+          const double s   = safety_in(shape_of_pvol(g, pv), to_local(P.t, P.r, lp));
+          if (s < safety) safety = s;
+        }
+      }
+    } else {
+      const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      const double sl = box_safety(g.nodes[L.nodeBase + l], lp), sr = box_safety(g.nodes[L.nodeBase + r], lp);
+      const bool tl = sl < safety, tr = sr < safety;
+      if (sr < sl) {
+        if (tr) stk.at(sp++) = r;
+        if (tl) stk.at(sp++) = l;
+      } else {
+        if (tl) stk.at(sp++) = l;
+        if (tr) stk.at(sp++) = r;
+      }
+    }
+  } while (sp > 0);
+  return safety;
+}
+
+// GlobalLocator::LocateGlobalPoint (top = false) / LocateGlobalPointExclVolume (excl >= 0)
+VGMI_DEV uint32_t locate_down(GeomView const &g, int pvol, int excl, V3 p, uint32_t path, Stack stk)
+{
+  path              = push(g, path, pvol);
+  LVRec const *L    = &g.lvs[g.pvols[pvol].lv];
+  bool deeper       = true;
+  while (L->ndaughters > 0 && deeper) {
+    int next = -1;
+    V3 dlp;
+    deeper = level_locate(g, *L, excl, p, next, dlp, stk);
+    if (deeper) {
+      path = push(g, path, next);
+      L    = &g.lvs[g.pvols[next].lv];
+      p    = dlp;
+    }
+  }
+  return path;
+}
+
+VGMI_DEV uint32_t locate_global(GeomView const &g, V3 const &p, Stack stk)
+{
+  if (!contains(shape_of_pvol(g, g.worldPvol), p)) return 0;
+  return locate_down(g, g.worldPvol, -1, p, 0, stk);
+}
+
+// GlobalLocator::RelocatePointFromPathForceDifferent
+VGMI_DEV uint32_t relocate_force_different(GeomView const &g, V3 const &lp, uint32_t path, Stack stk)
+{
+  int mother      = top(g, path);
+  const int entry = mother;
+  if (mother < 0) return path;
+  V3 tmp = lp;
+  while (mother >= 0) {
+    if (mother == entry || !contains(shape_of_pvol(g, mother), tmp)) {
+      path             = pop(g, path);
+      PVolRec const &P = g.pvols[mother];
+      tmp              = to_master(P.t, P.r, tmp);
+      mother           = top(g, path);
+    } else {
+      break;
+    }
+  }
+  if (mother >= 0) {
+    path = pop(g, path);
+    return locate_down(g, mother, entry, tmp, path, stk);
+  }
+  return path;
+}
+
+struct State {
+  uint32_t nav, last;
+  bool boundary;
+};
+
+// VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState / ComputeStep
+VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, double limit, State const &in,
+                             State &out, bool relocate, Stack stk)
+{
+  Mat m;
+  top_matrix(g, in.nav, m);
+  const V3 lp       = to_local(m.t, m.r, gp);
+  const V3 ld       = rotate(m.r, gd);
+  const int pvol    = top(g, in.nav);
+  LVRec const &L    = g.lvs[g.pvols[pvol].lv];
+  double step       = dist_out(g.shapes[L.shape], lp, ld);
+  if (step < 0.) step = 0.;
+  int hit = -1;
+  if (L.ndaughters > 0) check_daughters(g, L, lp, ld, step, top(g, in.last), hit, stk);
+  out = in;
+  if (step == kInfLength && limit > 0.) {
+    out.boundary = true;
+    out.nav      = pop(g, out.nav);
+    return kTolerance;
+  }
+  if (step > limit) {
+    out.boundary = false;
+    return limit;
+  }
+  out.boundary = true;
+  out.last     = out.nav;
+  if (hit >= 0) out.nav = push(g, out.nav, hit);
+  if (step < 0.) step = 0.;
+  if (!relocate) return step;
+  const V3 after = lp + (step + 1E-6) * ld;
+  if (top(g, out.nav) == pvol) {
+    out.nav = relocate_force_different(g, after, out.nav, stk);
+  } else {
+    PVolRec const &N = g.pvols[hit];
+    out.nav          = pop(g, out.nav);
+    out.nav          = locate_down(g, hit, -1, to_local(N.t, N.r, after), out.nav, stk);
+  }
+  return step;
+}
+
+// VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
+VGMI_DEV double compute_safety(GeomView const &g, V3 const &gp, uint32_t nav, Stack stk)
+{
+  Mat m;
+  top_matrix(g, nav, m);
+  const V3 lp    = to_local(m.t, m.r, gp);
+  LVRec const &L = g.lvs[g.pvols[top(g, nav)].lv];
+  double s       = safety_out(g.shapes[L.shape], lp);
+  if (s > 0.0 && L.ndaughters > 0) s = safety_to_daughters(g, L, lp, s, stk);
+  return s;
+}
+
+} // namespace nav
+} // namespace vgmi

@@ -56,6 +56,14 @@ struct TubeData {
   int32_t pad_;
 };
 
+// Trapezoid (VecGeom/volumes/TrapezoidStruct.h): half length in z and the four
+// side planes a*x + b*y + c*z + d = 0 with outward unit normals (PlaneShell<4>).
+struct TrapezoidData {
+  double dz;
+  double a[4], b[4], c[4], d[4];
+  double params[11]; // defining parameters, alphas already converted to tan(alpha)
+};
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -65,6 +73,7 @@ struct ShapeRecord {
   union {
     BoxData box;
     TubeData tube;
+    TrapezoidData trap;
     double raw[64];
   } u;
 };

@@ -62,6 +62,84 @@ int MakeTube(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
+// Trapezoid: volumes/UnplacedTrapezoid.h:56-63 (tan(alpha)), TrapezoidStruct::CalculateCached,
+// corners source/UnplacedTrapezoid.cpp:447-465, side planes from corner diagonals :575-619
+// (outward unit normal (p3-p1)x(p4-p2), offset through the corner centroid).
+int MakeTrapezoid(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 11) return err = "trapezoid needs 11 parameters (dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2)", -1;
+  if (!(a[0] > 0 && a[3] > 0 && a[4] > 0 && a[5] > 0 && a[7] > 0 && a[8] > 0 && a[9] > 0))
+    return err = "trapezoid half-lengths must be positive", -1;
+  r                = ShapeRecord();
+  r.type           = kTrapezoid;
+  TrapezoidData &t = r.u.trap;
+  const double dz = a[0], theta = a[1], phi = a[2], dy1 = a[3], dx1 = a[4], dx2 = a[5];
+  const double ta1 = std::tan(a[6]), dy2 = a[7], dx3 = a[8], dx4 = a[9], ta2 = std::tan(a[10]);
+  const double tthetaCphi = std::tan(theta) * std::cos(phi);
+  const double tthetaSphi = std::tan(theta) * std::sin(phi);
+  t.dz = dz;
+  double prm[11] = {dz, theta, phi, dy1, dx1, dx2, ta1, dy2, dx3, dx4, ta2};
+  for (int i = 0; i < 11; ++i) t.params[i] = prm[i];
+  struct P3 {
+    double x, y, z;
+  };
+  const double dxdyDy1 = ta1 * dy1, dxdyDy2 = ta2 * dy2, dxdzDz = tthetaCphi * dz, dydzDz = tthetaSphi * dz;
+  P3 c[8] = {{-dxdzDz - dxdyDy1 - dx1, -dydzDz - dy1, -dz}, {-dxdzDz - dxdyDy1 + dx1, -dydzDz - dy1, -dz},
+             {-dxdzDz + dxdyDy1 - dx2, -dydzDz + dy1, -dz}, {-dxdzDz + dxdyDy1 + dx2, -dydzDz + dy1, -dz},
+             {+dxdzDz - dxdyDy2 - dx3, +dydzDz - dy2, +dz}, {+dxdzDz - dxdyDy2 + dx3, +dydzDz - dy2, +dz},
+             {+dxdzDz + dxdyDy2 - dx4, +dydzDz + dy2, +dz}, {+dxdzDz + dxdyDy2 + dx4, +dydzDz + dy2, +dz}};
+  auto plane = [&](int i, int i1, int i2, int i3, int i4) {
+    P3 const &p1 = c[i1], &p2 = c[i2], &p3 = c[i3], &p4 = c[i4];
+    double ux = p3.x - p1.x, uy = p3.y - p1.y, uz = p3.z - p1.z;
+    double vx = p4.x - p2.x, vy = p4.y - p2.y, vz = p4.z - p2.z;
+    double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+    double inv = 1. / std::sqrt(nx * nx + ny * ny + nz * nz);
+    nx *= inv;
+    ny *= inv;
+    nz *= inv;
+    double cx = 0.25 * (((p1.x + p2.x) + p3.x) + p4.x);
+    double cy = 0.25 * (((p1.y + p2.y) + p3.y) + p4.y);
+    double cz = 0.25 * (((p1.z + p2.z) + p3.z) + p4.z);
+    t.a[i] = nx;
+    t.b[i] = ny;
+    t.c[i] = nz;
+    t.d[i] = -(nx * cx + ny * cy + nz * cz);
+  };
+  plane(0, 0, 1, 5, 4); // ~ -y
+  plane(1, 2, 6, 7, 3); // ~ +y
+  plane(2, 0, 4, 6, 2); // ~ -x
+  plane(3, 1, 3, 7, 5); // ~ +x
+  return 0;
+}
+
+// Extent from the corners recomputed from the planes (source/UnplacedTrapezoid.cpp:298-337, :467-494).
+static void TrapezoidExtent(TrapezoidData const &t, double lo[3], double hi[3])
+{
+  const double dz = t.dz;
+  const double *A = t.a, *B = t.b, *C = t.c, *D = t.d;
+  double y[8], x[8];
+  y[0] = y[1] = -(D[0] - dz * C[0]) / B[0];
+  y[2] = y[3] = -(D[1] - dz * C[1]) / B[1];
+  y[4] = y[5] = -(D[0] + dz * C[0]) / B[0];
+  y[6] = y[7] = -(D[1] + dz * C[1]) / B[1];
+  x[0] = ((B[2] / B[0]) * (D[0] - dz * C[0]) - (D[2] - dz * C[2])) / A[2];
+  x[1] = ((B[3] / B[0]) * (D[0] - dz * C[0]) - (D[3] - dz * C[3])) / A[3];
+  x[2] = ((B[2] / B[1]) * (D[1] - dz * C[1]) - (D[2] - dz * C[2])) / A[2];
+  x[3] = ((B[3] / B[1]) * (D[1] - dz * C[1]) - (D[3] - dz * C[3])) / A[3];
+  x[4] = ((B[2] / B[0]) * (D[0] + dz * C[0]) - (D[2] + dz * C[2])) / A[2];
+  x[5] = ((B[3] / B[0]) * (D[0] + dz * C[0]) - (D[3] + dz * C[3])) / A[3];
+  x[6] = ((B[2] / B[1]) * (D[1] + dz * C[1]) - (D[2] + dz * C[2])) / A[2];
+  x[7] = ((B[3] / B[1]) * (D[1] + dz * C[1]) - (D[3] + dz * C[3])) / A[3];
+  auto mx = [](double a, double b) { return a < b ? b : a; };
+  auto mn = [](double a, double b) { return b < a ? b : a; };
+  hi[0] = mx(mx(mx(x[0], x[1]), mx(x[2], x[3])), mx(mx(x[4], x[5]), mx(x[6], x[7])));
+  lo[0] = mn(mn(mn(x[0], x[1]), mn(x[2], x[3])), mn(mn(x[4], x[5]), mn(x[6], x[7])));
+  hi[1] = mx(mx(mx(y[0], y[1]), mx(y[2], y[3])), mx(mx(y[4], y[5]), mx(y[6], y[7])));
+  lo[1] = mn(mn(mn(y[0], y[1]), mn(y[2], y[3])), mn(mn(y[4], y[5]), mn(y[6], y[7])));
+  lo[2] = -dz;
+  hi[2] = dz;
+}
+
 int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::string &err)
 {
   switch (type) {
@@ -69,6 +147,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeBox(params, nparams, r, err);
   case kTube:
     return MakeTube(params, nparams, r, err);
+  case kTrapezoid:
+    return MakeTrapezoid(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -86,6 +166,9 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     hi[0] = hi[1] = r.u.tube.rmax;
     lo[2]         = -r.u.tube.z;
     hi[2]         = r.u.tube.z;
+    return;
+  case kTrapezoid:
+    TrapezoidExtent(r.u.trap, lo, hi);
     return;
   default:
     lo[0] = lo[1] = lo[2] = 0;

@@ -10,6 +10,7 @@
 #include "vgmi_internal.h"
 #include "dev_box.h"
 #include "dev_tube.h"
+#include "dev_trapezoid.h"
 
 namespace vgmi {
 
@@ -45,6 +46,17 @@ struct Shape<R, TubeData> {
   static VGMI_DEV R DistOut(TubeData const &s, Vec3<R> const &p, Vec3<R> const &d) { return tube::DistanceToOut(s, p, d); }
   static VGMI_DEV R SafeIn(TubeData const &s, Vec3<R> const &p) { return tube::SafetyToIn(s, p); }
   static VGMI_DEV R SafeOut(TubeData const &s, Vec3<R> const &p) { return tube::SafetyToOut(s, p); }
+};
+
+template <class R>
+struct Shape<R, TrapezoidData> {
+  using D = TrapezoidData;
+  static VGMI_DEV bool Contains(D const &s, Vec3<R> const &p) { return trap::Contains(s, p); }
+  static VGMI_DEV int Inside(D const &s, Vec3<R> const &p) { return trap::Inside(s, p); }
+  static VGMI_DEV R DistIn(D const &s, Vec3<R> const &p, Vec3<R> const &d) { return trap::DistanceToIn(s, p, d); }
+  static VGMI_DEV R DistOut(D const &s, Vec3<R> const &p, Vec3<R> const &d) { return trap::DistanceToOut(s, p, d); }
+  static VGMI_DEV R SafeIn(D const &s, Vec3<R> const &p) { return trap::SafetyToIn(s, p); }
+  static VGMI_DEV R SafeOut(D const &s, Vec3<R> const &p) { return trap::SafetyToOut(s, p); }
 };
 
 struct QueryArgs {
@@ -170,6 +182,9 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
   case kTube:
     e = launch_query(shape->rec.u.tube, op, a, st);
     break;
+  case kTrapezoid:
+    e = launch_query(shape->rec.u.trap, op, a, st);
+    break;
   default:
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query: shape type %d has no kernel", shape->rec.type);
   }
@@ -198,6 +213,9 @@ vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision, const d
     break;
   case kTube:
     hipLaunchKernelGGL((k_query_all_fp64<TubeData>), g, b, 0, st, shape->rec.u.tube, a);
+    break;
+  case kTrapezoid:
+    hipLaunchKernelGGL((k_query_all_fp64<TrapezoidData>), g, b, 0, st, shape->rec.u.trap, a);
     break;
   default:
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query_all: shape type %d has no kernel", shape->rec.type);

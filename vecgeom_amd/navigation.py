@@ -1,0 +1,141 @@
+"""Closed geometry on the device and the BVH navigator interface
+(VNavigator::ComputeStep / ComputeStepAndPropagatedState, VSafetyEstimator::ComputeSafety,
+GlobalLocator::LocateGlobalPoint; VecGeom/navigation/VNavigator.h:51-68,
+VecGeom/navigation/VSafetyEstimator.h:45,98, VecGeom/navigation/GlobalLocator.h:35-82).
+
+States are NavStateIndex values held in uint32 device tensors (0 = outside the world).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .shapes import FP64, _stream
+
+HOST_ONLY = 1
+
+
+def _declare():
+    L = lib()
+    if getattr(L, "_nav_declared", False):
+        return L
+    vp, i64, ci, sz = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_size_t
+    sigs = {
+        "vgmi_geom_build": [vp, ci, ctypes.POINTER(vp)],
+        "vgmi_geom_destroy": [vp],
+        "vgmi_geom_info": [vp, vp],
+        "vgmi_geom_navindex": [vp, vp, i64, vp],
+        "vgmi_geom_bvh": [vp, ci] + [vp] * 6,
+        "vgmi_nav_locate": [vp, ci] + [vp] * 4 + [sz, vp],
+        "vgmi_nav_step": [vp, ci] + [vp] * 13 + [sz, ci, vp],
+        "vgmi_nav_safety": [vp, ci] + [vp] * 5 + [sz, vp],
+        "vgmi_nav_propagate": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
+    }
+    for name, args in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    L._nav_declared = True
+    return L
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("nshapes", ctypes.c_int), ("shape_type", ctypes.c_void_p), ("shape_param_offset", ctypes.c_void_p),
+                ("shape_params", ctypes.c_void_p), ("nlogical", ctypes.c_int), ("logical_shape", ctypes.c_void_p),
+                ("nplaced", ctypes.c_int), ("placed_mother", ctypes.c_void_p), ("placed_logical", ctypes.c_void_p),
+                ("placed_transform", ctypes.c_void_p), ("world_logical", ctypes.c_int)]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Geometry:
+    """A closed geometry: host tables plus their device copy (unless host_only)."""
+
+    def __init__(self, desc, host_only=False):
+        L = _declare()
+        self.desc = desc
+        arrs = desc.arrays()
+        stype, soff, sparams, lvshape, plm, pll, plt = arrs
+        d = _Desc(len(stype), _p(stype), _p(soff), _p(sparams), len(lvshape), _p(lvshape), len(plm), _p(plm),
+                  _p(pll), _p(plt), desc.world)
+        h = ctypes.c_void_p()
+        check(L.vgmi_geom_build(ctypes.byref(d), HOST_ONLY if host_only else 0, ctypes.byref(h)), "vgmi_geom_build")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().vgmi_geom_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def info(self):
+        c = np.zeros(6, np.int64)
+        check(lib().vgmi_geom_info(self._h, _p(c)), "vgmi_geom_info")
+        return {"placed": int(c[0]), "logical": int(c[1]), "navindex_words": int(c[2]), "max_depth": int(c[3]),
+                "world_placed": int(c[4]), "device_bytes": int(c[5])}
+
+    def navindex(self):
+        n = ctypes.c_int64()
+        check(lib().vgmi_geom_navindex(self._h, None, 0, ctypes.byref(n)), "vgmi_geom_navindex")
+        out = np.zeros(n.value, np.uint32)
+        check(lib().vgmi_geom_navindex(self._h, _p(out), n.value, None), "vgmi_geom_navindex")
+        return out
+
+    def bvh(self, lv):
+        counts = np.zeros(3, np.int32)
+        if lib().vgmi_geom_bvh(self._h, lv, _p(counts), None, None, None, None, None) != 0:
+            return None
+        depth, nn, nd = (int(x) for x in counts)
+        nchild, offset, prim = np.zeros(nn, np.int32), np.zeros(nn, np.int32), np.zeros(nd, np.int32)
+        nodes, aabbs = np.zeros(6 * nn), np.zeros(6 * nd)
+        check(lib().vgmi_geom_bvh(self._h, lv, _p(counts), _p(nchild), _p(offset), _p(prim), _p(nodes), _p(aabbs)),
+              "vgmi_geom_bvh")
+        return {"depth": depth, "nchild": nchild, "offset": offset, "primid": prim, "nodes": nodes.reshape(-1, 6),
+                "aabbs": aabbs.reshape(-1, 6)}
+
+    # ---- device navigation
+    def LocateGlobalPoint(self, points, out=None, precision=FP64):
+        n = len(points)
+        if out is None:
+            out = torch.empty(n, dtype=torch.int32, device=points.x.device)
+        check(lib().vgmi_nav_locate(self._h, precision, *points.ptrs(), out.data_ptr(), n, _stream()),
+              "vgmi_nav_locate")
+        return out
+
+    def ComputeStep(self, points, dirs, step_limit, nav_in, last_in=None, relocate=True, precision=FP64):
+        """Returns (step, nav_out, last_out, boundary) device tensors."""
+        n = len(points)
+        dev = points.x.device
+        if not torch.is_tensor(step_limit):
+            step_limit = torch.full((n,), float(step_limit), dtype=torch.float64, device=dev)
+        step = torch.empty(n, dtype=torch.float64, device=dev)
+        nav_out = torch.empty(n, dtype=torch.int32, device=dev)
+        last_out = torch.empty(n, dtype=torch.int32, device=dev)
+        bnd = torch.empty(n, dtype=torch.uint8, device=dev)
+        check(lib().vgmi_nav_step(self._h, precision, *points.ptrs(), *dirs.ptrs(), step_limit.data_ptr(),
+                                  nav_in.data_ptr(), last_in.data_ptr() if last_in is not None else None,
+                                  nav_out.data_ptr(), last_out.data_ptr(), bnd.data_ptr(), step.data_ptr(), n,
+                                  1 if relocate else 0, _stream()), "vgmi_nav_step")
+        return step, nav_out, last_out, bnd
+
+    def ComputeSafety(self, points, nav, precision=FP64):
+        n = len(points)
+        out = torch.empty(n, dtype=torch.float64, device=points.x.device)
+        check(lib().vgmi_nav_safety(self._h, precision, *points.ptrs(), nav.data_ptr(), out.data_ptr(), n,
+                                    _stream()), "vgmi_nav_safety")
+        return out
+
+    def Propagate(self, points, dirs, step_limit, nav, last, nsteps, taken=None, precision=FP64):
+        """In-place transport loop on the device (points, nav, last are updated)."""
+        n = len(points)
+        check(lib().vgmi_nav_propagate(self._h, precision, *points.ptrs(), *dirs.ptrs(), float(step_limit),
+                                       nav.data_ptr(), last.data_ptr(),
+                                       taken.data_ptr() if taken is not None else None, n, int(nsteps), _stream()),
+              "vgmi_nav_propagate")
+        return taken
