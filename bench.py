@@ -37,6 +37,9 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tracks", type=int, default=1 << 20)
+    ap.add_argument("--nav-steps", type=int, default=50)
+    ap.add_argument("--no-nav", action="store_true")
     return ap.parse_args()
 
 
@@ -135,6 +138,79 @@ def run_shapes(args):
     }
 
 
+def run_nav(args):
+    """Calorimeter navigation: 2^20 tracks (default) from uniform world points with isotropic
+    directions, located once, then nsteps x ComputeStepAndPropagatedState (10 mm limit) with
+    the transport loop fused on the device. One step of the benchmark = the whole loop."""
+    import numpy as np
+    import torch
+
+    from vecgeom_amd import SOA3D, geometry
+    from vecgeom_amd.navigation import Geometry
+
+    g = geometry.calorimeter()
+    G = Geometry(g)
+    n, nsteps, limit = args.tracks, args.nav_steps, 10.0
+    p, d = geometry.world_points(g, n, seed=0x5EED0004)
+    P0, D = SOA3D.from_array(p), SOA3D.from_array(d)
+    nav0 = G.LocateGlobalPoint(P0)
+    torch.cuda.synchronize()
+    P = SOA3D.empty(n)
+    nav = torch.empty_like(nav0)
+    last = torch.zeros_like(nav0)
+    taken = torch.zeros(n, dtype=torch.int32, device="cuda")
+
+    def reset():
+        P.x.copy_(P0.x), P.y.copy_(P0.y), P.z.copy_(P0.z)
+        nav.copy_(nav0)
+        last.zero_()
+
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        reset()
+        G.Propagate(P, D, limit, nav, last, nsteps, taken)
+    torch.cuda.synchronize()
+    steps_per_run = int(taken.sum())
+    total_us = 0.0
+    for _ in range(args.steps):
+        reset()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        G.Propagate(P, D, limit, nav, last, nsteps, taken)
+        b.record(stream)
+        torch.cuda.synchronize()
+        total_us += a.elapsed_time(b) * 1e3
+    avg_us = total_us / args.steps
+    rate = steps_per_run / (avg_us * 1e-6)
+    # algorithmic HBM bytes of one fused launch: per track read p, d (48 B) and the state (8 B),
+    # write p (24 B) and the state (8 B)
+    bytes_launch = 88 * n
+    gbs = bytes_launch / (avg_us * 1e-6) / 1e9
+    res = {"nav_steps_per_s": round(rate, 1), "avg_launch_us": round(avg_us, 2), "steps_per_launch": steps_per_run,
+           "tracks": n, "nsteps": nsteps, "step_limit_mm": limit,
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_propagate",
+                        "bytes_per_launch": bytes_launch,
+                        "note": "latency/divergence-bound: state stays in registers for all steps"}}
+    if not args.no_cpu:
+        from oracle.pyoracle import OracleGeometry
+        O = OracleGeometry(g)
+        m = min(n, 1 << 15)
+        pn = nav0[:m].cpu().numpy().astype(np.uint32)
+        t0 = time.perf_counter()
+        done, reps = 0, 0
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            _, _, _, k = O.propagate(p[:, :m], d[:, :m], limit, pn, np.zeros_like(pn), nsteps, threads=1)
+            done += k
+            reps += 1
+        dt = time.perf_counter() - t0
+        model, ncpu = cpu_info()
+        res["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "nav-steps/s", "cores": 1, "kind": "port",
+                               "sample": "%d x (first %d tracks x %d steps) = %d nav-steps; host %s, %d logical CPUs"
+                                         % (reps, m, nsteps, done, model, ncpu)}
+    return res
+
+
 def cpu_baseline(shapes, budget_s):
     """Oracle (CPU scalar FP64 restatement) on a bounded sample of the same inputs, 1 thread."""
     import numpy as np
@@ -172,11 +248,12 @@ def main():
         sys.exit(1)
     torch.cuda.synchronize()
     res = run_shapes(args)
+    nav = None if args.no_nav else run_nav(args)
     line = {"metric": res["metric"], "value": res["value"], "unit": res["unit"], "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res["dtype"],
             "data": "synthetic", "config": res["config"], "roofline": res["roofline"],
-            "cpu_baseline": res["cpu_baseline"], "kernels": res["kernels"]}
+            "cpu_baseline": res["cpu_baseline"], "kernels": res["kernels"], "nav": nav}
     print(json.dumps(line), flush=True)
 
 

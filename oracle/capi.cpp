@@ -20,6 +20,10 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 3 ? new Box(a[0], a[1], a[2]) : nullptr;
   case 1:
     return n >= 5 ? new Tube(a[0], a[1], a[2], a[3], a[4]) : nullptr;
+  case 2:
+    return n >= 7 ? new Cone(a[0], a[1], a[2], a[3], a[4], a[5], a[6]) : nullptr;
+  case 8:
+    return n >= 1 ? new Orb(a[0]) : nullptr;
   case 5:
     return n >= 11 ? new Trapezoid(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]) : nullptr;
   default:

@@ -104,3 +104,54 @@ public:
   void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
 };
 } // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Orb
+// VecGeom/volumes/kernel/OrbImplementation.h:65-220
+class Orb : public Shape {
+public:
+  double r;
+  explicit Orb(double r_) : r(r_) {}
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override { lo = V3(-r, -r, -r), hi = V3(r, r, r); }
+};
+
+// ---------------------------------------------------------------- Cone (UniversalCone)
+// VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
+// ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820
+class Cone : public Shape {
+public:
+  double rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi;
+  double origRmax1, origRmax2;
+  double a1x, a1y, a2x, a2y;
+  Wedge wedge;
+  double innerSlope, outerSlope, innerOffset, outerOffset, innerTol, outerTol;
+  double sqRmin1, sqRmin2, sqRmax1, sqRmax2, tolIz, tolOz;
+  double innerApex, tanInnerApex, outerApex, tanOuterApex;
+  double secRMin, secRMax, invSecRMin, invSecRMax, tanRMin, tanRMax, zNormInner, zNormOuter;
+  Cone(double rmin1, double rmax1, double rmin2, double rmax2, double dz, double sphi, double dphi);
+  bool hasRmin() const { return rmin1 > 0 || rmin2 > 0; }
+  bool hasPhi() const { return dphi < kTwoPi; }
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  // helpers shared with the polycone
+  double RadiusAt(bool inner, double z) const;
+  bool InSector(double x, double y, bool includeSurface) const;
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  bool ConicalHit(bool toIn, bool inner, V3 const &p, V3 const &d, double &dist) const;
+  bool PhiPlaneHit(double ax, double ay, double nx, double ny, V3 const &p, V3 const &d, bool toIn,
+                   double &dist) const;
+  V3 Normal(bool inner, V3 const &p) const;
+  double SafeDistConical(bool inner, V3 const &p) const;
+};
+} // namespace vgo

@@ -44,15 +44,25 @@ class Unsupported(Exception):
 
 def _norm_expr(s):
     s = s.strip()
-    s = re.sub(r"\bstd::|\bvecgeom::|\bvecCore::math::|\bvecgeom::cxx::|\bcxx::", "", s)
+    s = re.sub(r"\bstd::|\bvecgeom::|\bvecCore::math::|\bvecgeom::cxx::|\bcxx::|\bVECGEOM_NAMESPACE::", "", s)
     s = re.sub(r"InfinityLength<\w+>\(\)", "kInfLength", s)
     s = re.sub(r"(\d)[fF]\b", r"\1", s)
     s = re.sub(r"\(\s*(?:double|Precision|float)\s*\)", "", s)
     return s
 
 
+CURRENT_VECS = {}
+
+
 def eval_scalar(expr, scalars):
     expr = _norm_expr(expr)
+    # vector components of known vectors: name.x() / name.y() / name.z()
+    def comp(m):
+        v = CURRENT_VECS.get(m.group(1))
+        if v is None:
+            raise Unsupported(m.group(0))
+        return repr(v["xyz".index(m.group(2))])
+    expr = re.sub(r"\b(\w+)\.([xyz])\(\)", comp, expr)
     try:
         tree = ast.parse(expr, mode="eval")
     except SyntaxError as e:
@@ -158,6 +168,8 @@ METHODS = "DistanceToIn|DistanceToOut|SafetyToIn|SafetyToOut|Inside|Contains"
 def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
     text = open(path).read()
     vecs, scalars, shapes, pending = {}, {}, {}, {}
+    global CURRENT_VECS
+    CURRENT_VECS = vecs
     records = []
     fname = os.path.relpath(path, REF_ROOT)
 
@@ -179,10 +191,16 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
         # shape definitions
         m = re.match(r"(?:%s)\s*\*?\s*(\w+)\s*(?:=\s*new\s+(?:%s)\s*)?\((.*)\)$" % (shape_t, shape_t), st)
         if m:
-            try:
-                shapes[m.group(1)] = shape_params(m.group(2))
-            except Unsupported:
-                shapes.pop(m.group(1), None)
+            # one or several declarations: T a(...), b(...)
+            decls = split_args(st[re.match(r"(?:%s)\s*\*?\s*" % shape_t, st).end():])
+            for decl in decls:
+                dm = re.fullmatch(r"\*?\s*(\w+)\s*(?:=\s*new\s+(?:%s)\s*)?\((.*)\)" % shape_t, decl.strip(), re.S)
+                if not dm:
+                    continue
+                try:
+                    shapes[dm.group(1)] = shape_params(dm.group(2))
+                except Unsupported:
+                    shapes.pop(dm.group(1), None)
             continue
         # scalar definitions
         m = re.match(r"(?:Precision|double)\s+(\w+)\s*=\s*([^=]+)$", st)
@@ -259,7 +277,23 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                 records.append({"line": "%s:%d" % (fname, line), "shape": shapes[sh], "method": meth,
                                 "p": p, "d": d, "expect": exp})
             continue
-        # anything that mutates a known vector or scalar makes it unknown
+        # plain re-assignment of a scalar or vector: name = expr
+        m = re.fullmatch(r"(\w+)\s*=\s*([^=].*)", st)
+        if m and m.group(1) not in shapes:
+            name, expr = m.groups()
+            pending.pop(name, None)
+            try:
+                scalars[name] = eval_scalar(expr, scalars)
+                vecs.pop(name, None)
+                continue
+            except Unsupported:
+                scalars.pop(name, None)
+            try:
+                vecs[name] = eval_vec(expr, vecs, scalars, vec_type)
+            except Unsupported:
+                vecs.pop(name, None)
+            continue
+        # anything else that mutates a known vector or scalar makes it unknown
         m = re.match(r"(\w+)\s*(?:\.\s*\w+\(\))?\s*(?:[-+*/]?=)", st)
         if m:
             vecs.pop(m.group(1), None)
@@ -273,6 +307,8 @@ SHAPES = {
     "box": ("TestBox.cpp", r"Box_t", (3,)),
     "tube": ("TestTube.cpp", r"Tube_t", (5,)),
     "trapezoid": ("TestTrap.cpp", r"Trap_t", (11,)),
+    "cone": ("TestCons.cpp", r"Cone_t", (7,)),
+    "orb": ("TestOrb.cpp", r"Orb_t", (1,)),
 }
 
 if __name__ == "__main__":

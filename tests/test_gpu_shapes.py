@@ -25,6 +25,13 @@ CASES = [
     ("tube", [50.0, 100.0, 50.0, 0.0, 1.5 * math.pi]),
     ("tube", [45.0, 50.0, 50.0, math.pi / 4, math.pi / 8]),
     ("tube", [0.0, 50.0, 50.0, math.pi, math.pi]),
+    ("trapezoid", [60, math.radians(20), math.radians(5), 40, 30, 40, math.radians(10), 16, 10, 14,
+                   math.radians(10)]),
+    ("trapezoid", [40, 0, 0, 30, 20, 20, 0, 30, 20, 20, 0]),
+    ("orb", [10.0]),
+    ("cone", [5.0, 10.0, 7.0, 15.0, 10.0, math.radians(20), math.radians(300)]),
+    ("cone", [0.0, 10.0, 0.0, 20.0, 15.0, 0.0, 2 * math.pi]),
+    ("cone", [20.0, 60.0, 80.0, 140.0, 100.0, math.radians(10), math.radians(300)]),
 ]
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 
@@ -60,7 +67,7 @@ def test_query_all_matches_single_methods():
         assert torch.equal(res[m], single), m
 
 
-@pytest.mark.parametrize("shape", ["box", "tube"])
+@pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

@@ -9,9 +9,7 @@
 // The traversal stack lives in LDS, one column per lane (stride = block size), so
 // pushes/pops are bank-conflict free and no scratch memory is used.
 #pragma once
-#include "dev_box.h"
-#include "dev_trapezoid.h"
-#include "dev_tube.h"
+#include "dev_shapes.h"
 #include "geom_data.h"
 
 namespace vgmi {
@@ -90,71 +88,11 @@ VGMI_DEV void top_matrix(GeomView const &g, uint32_t nav, Mat &m)
 }
 
 // ---- shape dispatch on a device shape record
-VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d)
-{
-  switch (s.type) {
-  case kBox:
-    return box::DistanceToIn(s.u.box, p, d);
-  case kTube:
-    return tube::DistanceToIn(s.u.tube, p, d);
-  case kTrapezoid:
-    return trap::DistanceToIn(s.u.trap, p, d);
-  default:
-    return kInfLength;
-  }
-}
-VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d)
-{
-  switch (s.type) {
-  case kBox:
-    return box::DistanceToOut(s.u.box, p, d);
-  case kTube:
-    return tube::DistanceToOut(s.u.tube, p, d);
-  case kTrapezoid:
-    return trap::DistanceToOut(s.u.trap, p, d);
-  default:
-    return -1.;
-  }
-}
-VGMI_DEV bool contains(ShapeRecord const &s, V3 const &p)
-{
-  switch (s.type) {
-  case kBox:
-    return box::Contains(s.u.box, p);
-  case kTube:
-    return tube::Contains(s.u.tube, p);
-  case kTrapezoid:
-    return trap::Contains(s.u.trap, p);
-  default:
-    return false;
-  }
-}
-VGMI_DEV double safety_in(ShapeRecord const &s, V3 const &p)
-{
-  switch (s.type) {
-  case kBox:
-    return box::SafetyToIn(s.u.box, p);
-  case kTube:
-    return tube::SafetyToIn(s.u.tube, p);
-  case kTrapezoid:
-    return trap::SafetyToIn(s.u.trap, p);
-  default:
-    return kInfLength;
-  }
-}
-VGMI_DEV double safety_out(ShapeRecord const &s, V3 const &p)
-{
-  switch (s.type) {
-  case kBox:
-    return box::SafetyToOut(s.u.box, p);
-  case kTube:
-    return tube::SafetyToOut(s.u.tube, p);
-  case kTrapezoid:
-    return trap::SafetyToOut(s.u.trap, p);
-  default:
-    return 0.;
-  }
-}
+VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_in(s, p, d); }
+VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_out(s, p, d); }
+VGMI_DEV bool contains(ShapeRecord const &s, V3 const &p) { return rec_contains(s, p); }
+VGMI_DEV double safety_in(ShapeRecord const &s, V3 const &p) { return rec_safety_in(s, p); }
+VGMI_DEV double safety_out(ShapeRecord const &s, V3 const &p) { return rec_safety_out(s, p); }
 
 VGMI_DEV ShapeRecord const &shape_of_pvol(GeomView const &g, int pvol) { return g.shapes[g.lvs[g.pvols[pvol].lv].shape]; }
 

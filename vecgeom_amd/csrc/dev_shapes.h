@@ -1,0 +1,130 @@
+// All device shape kernels, a per-record-type trait for statically dispatched
+// launches, and run-time dispatch over a ShapeRecord for navigation and mixed batches.
+#pragma once
+#include "dev_box.h"
+#include "dev_cone.h"
+#include "dev_orb.h"
+#include "dev_trapezoid.h"
+#include "dev_tube.h"
+
+// (enum value, record type, union member, kernel namespace)
+#define VGMI_FOR_EACH_SHAPE(X)                                                                                       \
+  X(kBox, BoxData, box, box)                                                                                         \
+  X(kTube, TubeData, tube, tube)                                                                                     \
+  X(kCone, ConeData, cone, cone)                                                                                     \
+  X(kTrapezoid, TrapezoidData, trap, trap)                                                                           \
+  X(kOrb, OrbData, orb, orb)
+
+namespace vgmi {
+
+template <class Data>
+struct ShapeOps;
+
+#define VGMI_SHAPE_OPS(E, T, M, NS)                                                                                   \
+  template <>                                                                                                        \
+  struct ShapeOps<T> {                                                                                               \
+    static constexpr int kType = E;                                                                                  \
+    static T const &get(ShapeRecord const &r) { return r.u.M; }                                                     \
+    template <class R>                                                                                               \
+    static VGMI_DEV bool Contains(T const &s, Vec3<R> const &p)                                                      \
+    {                                                                                                                \
+      return NS::Contains(s, p);                                                                                     \
+    }                                                                                                                \
+    template <class R>                                                                                               \
+    static VGMI_DEV int Inside(T const &s, Vec3<R> const &p)                                                         \
+    {                                                                                                                \
+      return NS::Inside(s, p);                                                                                       \
+    }                                                                                                                \
+    template <class R>                                                                                               \
+    static VGMI_DEV R DistIn(T const &s, Vec3<R> const &p, Vec3<R> const &d)                                         \
+    {                                                                                                                \
+      return NS::DistanceToIn(s, p, d);                                                                              \
+    }                                                                                                                \
+    template <class R>                                                                                               \
+    static VGMI_DEV R DistOut(T const &s, Vec3<R> const &p, Vec3<R> const &d)                                        \
+    {                                                                                                                \
+      return NS::DistanceToOut(s, p, d);                                                                             \
+    }                                                                                                                \
+    template <class R>                                                                                               \
+    static VGMI_DEV R SafeIn(T const &s, Vec3<R> const &p)                                                           \
+    {                                                                                                                \
+      return NS::SafetyToIn(s, p);                                                                                   \
+    }                                                                                                                \
+    template <class R>                                                                                               \
+    static VGMI_DEV R SafeOut(T const &s, Vec3<R> const &p)                                                          \
+    {                                                                                                                \
+      return NS::SafetyToOut(s, p);                                                                                  \
+    }                                                                                                                \
+  };
+VGMI_FOR_EACH_SHAPE(VGMI_SHAPE_OPS)
+#undef VGMI_SHAPE_OPS
+
+// ---- run-time dispatch on a record (FP64)
+VGMI_DEV double rec_dist_in(ShapeRecord const &s, Vec3<double> const &p, Vec3<double> const &d)
+{
+  switch (s.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    return NS::DistanceToIn(s.u.M, p, d);
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  }
+  return kInfLength;
+}
+VGMI_DEV double rec_dist_out(ShapeRecord const &s, Vec3<double> const &p, Vec3<double> const &d)
+{
+  switch (s.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    return NS::DistanceToOut(s.u.M, p, d);
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  }
+  return -1.;
+}
+VGMI_DEV bool rec_contains(ShapeRecord const &s, Vec3<double> const &p)
+{
+  switch (s.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    return NS::Contains(s.u.M, p);
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  }
+  return false;
+}
+VGMI_DEV int rec_inside(ShapeRecord const &s, Vec3<double> const &p)
+{
+  switch (s.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    return NS::Inside(s.u.M, p);
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  }
+  return kOutside;
+}
+VGMI_DEV double rec_safety_in(ShapeRecord const &s, Vec3<double> const &p)
+{
+  switch (s.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    return NS::SafetyToIn(s.u.M, p);
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  }
+  return kInfLength;
+}
+VGMI_DEV double rec_safety_out(ShapeRecord const &s, Vec3<double> const &p)
+{
+  switch (s.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    return NS::SafetyToOut(s.u.M, p);
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  }
+  return 0.;
+}
+
+} // namespace vgmi

@@ -64,6 +64,26 @@ struct TrapezoidData {
   double params[11]; // defining parameters, alphas already converted to tan(alpha)
 };
 
+// Orb (VecGeom/volumes/OrbStruct.h)
+struct OrbData {
+  double r;
+};
+
+// Cone (VecGeom/volumes/ConeStruct.h:15-140): defining parameters plus the cached
+// slopes, offsets, apex data and tolerances of CalculateCached().
+struct ConeData {
+  double rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi;
+  double origRmax1, origRmax2;
+  double a1x, a1y, a2x, a2y;             // phi-plane vectors of the normalised angles
+  double w1x, w1y, w2x, w2y;             // wedge along vectors (raw input angles)
+  double n1x, n1y, n2x, n2y;             // wedge inward normals (raw input angles)
+  double innerSlope, outerSlope, innerOffset, outerOffset, innerTol, outerTol;
+  double sqRmin1, sqRmin2, sqRmax1, sqRmax2, tolIz;
+  double innerApex, tanInnerApex, outerApex, tanOuterApex;
+  double secRMin, secRMax, invSecRMin, invSecRMax, tanRMin, tanRMax, zNormInner, zNormOuter;
+  int32_t hasRmin, hasPhi, phiLePi, wedgeLtPi;
+};
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -74,6 +94,8 @@ struct ShapeRecord {
     BoxData box;
     TubeData tube;
     TrapezoidData trap;
+    OrbData orb;
+    ConeData cone;
     double raw[64];
   } u;
 };

@@ -140,6 +140,103 @@ static void TrapezoidExtent(TrapezoidData const &t, double lo[3], double hi[3])
   hi[2] = dz;
 }
 
+// Orb: VecGeom/volumes/OrbStruct.h
+int MakeOrb(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 1 || !(a[0] > 0)) return err = "orb needs a positive radius", -1;
+  r          = ShapeRecord();
+  r.type     = kOrb;
+  r.u.orb.r  = a[0];
+  return 0;
+}
+
+// Cone: VecGeom/volumes/ConeStruct.h:415-426 (constructor), :309-346 (angle normalisation: the
+// wedge keeps the raw angles, the phi-plane vectors use the normalised ones) and :76-140 (cache).
+int MakeCone(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 7) return err = "cone needs 7 parameters (rmin1 rmax1 rmin2 rmax2 dz sphi dphi)", -1;
+  if (!(a[4] > 0 && a[6] > 0 && a[1] >= 0 && a[3] >= 0)) return err = "cone dz and dphi must be positive", -1;
+  const double kConeTol = 1e-7, kAngTol = 1e-9;
+  r           = ShapeRecord();
+  r.type      = kCone;
+  ConeData &c = r.u.cone;
+  c.rmin1 = a[0] < 0.0 ? 0.0 : a[0];
+  c.rmax1 = a[1];
+  c.rmin2 = a[2] < 0.0 ? 0.0 : a[2];
+  c.rmax2 = a[3];
+  c.dz    = a[4];
+  const double sphi0 = a[5], dphi0 = a[6];
+  c.w1x = std::cos(sphi0);
+  c.w1y = std::sin(sphi0);
+  c.w2x = std::cos(sphi0 + dphi0);
+  c.w2y = std::sin(sphi0 + dphi0);
+  c.n1x = -std::sin(sphi0);
+  c.n1y = std::cos(sphi0);
+  c.n2x = std::sin(sphi0 + dphi0);
+  c.n2y = -std::cos(sphi0 + dphi0);
+  c.dphi = dphi0;
+  c.sphi = sphi0;
+  if (dphi0 >= 2. * hPi - 0.5 * kAngTol) {
+    c.dphi = 2. * hPi;
+    c.sphi = 0;
+  }
+  c.sphi = sphi0 < 0 ? 2. * hPi - std::fmod(std::fabs(sphi0), 2. * hPi) : std::fmod(sphi0, 2. * hPi);
+  if (c.sphi + c.dphi > 2. * hPi) c.sphi -= 2. * hPi;
+  c.a1x = std::cos(c.sphi);
+  c.a1y = std::sin(c.sphi);
+  c.a2x = std::cos(c.sphi + c.dphi);
+  c.a2y = std::sin(c.sphi + c.dphi);
+  c.origRmax1 = c.rmax1;
+  c.origRmax2 = c.rmax2;
+  if (c.rmin1 == c.rmax1) c.rmax1 += kConeTol;
+  if (c.rmin2 == c.rmax2) c.rmax2 += kConeTol;
+  c.sqRmin1     = c.rmin1 * c.rmin1;
+  c.sqRmax1     = c.rmax1 * c.rmax1;
+  c.sqRmin2     = c.rmin2 * c.rmin2;
+  c.sqRmax2     = c.rmax2 * c.rmax2;
+  c.tanRMin     = (c.rmin2 - c.rmin1) * 0.5 / c.dz;
+  c.secRMin     = std::sqrt(1.0 + c.tanRMin * c.tanRMin);
+  c.invSecRMin  = 1. / (c.secRMin + std::copysign(1e-30, c.secRMin));
+  c.tanRMax     = (c.rmax2 - c.rmax1) * 0.5 / c.dz;
+  c.secRMax     = std::sqrt(1.0 + c.tanRMax * c.tanRMax);
+  c.invSecRMax  = 1. / (c.secRMax + std::copysign(1e-30, c.secRMax));
+  c.innerSlope  = -(c.rmin1 - c.rmin2) / (2. * c.dz);
+  c.outerSlope  = -(c.rmax1 - c.rmax2) / (2. * c.dz);
+  c.innerOffset = c.rmin2 - c.innerSlope * c.dz;
+  c.outerOffset = c.rmax2 - c.outerSlope * c.dz;
+  c.innerTol    = kConeTol * c.secRMin;
+  c.outerTol    = kConeTol * c.secRMax;
+  auto nz       = [](double x) { return x + std::copysign(1e-30, x); };
+  if (c.rmin2 > c.rmin1) {
+    c.innerApex    = 2 * c.dz * c.rmin1 / (c.rmin2 - c.rmin1);
+    c.tanInnerApex = c.rmin2 / (2 * c.dz + c.innerApex);
+  } else {
+    c.innerApex    = 2 * c.dz * c.rmin2 / nz(c.rmin1 - c.rmin2);
+    c.tanInnerApex = c.rmin1 / (2 * c.dz + c.innerApex);
+  }
+  if (c.rmin1 == 0. || c.rmin2 == 0.) c.innerApex = 0.;
+  if (c.rmin1 == 0.) c.tanInnerApex = c.rmin2 / (2 * c.dz);
+  if (c.rmin2 == 0.) c.tanInnerApex = c.rmin1 / (2 * c.dz);
+  if (c.rmax2 > c.rmax1) {
+    c.outerApex    = 2 * c.dz * c.rmax1 / (c.rmax2 - c.rmax1);
+    c.tanOuterApex = c.rmax2 / (2 * c.dz + c.outerApex);
+  } else {
+    c.outerApex    = 2 * c.dz * c.rmax2 / nz(c.rmax1 - c.rmax2);
+    c.tanOuterApex = c.rmax1 / (2 * c.dz + c.outerApex);
+  }
+  if (c.rmax1 == 0. || c.rmax2 == 0.) c.outerApex = 0.;
+  if (c.rmax1 == 0.) c.tanOuterApex = c.rmax2 / (2 * c.dz);
+  if (c.rmax2 == 0.) c.tanOuterApex = c.rmax1 / (2 * c.dz);
+  c.zNormInner = c.tanRMin / nz(c.secRMin);
+  c.zNormOuter = -c.tanRMax / nz(c.secRMax);
+  c.tolIz      = c.dz - hHTol;
+  c.hasRmin    = c.rmin1 > 0 || c.rmin2 > 0;
+  c.hasPhi     = c.dphi < 2. * hPi;
+  c.phiLePi    = c.dphi <= M_PI;
+  c.wedgeLtPi  = c.dphi < hPi;
+  return 0;
+}
+
 int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::string &err)
 {
   switch (type) {
@@ -149,6 +246,10 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeTube(params, nparams, r, err);
   case kTrapezoid:
     return MakeTrapezoid(params, nparams, r, err);
+  case kOrb:
+    return MakeOrb(params, nparams, r, err);
+  case kCone:
+    return MakeCone(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -170,6 +271,17 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
   case kTrapezoid:
     TrapezoidExtent(r.u.trap, lo, hi);
     return;
+  case kOrb:
+    for (int i = 0; i < 3; ++i) lo[i] = -r.u.orb.r, hi[i] = r.u.orb.r;
+    return;
+  case kCone: {
+    const double rm = std::max(r.u.cone.origRmax1, r.u.cone.origRmax2);
+    lo[0] = lo[1] = -rm;
+    hi[0] = hi[1] = rm;
+    lo[2] = -r.u.cone.dz;
+    hi[2] = r.u.cone.dz;
+    return;
+  }
   default:
     lo[0] = lo[1] = lo[2] = 0;
     hi[0] = hi[1] = hi[2] = 0;

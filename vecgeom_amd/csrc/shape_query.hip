@@ -8,9 +8,7 @@
 // runs the same kernel in FP32 and re-evaluates every point whose FP32 result
 // is within the near-surface band in FP64, so decisions stay exact.
 #include "vgmi_internal.h"
-#include "dev_box.h"
-#include "dev_tube.h"
-#include "dev_trapezoid.h"
+#include "dev_shapes.h"
 
 namespace vgmi {
 
@@ -24,41 +22,6 @@ VGMI_DEV Vec3<R> load3(const double *__restrict__ x, const double *__restrict__ 
                  R(__builtin_nontemporal_load(z + i)));
 }
 
-// Static dispatch of one method on one shape record type, arithmetic type R.
-template <class R, class Data>
-struct Shape;
-
-template <class R>
-struct Shape<R, BoxData> {
-  static VGMI_DEV bool Contains(BoxData const &s, Vec3<R> const &p) { return box::Contains(s, p); }
-  static VGMI_DEV int Inside(BoxData const &s, Vec3<R> const &p) { return box::Inside(s, p); }
-  static VGMI_DEV R DistIn(BoxData const &s, Vec3<R> const &p, Vec3<R> const &d) { return box::DistanceToIn(s, p, d); }
-  static VGMI_DEV R DistOut(BoxData const &s, Vec3<R> const &p, Vec3<R> const &d) { return box::DistanceToOut(s, p, d); }
-  static VGMI_DEV R SafeIn(BoxData const &s, Vec3<R> const &p) { return box::SafetyToIn(s, p); }
-  static VGMI_DEV R SafeOut(BoxData const &s, Vec3<R> const &p) { return box::SafetyToOut(s, p); }
-};
-
-template <class R>
-struct Shape<R, TubeData> {
-  static VGMI_DEV bool Contains(TubeData const &s, Vec3<R> const &p) { return tube::Contains(s, p); }
-  static VGMI_DEV int Inside(TubeData const &s, Vec3<R> const &p) { return tube::Inside(s, p); }
-  static VGMI_DEV R DistIn(TubeData const &s, Vec3<R> const &p, Vec3<R> const &d) { return tube::DistanceToIn(s, p, d); }
-  static VGMI_DEV R DistOut(TubeData const &s, Vec3<R> const &p, Vec3<R> const &d) { return tube::DistanceToOut(s, p, d); }
-  static VGMI_DEV R SafeIn(TubeData const &s, Vec3<R> const &p) { return tube::SafetyToIn(s, p); }
-  static VGMI_DEV R SafeOut(TubeData const &s, Vec3<R> const &p) { return tube::SafetyToOut(s, p); }
-};
-
-template <class R>
-struct Shape<R, TrapezoidData> {
-  using D = TrapezoidData;
-  static VGMI_DEV bool Contains(D const &s, Vec3<R> const &p) { return trap::Contains(s, p); }
-  static VGMI_DEV int Inside(D const &s, Vec3<R> const &p) { return trap::Inside(s, p); }
-  static VGMI_DEV R DistIn(D const &s, Vec3<R> const &p, Vec3<R> const &d) { return trap::DistanceToIn(s, p, d); }
-  static VGMI_DEV R DistOut(D const &s, Vec3<R> const &p, Vec3<R> const &d) { return trap::DistanceToOut(s, p, d); }
-  static VGMI_DEV R SafeIn(D const &s, Vec3<R> const &p) { return trap::SafetyToIn(s, p); }
-  static VGMI_DEV R SafeOut(D const &s, Vec3<R> const &p) { return trap::SafetyToOut(s, p); }
-};
-
 struct QueryArgs {
   const double *px, *py, *pz, *dx, *dy, *dz;
   void *out;
@@ -68,23 +31,23 @@ struct QueryArgs {
 template <class Data, int OP>
 __global__ __launch_bounds__(kBlock) void k_query_fp64(Data s, QueryArgs a)
 {
-  using S      = Shape<double, Data>;
+  using S      = ShapeOps<Data>;
   size_t i     = size_t(blockIdx.x) * kBlock + threadIdx.x;
   size_t strid = size_t(gridDim.x) * kBlock;
   for (; i < a.n; i += strid) {
     Vec3<double> p = load3<double>(a.px, a.py, a.pz, i);
     if constexpr (OP == kOpContains) {
-      __builtin_nontemporal_store(uint8_t(S::Contains(s, p)), (uint8_t *)a.out + i);
+      __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), (uint8_t *)a.out + i);
     } else if constexpr (OP == kOpInside) {
-      __builtin_nontemporal_store(int32_t(S::Inside(s, p)), (int32_t *)a.out + i);
+      __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), (int32_t *)a.out + i);
     } else if constexpr (OP == kOpDistIn || OP == kOpDistOut) {
       Vec3<double> d = load3<double>(a.dx, a.dy, a.dz, i);
-      double r       = OP == kOpDistIn ? S::DistIn(s, p, d) : S::DistOut(s, p, d);
+      double r       = OP == kOpDistIn ? S::template DistIn<double>(s, p, d) : S::template DistOut<double>(s, p, d);
       __builtin_nontemporal_store(r, (double *)a.out + i);
     } else if constexpr (OP == kOpSafetyIn) {
-      __builtin_nontemporal_store(S::SafeIn(s, p), (double *)a.out + i);
+      __builtin_nontemporal_store(S::template SafeIn<double>(s, p), (double *)a.out + i);
     } else {
-      __builtin_nontemporal_store(S::SafeOut(s, p), (double *)a.out + i);
+      __builtin_nontemporal_store(S::template SafeOut<double>(s, p), (double *)a.out + i);
     }
   }
 }
@@ -100,19 +63,19 @@ struct QueryAllArgs {
 template <class Data>
 __global__ __launch_bounds__(kBlock) void k_query_all_fp64(Data s, QueryAllArgs a)
 {
-  using S      = Shape<double, Data>;
+  using S      = ShapeOps<Data>;
   size_t i     = size_t(blockIdx.x) * kBlock + threadIdx.x;
   size_t strid = size_t(gridDim.x) * kBlock;
   for (; i < a.n; i += strid) {
     Vec3<double> p = load3<double>(a.px, a.py, a.pz, i);
-    if (a.contains) __builtin_nontemporal_store(uint8_t(S::Contains(s, p)), a.contains + i);
-    if (a.inside) __builtin_nontemporal_store(int32_t(S::Inside(s, p)), a.inside + i);
-    if (a.sin) __builtin_nontemporal_store(S::SafeIn(s, p), a.sin + i);
-    if (a.sout) __builtin_nontemporal_store(S::SafeOut(s, p), a.sout + i);
+    if (a.contains) __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), a.contains + i);
+    if (a.inside) __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), a.inside + i);
+    if (a.sin) __builtin_nontemporal_store(S::template SafeIn<double>(s, p), a.sin + i);
+    if (a.sout) __builtin_nontemporal_store(S::template SafeOut<double>(s, p), a.sout + i);
     if (a.din || a.dout) {
       Vec3<double> d = load3<double>(a.dx, a.dy, a.dz, i);
-      if (a.din) __builtin_nontemporal_store(S::DistIn(s, p, d), a.din + i);
-      if (a.dout) __builtin_nontemporal_store(S::DistOut(s, p, d), a.dout + i);
+      if (a.din) __builtin_nontemporal_store(S::template DistIn<double>(s, p, d), a.din + i);
+      if (a.dout) __builtin_nontemporal_store(S::template DistOut<double>(s, p, d), a.dout + i);
     }
   }
 }
@@ -176,15 +139,12 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   switch (shape->rec.type) {
-  case kBox:
-    e = launch_query(shape->rec.u.box, op, a, st);
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    e = launch_query(shape->rec.u.M, op, a, st);                                                                     \
     break;
-  case kTube:
-    e = launch_query(shape->rec.u.tube, op, a, st);
-    break;
-  case kTrapezoid:
-    e = launch_query(shape->rec.u.trap, op, a, st);
-    break;
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
   default:
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query: shape type %d has no kernel", shape->rec.type);
   }
@@ -208,15 +168,12 @@ vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision, const d
   hipStream_t st = (hipStream_t)stream;
   dim3 g(grid_for(n)), b(kBlock);
   switch (shape->rec.type) {
-  case kBox:
-    hipLaunchKernelGGL((k_query_all_fp64<BoxData>), g, b, 0, st, shape->rec.u.box, a);
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    hipLaunchKernelGGL((k_query_all_fp64<T>), g, b, 0, st, shape->rec.u.M, a);                                      \
     break;
-  case kTube:
-    hipLaunchKernelGGL((k_query_all_fp64<TubeData>), g, b, 0, st, shape->rec.u.tube, a);
-    break;
-  case kTrapezoid:
-    hipLaunchKernelGGL((k_query_all_fp64<TrapezoidData>), g, b, 0, st, shape->rec.u.trap, a);
-    break;
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
   default:
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query_all: shape type %d has no kernel", shape->rec.type);
   }
