@@ -24,6 +24,12 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 7 ? new Cone(a[0], a[1], a[2], a[3], a[4], a[5], a[6]) : nullptr;
   case 8:
     return n >= 1 ? new Orb(a[0]) : nullptr;
+  case 3: {
+    if (n < 3) return nullptr;
+    int nz = int(a[2]);
+    if (nz < 2 || n < 3 + 3 * nz) return nullptr;
+    return new Polycone(a[0], a[1], nz, a + 3, a + 3 + nz, a + 3 + 2 * nz);
+  }
   case 5:
     return n >= 11 ? new Trapezoid(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]) : nullptr;
   default:

@@ -155,3 +155,29 @@ public:
   double SafeDistConical(bool inner, V3 const &p) const;
 };
 } // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Polycone
+// VecGeom/volumes/PolyconeStruct.h:144-290 (sections = cones between consecutive distinct z planes,
+// shifted by their mid z), VecGeom/volumes/kernel/PolyconeImplementation.h:70-520
+class Polycone : public Shape {
+public:
+  double sphi, dphi;
+  std::vector<double> zs;        // section boundaries
+  std::vector<double> shift;     // per section
+  std::vector<Cone> secs;
+  double rmaxExtent = 0, zlo = 0, zhi = 0;
+  Polycone(double sphi, double dphi, int nz, const double *z, const double *rmin, const double *rmax);
+  int NSec() const { return int(secs.size()); }
+  int SectionIndex(double z) const;
+  void SectionInOut(int isec, V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+};
+} // namespace vgo

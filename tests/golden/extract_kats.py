@@ -167,7 +167,7 @@ METHODS = "DistanceToIn|DistanceToOut|SafetyToIn|SafetyToOut|Inside|Contains"
 
 def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
     text = open(path).read()
-    vecs, scalars, shapes, pending = {}, {}, {}, {}
+    vecs, scalars, shapes, pending, arrays = {}, {}, {}, {}, {}
     global CURRENT_VECS
     CURRENT_VECS = vecs
     records = []
@@ -180,8 +180,8 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
         vals = []
         for a in args:
             a = a.strip()
-            if fixed_arrays is not None and a in fixed_arrays:
-                vals.append(fixed_arrays[a])
+            if a in arrays:
+                vals.extend(arrays[a])
             else:
                 vals.append(eval_scalar(a, scalars))
         return vals
@@ -201,6 +201,22 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                     shapes[dm.group(1)] = shape_params(dm.group(2))
                 except Unsupported:
                     shapes.pop(dm.group(1), None)
+            continue
+        # array definitions: T name[] = {a, b, ...} (flattened into shape parameter lists)
+        m = re.match(r"(?:Precision|double)\s+(\w+)\s*\[\s*\w*\s*\]\s*=\s*\{(.*)\}$", st)
+        if m:
+            try:
+                arrays[m.group(1)] = [eval_scalar(a, scalars) for a in split_args(m.group(2))]
+            except Unsupported:
+                arrays.pop(m.group(1), None)
+            continue
+        # integer scalars
+        m = re.match(r"(?:int|unsigned int|size_t)\s+(\w+)\s*=\s*([^=]+)$", st)
+        if m:
+            try:
+                scalars[m.group(1)] = eval_scalar(m.group(2), scalars)
+            except Unsupported:
+                scalars.pop(m.group(1), None)
             continue
         # scalar definitions
         m = re.match(r"(?:Precision|double)\s+(\w+)\s*=\s*([^=]+)$", st)
@@ -309,6 +325,7 @@ SHAPES = {
     "trapezoid": ("TestTrap.cpp", r"Trap_t", (11,)),
     "cone": ("TestCons.cpp", r"Cone_t", (7,)),
     "orb": ("TestOrb.cpp", r"Orb_t", (1,)),
+    "polycone": ("TestPolycone.cpp", r"Polycone_t", None),
 }
 
 if __name__ == "__main__":
@@ -318,7 +335,10 @@ if __name__ == "__main__":
     for name in only:
         fn, pat, counts = SHAPES[name]
         recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat)
-        recs = [r for r in recs if len(r["shape"]) in counts]
+        if counts is None:  # sphi dphi nz z[nz] rmin[nz] rmax[nz]
+            recs = [r for r in recs if len(r["shape"]) >= 3 and len(r["shape"]) == 3 + 3 * int(r["shape"][2])]
+        else:
+            recs = [r for r in recs if len(r["shape"]) in counts]
         with open(os.path.join(out_dir, "kat_%s.json" % name), "w") as f:
             json.dump({"source": "test/unit_tests/%s" % fn, "records": recs}, f, indent=0)
         print("%-12s %4d records" % (name, len(recs)))

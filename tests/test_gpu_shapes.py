@@ -32,6 +32,11 @@ CASES = [
     ("cone", [5.0, 10.0, 7.0, 15.0, 10.0, math.radians(20), math.radians(300)]),
     ("cone", [0.0, 10.0, 0.0, 20.0, 15.0, 0.0, 2 * math.pi]),
     ("cone", [20.0, 60.0, 80.0, 140.0, 100.0, math.radians(10), math.radians(300)]),
+    ("polycone", [math.radians(30), math.radians(270), 10,
+                  -50, -40, -25, -10, 0, 5, 20, 30, 40, 50,
+                  0, 5, 5, 8, 8, 3, 3, 10, 10, 0,
+                  20, 25, 25, 30, 40, 40, 30, 35, 35, 20]),
+    ("polycone", [0.0, 2 * math.pi, 4, -1, -0.5, 0.5, 2, 0.1, 0.0, 0.0, 0.4, 1.0, 2.0, 2.0, 1.5]),
 ]
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 
@@ -67,7 +72,7 @@ def test_query_all_matches_single_methods():
         assert torch.equal(res[m], single), m
 
 
-@pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb"])
+@pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

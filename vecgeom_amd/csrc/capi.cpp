@@ -37,10 +37,20 @@ vgmi_status vgmi_shape_create(int type, const double *params, int nparams, vgmi_
   vgmi_shape *s = new (std::nothrow) vgmi_shape();
   if (!s) return set_error(VGMI_ERR_INVALID, "vgmi_shape_create: out of host memory");
   std::string err;
-  if (MakeShapeRecord(type, params, nparams, s->rec, err) != 0) {
+  if (MakeShapeRecord(type, params, nparams, s->rec, s->extra, err) != 0) {
     delete s;
-    return set_error(type >= 0 && type < kNumShapeTypes ? VGMI_ERR_INVALID : VGMI_ERR_INVALID,
-                     "vgmi_shape_create: %s", err.c_str());
+    return set_error(VGMI_ERR_INVALID, "vgmi_shape_create: %s", err.c_str());
+  }
+  if (!s->extra.empty()) {
+    const size_t bytes = s->extra.size() * sizeof(double);
+    hipError_t e       = hipMalloc(&s->extra_dev, bytes);
+    if (e == hipSuccess) e = hipMemcpy(s->extra_dev, s->extra.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (s->extra_dev) (void)hipFree(s->extra_dev);
+      delete s;
+      return set_error(VGMI_ERR_DEVICE, "vgmi_shape_create: device copy of the shape data: %s", hipGetErrorString(e));
+    }
+    SetShapeExtraBase(s->rec, static_cast<const double *>(s->extra_dev));
   }
   *out = s;
   return VGMI_OK;
@@ -48,6 +58,7 @@ vgmi_status vgmi_shape_create(int type, const double *params, int nparams, vgmi_
 
 vgmi_status vgmi_shape_destroy(vgmi_shape *shape)
 {
+  if (shape && shape->extra_dev) (void)hipFree(shape->extra_dev);
   delete shape;
   return VGMI_OK;
 }

@@ -1,6 +1,7 @@
 // C ABI: geometry build (host tables + one device block) and table access.
 #include "geom_internal.h"
 #include "vgmi_internal.h"
+#include "shape_host.h"
 #include <cstring>
 #include <new>
 
@@ -28,7 +29,8 @@ int upload(vgmi_geom &G)
                   {h.offset.data(), h.offset.size() * sizeof(int32_t), 0},
                   {h.nodes.data(), h.nodes.size() * sizeof(AABBRec), 0},
                   {h.prim.data(), h.prim.size() * sizeof(int32_t), 0},
-                  {h.navind.data(), h.navind.size() * sizeof(uint32_t), 0}};
+                  {h.navind.data(), h.navind.size() * sizeof(uint32_t), 0},
+                  {h.extraPool.data(), h.extraPool.size() * sizeof(double), 0}};
   size_t total = 0;
   for (auto &p : parts) {
     p.off = total;
@@ -41,6 +43,13 @@ int upload(vgmi_geom &G)
   std::vector<unsigned char> staging(total, 0);
   for (auto &p : parts)
     if (p.bytes) std::memcpy(staging.data() + p.off, p.src, p.bytes);
+  // shape records that own variable-length data point into the uploaded pool
+  {
+    const double *pool = reinterpret_cast<const double *>(static_cast<unsigned char *>(blk) + parts[10].off);
+    ShapeRecord *recs  = reinterpret_cast<ShapeRecord *>(staging.data() + parts[0].off);
+    for (size_t i = 0; i < h.shapes.size(); ++i)
+      if (h.extraOffset[i] >= 0) SetShapeExtraBase(recs[i], pool + h.extraOffset[i]);
+  }
   e = hipMemcpy(blk, staging.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(blk);

@@ -4,6 +4,7 @@
 #include "dev_box.h"
 #include "dev_cone.h"
 #include "dev_orb.h"
+#include "dev_polycone.h"
 #include "dev_trapezoid.h"
 #include "dev_tube.h"
 
@@ -13,7 +14,8 @@
   X(kTube, TubeData, tube, tube)                                                                                     \
   X(kCone, ConeData, cone, cone)                                                                                     \
   X(kTrapezoid, TrapezoidData, trap, trap)                                                                           \
-  X(kOrb, OrbData, orb, orb)
+  X(kOrb, OrbData, orb, orb)                                                                                         \
+  X(kPolycone, PolyconeData, pcon, pcon)
 
 namespace vgmi {
 

@@ -189,12 +189,18 @@ int build_host_geometry(vgmi_geom_desc const &d, HostGeometry &h, std::string &e
   h = HostGeometry();
   // shapes
   h.shapes.resize(d.nshapes);
+  h.extraOffset.assign(d.nshapes, -1);
   for (int i = 0; i < d.nshapes; ++i) {
     const int off = d.shape_param_offset[i], cnt = d.shape_param_offset[i + 1] - off;
     if (cnt < 0) return err = "shape parameter offsets must be non-decreasing", -1;
     std::string e;
-    if (MakeShapeRecord(d.shape_type[i], d.shape_params + off, cnt, h.shapes[i], e) != 0)
+    std::vector<double> extra;
+    if (MakeShapeRecord(d.shape_type[i], d.shape_params + off, cnt, h.shapes[i], extra, e) != 0)
       return err = "shape " + std::to_string(i) + ": " + e, -1;
+    if (!extra.empty()) {
+      h.extraOffset[i] = int64_t(h.extraPool.size());
+      h.extraPool.insert(h.extraPool.end(), extra.begin(), extra.end());
+    }
   }
   // logical volumes and placements (placement k -> placed volume k, world -> nplaced)
   std::vector<std::vector<int>> dlist(d.nlogical);

@@ -18,6 +18,8 @@ struct HostGeometry {
   std::vector<AABBRec> nodes;
   std::vector<int32_t> bvhDepth;
   std::vector<uint32_t> navind;
+  std::vector<double> extraPool;      // variable-length shape data of all shapes
+  std::vector<int64_t> extraOffset;   // per shape offset into extraPool (-1: none)
   int worldPvol = -1;
   int maxDepth  = 0;
 };

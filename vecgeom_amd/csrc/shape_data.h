@@ -84,6 +84,18 @@ struct ConeData {
   int32_t hasRmin, hasPhi, phiLePi, wedgeLtPi;
 };
 
+// Polycone (VecGeom/volumes/PolyconeStruct.h:144-290): nsec cone sections; the
+// section records, their z shifts and the nsec+1 section boundaries live in the
+// shape's variable-length block (doubles), addressed from `base` once uploaded.
+struct PolyconeData {
+  int32_t nsec;
+  int32_t secOff;   // ConeData[nsec]
+  int32_t shiftOff; // double[nsec]
+  int32_t zsOff;    // double[nsec + 1]
+  const double *base;
+  double rmax, zlo, zhi;
+};
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -96,6 +108,7 @@ struct ShapeRecord {
     TrapezoidData trap;
     OrbData orb;
     ConeData cone;
+    PolyconeData pcon;
     double raw[64];
   } u;
 };

@@ -237,8 +237,74 @@ int MakeCone(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
-int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::string &err)
+// Polycone: PolyconeStruct::Init (VecGeom/volumes/PolyconeStruct.h:144-290). Planes are
+// put in increasing z; every pair of consecutive planes more than kTolerance apart
+// becomes a cone section (built with the raw phi angles) centred at its mid z.
+// Parameters: sphi dphi nz z[nz] rmin[nz] rmax[nz].
+int MakePolycone(double const *a, int n, ShapeRecord &r, std::vector<double> &extra, std::string &err)
 {
+  if (n < 3) return err = "polycone needs sphi dphi nz z[nz] rmin[nz] rmax[nz]", -1;
+  const int nz = int(a[2]);
+  if (nz < 2 || n < 3 + 3 * nz) return err = "polycone needs at least 2 z planes and 3*nz values", -1;
+  std::vector<double> Z(a + 3, a + 3 + nz), Rmin(a + 3 + nz, a + 3 + 2 * nz), Rmax(a + 3 + 2 * nz, a + 3 + 3 * nz);
+  if (Z[0] > Z[nz - 1]) {
+    std::reverse(Z.begin(), Z.end());
+    std::reverse(Rmin.begin(), Rmin.end());
+    std::reverse(Rmax.begin(), Rmax.end());
+  }
+  std::vector<ShapeRecord> cones;
+  std::vector<double> zs, shifts;
+  double prevZ = Z[0], prevRmin = 0, prevRmax = 0, rmaxAll = 0;
+  const int dirZ = Z[1] < Z[0] ? -1 : 1;
+  for (int i = 0; i < nz; ++i) {
+    rmaxAll = std::max(rmaxAll, Rmax[i]);
+    if (i == 0) {
+      zs.push_back(Z[0]);
+    } else if ((Z[i] > prevZ + hTol && dirZ > 0) || (Z[i] < prevZ - hTol && dirZ < 0)) {
+      double cp[7] = {prevRmin, prevRmax, Rmin[i], Rmax[i], (Z[i] - prevZ) / 2, a[0], a[1]};
+      ShapeRecord c;
+      std::string e;
+      if (MakeCone(cp, 7, c, e) != 0) return err = "polycone section: " + e, -1;
+      cones.push_back(c);
+      zs.push_back(Z[i]);
+      const size_t zi = zs.size() - 1;
+      shifts.push_back(zs[zi - 1] + 0.5 * (zs[zi] - zs[zi - 1]));
+    }
+    prevZ    = Z[i];
+    prevRmin = Rmin[i];
+    prevRmax = Rmax[i];
+  }
+  if (cones.empty()) return err = "polycone has no section of non-zero length", -1;
+  r              = ShapeRecord();
+  r.type         = kPolycone;
+  PolyconeData &P = r.u.pcon;
+  P.nsec         = int(cones.size());
+  static_assert(sizeof(ConeData) % sizeof(double) == 0, "cone records must be a whole number of doubles");
+  const size_t coneWords = sizeof(ConeData) / sizeof(double);
+  extra.clear();
+  P.secOff = 0;
+  for (auto const &c : cones) {
+    const double *w = reinterpret_cast<const double *>(&c.u.cone);
+    extra.insert(extra.end(), w, w + coneWords);
+  }
+  P.shiftOff = int(extra.size());
+  extra.insert(extra.end(), shifts.begin(), shifts.end());
+  P.zsOff = int(extra.size());
+  extra.insert(extra.end(), zs.begin(), zs.end());
+  P.base = nullptr;
+  double rm = 0;
+  for (auto const &c : cones) rm = std::max(rm, std::max(c.u.cone.rmax1, c.u.cone.rmax2));
+  P.rmax = rm;
+  P.zlo  = zs.front();
+  P.zhi  = zs.back();
+  (void)rmaxAll;
+  return 0;
+}
+
+int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::vector<double> &extra,
+                    std::string &err)
+{
+  extra.clear();
   switch (type) {
   case kBox:
     return MakeBox(params, nparams, r, err);
@@ -250,10 +316,17 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeOrb(params, nparams, r, err);
   case kCone:
     return MakeCone(params, nparams, r, err);
+  case kPolycone:
+    return MakePolycone(params, nparams, r, extra, err);
   default:
     err = "shape type not supported";
     return -1;
   }
+}
+
+void SetShapeExtraBase(ShapeRecord &r, const double *device_base)
+{
+  if (r.type == kPolycone) r.u.pcon.base = device_base;
 }
 
 void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
@@ -273,6 +346,12 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     return;
   case kOrb:
     for (int i = 0; i < 3; ++i) lo[i] = -r.u.orb.r, hi[i] = r.u.orb.r;
+    return;
+  case kPolycone:
+    lo[0] = lo[1] = -r.u.pcon.rmax;
+    hi[0] = hi[1] = r.u.pcon.rmax;
+    lo[2]         = r.u.pcon.zlo;
+    hi[2]         = r.u.pcon.zhi;
     return;
   case kCone: {
     const double rm = std::max(r.u.cone.origRmax1, r.u.cone.origRmax2);

@@ -2,8 +2,14 @@
 #pragma once
 #include "shape_data.h"
 #include <string>
+#include <vector>
 
 namespace vgmi {
-int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::string &err);
+// Builds the record and, for shapes with variable-length data (polycone sections,
+// ...), fills `extra` with that data; offsets inside the record index `extra`.
+int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::vector<double> &extra,
+                    std::string &err);
+// Points the record at the device copy of its variable-length block.
+void SetShapeExtraBase(ShapeRecord &r, const double *device_base);
 void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3]);
 } // namespace vgmi

@@ -3,9 +3,12 @@
 #include <hip/hip_runtime.h>
 #include "../../include/vgmi.h"
 #include "shape_data.h"
+#include <vector>
 
 struct vgmi_shape {
   vgmi::ShapeRecord rec;
+  std::vector<double> extra;   // variable-length shape data (host copy)
+  void *extra_dev = nullptr;   // device copy, referenced by rec
 };
 
 namespace vgmi {
