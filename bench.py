@@ -1,16 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark of the batched shape-query hot path on one MI355X.
+"""Benchmark of the batched shape-query and BVH-navigation hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-A step is one pass of the hot path over one batch of synthetic input resident in
-HBM. Workload "shapes-box-tube": UnplacedBox (10, 20, 30) and full-phi
-UnplacedTube (5, 10, 15), 2^20 points each (uniform in 1.2x the bounding box,
-isotropic directions, float64 SOA), all six methods per shape per step.
+Headline (value): shape queries/s of the one-launch batched FP64 shape set — every
+implemented shape of the 10-shape mix, per-shape query counts log-uniform in
+2^14..2^20, half the points uniform in 1.2x the bounding box and half within
+1e-3 mm of the surface, isotropic directions, float64 SOA resident in HBM; a step
+is one replay of the captured HIP graph evaluating all six methods
+(Contains, Inside, DistanceToIn/Out, SafetyToIn/Out) for every point.
 
-Prints ONE JSON line (rank 0) with throughput, the roofline fraction of the
-dominant kernel (timed with HIP events on the stream it runs on) and the CPU
-scalar baseline (the oracle restatement, timed on this host).
+Also reported in the same JSON line:
+  box_tube: Box (10,20,30) and full-phi Tube (5,10,15), 2^20 points each, six methods;
+  nav:      calorimeter (23k touchables), 2^20 tracks x 50 ComputeStepAndPropagatedState
+            steps (10 mm limit), fused transport loop; nav-steps/s.
+Each carries a roofline entry (HIP events on the launch stream) and a CPU baseline
+(the oracle restatement of the scalar backend, 1 thread, bounded sample).
 """
 import argparse
 import json
@@ -22,24 +27,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 # algorithmic bytes per query (float64 SOA in, result out)
 BYTES = {"Contains": 25, "Inside": 28, "DistanceToIn": 56, "DistanceToOut": 56, "SafetyToIn": 32,
          "SafetyToOut": 32}
-METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
+# all six methods in one pass: p and d read once, six results written
+BYTES_ALL = 24 + 24 + 1 + 4 + 4 * 8
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="shapes-box-tube")
-    ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20, help="points per shape of the box/tube workload")
     ap.add_argument("--tracks", type=int, default=1 << 20)
     ap.add_argument("--nav-steps", type=int, default=50)
+    ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nav", action="store_true")
+    ap.add_argument("--no-box-tube", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     return ap.parse_args()
 
 
@@ -55,7 +63,91 @@ def cpu_info():
     return model, os.cpu_count()
 
 
-def run_shapes(args):
+def event_time_us(fn, reps):
+    import torch
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) * 1e3 / reps
+
+
+# ------------------------------------------------------------------ headline: batched mix
+def run_mix(args):
+    import numpy as np
+    import torch
+
+    from vecgeom_amd import SOA3D, make_shape, workloads
+    from vecgeom_amd.batch import ShapeBatch, autotune
+
+    shapes = workloads.mix_shapes()
+    gpu = [make_shape(k, p) for k, p in shapes]
+    counts = workloads.mix_counts(len(shapes))
+
+    def dev_fn(s, m):
+        def f(p, d=None):
+            P = SOA3D.from_array(p)
+            if d is None:
+                return getattr(s, m)(P).cpu().numpy()
+            return getattr(s, m)(P, SOA3D.from_array(d)).cpu().numpy()
+        return f
+
+    exts = [tuple(np.array(v) for v in s.Extent()) for s in gpu]
+    p, d = workloads.mix_inputs(exts, counts, [dev_fn(s, "Contains") for s in gpu],
+                                [dev_fn(s, "DistanceToOut") for s in gpu])
+    P, D = SOA3D.from_array(p), SOA3D.from_array(d)
+    batch = ShapeBatch(gpu, counts)
+    outs = batch.alloc_outputs()
+    tuning = None
+    if not args.no_tune:
+        tuning, _ = autotune(batch, P, D, outs)
+    batch.capture(P, D, outs)
+    for _ in range(args.warmup):
+        batch.replay()
+    torch.cuda.synchronize()
+    us = event_time_us(batch.replay, args.steps)
+    total = batch.total
+    queries = total * len(METHODS)
+    gbs = BYTES_ALL * total / (us * 1e-6) / 1e9
+    res = {"value": round(queries / (us * 1e-6), 1), "ms_per_step": round(us / 1e3, 5),
+           "config": {"workload": "batched shape mix, one launch (HIP graph replay), all six methods",
+                      "shapes": [[k, c] for (k, _), c in zip(shapes, counts)], "points": total,
+                      "queries_per_step": queries, "near_surface_fraction": 0.5,
+                      "not_yet_in_mix": ["polyhedron", "sphere", "torus2", "tessellated"],
+                      "tuning": tuning or batch.info()},
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_batch",
+                        "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)}}
+    if not args.no_cpu:
+        from oracle.pyoracle import OracleShape
+        orc = [OracleShape(k, prm) for k, prm in shapes]
+        sample = 1 << 13
+        done, used = 0, 0.0
+        while used < args.cpu_seconds:
+            off = 0
+            for o, c in zip(orc, counts):
+                m = min(c, sample)
+                ps = np.ascontiguousarray(p[:, off:off + m])
+                dsm = np.ascontiguousarray(d[:, off:off + m])
+                t0 = time.perf_counter()
+                for meth in METHODS:
+                    o.query(meth, ps, dsm if meth.startswith("Distance") else None, threads=1)
+                used += time.perf_counter() - t0
+                done += m * len(METHODS)
+                off += c
+        model, ncpu = cpu_info()
+        res["cpu_baseline"] = {"value": round(done / used, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+                               "sample": "%d queries: first %d points of every shape of the same mix x 6 methods, "
+                                         "repeated for ~%.0f s; host %s, %d logical CPUs"
+                                         % (done, sample, args.cpu_seconds, model, ncpu)}
+    return res
+
+
+# ------------------------------------------------------------------ box / tube
+def run_box_tube(args):
     import numpy as np
     import torch
 
@@ -70,78 +162,49 @@ def run_shapes(args):
         outs = {m: torch.empty(n, dtype=torch.uint8 if m == "Contains" else
                                (torch.int32 if m == "Inside" else torch.float64), device="cuda") for m in METHODS}
         shapes.append((kind, params, s, P, D, outs, p, d))
-
-    def step():
-        for kind, params, s, P, D, outs, _, _ in shapes:
-            for m in METHODS:
-                if m.startswith("Distance"):
-                    getattr(s, m)(P, D, out=outs[m])
-                else:
-                    getattr(s, m)(P, out=outs[m])
-
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        step()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = e0.elapsed_time(e1)
-    ms_per_step = ms_total / args.steps
-    queries_per_step = n * len(METHODS) * len(shapes)
-    value = queries_per_step / (ms_per_step * 1e-3)
-
-    # per-kernel average durations over their own timed loops (same stream, HIP events)
-    reps = max(20, args.steps)
     per_kernel = []
+    total_us = 0.0
     for kind, params, s, P, D, outs, _, _ in shapes:
         for m in METHODS:
             f = (lambda s=s, m=m, P=P, D=D, o=outs[m]: getattr(s, m)(P, D, out=o)) if m.startswith("Distance") \
                 else (lambda s=s, m=m, P=P, o=outs[m]: getattr(s, m)(P, out=o))
-            f()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            for _ in range(reps):
+            for _ in range(args.warmup):
                 f()
-            b.record(stream)
-            torch.cuda.synchronize()
-            us = a.elapsed_time(b) * 1e3 / reps
-            gbs = BYTES[m] * n / (us * 1e-6) / 1e9
-            per_kernel.append({"kernel": "%s.%s" % (kind, m), "avg_us": round(us, 3), "GBps": round(gbs, 1)})
+            us = event_time_us(f, max(10, args.steps))
+            total_us += us
+            per_kernel.append({"kernel": "%s.%s" % (kind, m), "avg_us": round(us, 3),
+                               "GBps": round(BYTES[m] * n / (us * 1e-6) / 1e9, 1)})
     dom = max(per_kernel, key=lambda k: k["avg_us"])
     kind, m = dom["kernel"].split(".")
-    roofline = {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_query_fp64<%sData,%s> (%s)" % (kind.capitalize(), m, dom["kernel"]),
-                "bytes_per_launch": BYTES[m] * n}
-
-    cpu = None
+    queries = n * len(METHODS) * len(shapes)
+    res = {"queries_per_s": round(queries / (total_us * 1e-6), 1), "points_per_shape": n,
+           "shapes": [[k, p] for k, p, *_ in shapes],
+           "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "k_query_fp64 (%s)" % dom["kernel"], "bytes_per_launch": BYTES[m] * n},
+           "kernels": per_kernel}
     if not args.no_cpu:
-        cpu = cpu_baseline(shapes, args.cpu_seconds)
-    return {
-        "metric": "shape queries/s (FP64 GPU)",
-        "value": round(value, 1),
-        "unit": "queries/s",
-        "ms_per_step": round(ms_per_step, 5),
-        "wall_s": round(wall, 4),
-        "config": {"workload": "shapes-box-tube", "shapes": [[k, p] for k, p, *_ in shapes], "points_per_shape": n,
-                   "methods": METHODS, "queries_per_step": queries_per_step},
-        "dtype": "f64",
-        "roofline": roofline,
-        "kernels": per_kernel,
-        "cpu_baseline": cpu,
-    }
+        from oracle.pyoracle import OracleShape
+        sample = 1 << 17
+        done, used = 0, 0.0
+        while used < args.cpu_seconds / 2:
+            for kind, params, s, P, D, outs, p, d in shapes:
+                o = OracleShape(kind, params)
+                ps, dsm = np.ascontiguousarray(p[:, :sample]), np.ascontiguousarray(d[:, :sample])
+                t0 = time.perf_counter()
+                for meth in METHODS:
+                    o.query(meth, ps, dsm if meth.startswith("Distance") else None, threads=1)
+                used += time.perf_counter() - t0
+                done += sample * len(METHODS)
+        model, ncpu = cpu_info()
+        res["cpu_baseline"] = {"value": round(done / used, 1), "unit": "queries/s", "cores": 1, "kind": "port",
+                               "sample": "%d queries (first 2^17 points per shape x 6 methods); host %s, %d CPUs"
+                                         % (done, model, ncpu)}
+    return res
 
 
+# ------------------------------------------------------------------ navigation
 def run_nav(args):
-    """Calorimeter navigation: 2^20 tracks (default) from uniform world points with isotropic
-    directions, located once, then nsteps x ComputeStepAndPropagatedState (10 mm limit) with
-    the transport loop fused on the device. One step of the benchmark = the whole loop."""
     import numpy as np
     import torch
 
@@ -156,8 +219,7 @@ def run_nav(args):
     nav0 = G.LocateGlobalPoint(P0)
     torch.cuda.synchronize()
     P = SOA3D.empty(n)
-    nav = torch.empty_like(nav0)
-    last = torch.zeros_like(nav0)
+    nav, last = torch.empty_like(nav0), torch.zeros_like(nav0)
     taken = torch.zeros(n, dtype=torch.int32, device="cuda")
 
     def reset():
@@ -165,75 +227,48 @@ def run_nav(args):
         nav.copy_(nav0)
         last.zero_()
 
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
+    s = torch.cuda.current_stream()
+    for _ in range(max(1, args.warmup)):
         reset()
         G.Propagate(P, D, limit, nav, last, nsteps, taken)
     torch.cuda.synchronize()
     steps_per_run = int(taken.sum())
     total_us = 0.0
-    for _ in range(args.steps):
+    reps = max(3, args.steps // 4)
+    for _ in range(reps):
         reset()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
+        a.record(s)
         G.Propagate(P, D, limit, nav, last, nsteps, taken)
-        b.record(stream)
+        b.record(s)
         torch.cuda.synchronize()
         total_us += a.elapsed_time(b) * 1e3
-    avg_us = total_us / args.steps
-    rate = steps_per_run / (avg_us * 1e-6)
-    # algorithmic HBM bytes of one fused launch: per track read p, d (48 B) and the state (8 B),
-    # write p (24 B) and the state (8 B)
-    bytes_launch = 88 * n
+    avg_us = total_us / reps
+    bytes_launch = 88 * n  # per track: read p, d, state (56 B), write p, state (32 B)
     gbs = bytes_launch / (avg_us * 1e-6) / 1e9
-    res = {"nav_steps_per_s": round(rate, 1), "avg_launch_us": round(avg_us, 2), "steps_per_launch": steps_per_run,
-           "tracks": n, "nsteps": nsteps, "step_limit_mm": limit,
+    res = {"nav_steps_per_s": round(steps_per_run / (avg_us * 1e-6), 1), "avg_launch_us": round(avg_us, 2),
+           "steps_per_launch": steps_per_run, "tracks": n, "nsteps": nsteps, "step_limit_mm": limit,
            "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_propagate",
                         "bytes_per_launch": bytes_launch,
-                        "note": "latency/divergence-bound: state stays in registers for all steps"}}
+                        "note": "latency/divergence bound: track state stays in registers for all steps"}}
     if not args.no_cpu:
         from oracle.pyoracle import OracleGeometry
         O = OracleGeometry(g)
-        m = min(n, 1 << 15)
+        m = min(n, 1 << 14)
         pn = nav0[:m].cpu().numpy().astype(np.uint32)
         t0 = time.perf_counter()
-        done, reps = 0, 0
+        done, reps_c = 0, 0
         while time.perf_counter() - t0 < args.cpu_seconds:
             _, _, _, k = O.propagate(p[:, :m], d[:, :m], limit, pn, np.zeros_like(pn), nsteps, threads=1)
             done += k
-            reps += 1
+            reps_c += 1
         dt = time.perf_counter() - t0
         model, ncpu = cpu_info()
         res["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "nav-steps/s", "cores": 1, "kind": "port",
-                               "sample": "%d x (first %d tracks x %d steps) = %d nav-steps; host %s, %d logical CPUs"
-                                         % (reps, m, nsteps, done, model, ncpu)}
+                               "sample": "%d x (first %d tracks x %d steps) = %d nav-steps; host %s, %d CPUs"
+                                         % (reps_c, m, nsteps, done, model, ncpu)}
     return res
-
-
-def cpu_baseline(shapes, budget_s):
-    """Oracle (CPU scalar FP64 restatement) on a bounded sample of the same inputs, 1 thread."""
-    import numpy as np
-
-    from oracle.pyoracle import OracleShape
-
-    sample = 1 << 18
-    done, t_used = 0, 0.0
-    while t_used < budget_s:
-        for kind, params, s, P, D, outs, p, d in shapes:
-            o = OracleShape(kind, params)
-            ps, ds = np.ascontiguousarray(p[:, :sample]), np.ascontiguousarray(d[:, :sample])
-            t0 = time.perf_counter()
-            for m in METHODS:
-                o.query(m, ps, ds if m.startswith("Distance") else None, threads=1)
-            t_used += time.perf_counter() - t0
-            done += sample * len(METHODS)
-        if t_used <= 0:
-            break
-    model, ncpu = cpu_info()
-    return {"value": round(done / t_used, 1), "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": "%d queries: first 2^18 points of each shape's stream x 6 methods, repeated for ~%.0f s; "
-                      "host %s, %d logical CPUs" % (done, budget_s, model, ncpu)}
 
 
 def main():
@@ -246,14 +281,21 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    torch.cuda.synchronize()
-    res = run_shapes(args)
-    nav = None if args.no_nav else run_nav(args)
-    line = {"metric": res["metric"], "value": res["value"], "unit": res["unit"], "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": res["dtype"],
-            "data": "synthetic", "config": res["config"], "roofline": res["roofline"],
-            "cpu_baseline": res["cpu_baseline"], "kernels": res["kernels"], "nav": nav}
+    print("bench: batched shape mix", file=sys.stderr, flush=True)
+    mix = run_mix(args)
+    extra = {}
+    if not args.no_box_tube:
+        print("bench: box/tube", file=sys.stderr, flush=True)
+        extra["box_tube"] = run_box_tube(args)
+    if not args.no_nav:
+        print("bench: calorimeter navigation", file=sys.stderr, flush=True)
+        extra["nav"] = run_nav(args)
+    line = {"metric": "shape queries/s (FP64 GPU, batched one-launch mix)", "value": mix["value"],
+            "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": mix["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic", "config": mix["config"], "roofline": mix["roofline"],
+            "cpu_baseline": mix.get("cpu_baseline")}
+    line.update(extra)
     print(json.dumps(line), flush=True)
 
 

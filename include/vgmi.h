@@ -107,6 +107,34 @@ VGMI_API vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision
                                  double *safety_in, double *safety_out, size_t n, void *stream);
 
 /* ------------------------------------------------------------------------
+ * One-launch batches over many shapes with uneven point counts.
+ * No single reference counterpart: it batches the per-shape SOA calls above
+ * (the reference issues one kernel per volume and method,
+ * test/VecGeomBenchmark/Benchmarker.cu:140-216). Points of shape s occupy
+ * [sum(counts[0..s)), +counts[s]) of the SOA arrays. The shapes must outlive the batch.
+ * ------------------------------------------------------------------------ */
+typedef struct vgmi_batch vgmi_batch;
+
+VGMI_API vgmi_status vgmi_batch_create(const vgmi_shape *const *shapes, const int64_t *counts, int nshapes,
+                                       vgmi_batch **out);
+VGMI_API vgmi_status vgmi_batch_destroy(vgmi_batch *batch);
+/* Launch shape: points per tile, threads per block (64/128/256/512), resident blocks per CU. */
+VGMI_API vgmi_status vgmi_batch_set_tuning(vgmi_batch *batch, int tile_points, int block, int blocks_per_cu);
+/* info = nshapes, total points, tiles, tile points, block, blocks per CU */
+VGMI_API vgmi_status vgmi_batch_info(const vgmi_batch *batch, int64_t info[6]);
+/* All six methods for every point of the batch in one persistent launch (NULL outputs are skipped). */
+VGMI_API vgmi_status vgmi_batch_query_all(vgmi_batch *batch, int precision, const double *px, const double *py,
+                                          const double *pz, const double *dx, const double *dy, const double *dz,
+                                          uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,
+                                          double *safety_in, double *safety_out, void *stream);
+/* Capture the same launch into a HIP graph bound to these pointers; replay with vgmi_batch_graph_launch. */
+VGMI_API vgmi_status vgmi_batch_graph_capture(vgmi_batch *batch, int precision, const double *px, const double *py,
+                                              const double *pz, const double *dx, const double *dy, const double *dz,
+                                              uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,
+                                              double *safety_in, double *safety_out, void *stream);
+VGMI_API vgmi_status vgmi_batch_graph_launch(vgmi_batch *batch, void *stream);
+
+/* ------------------------------------------------------------------------
  * Geometry and BVH navigation.
  *
  * A geometry is described by plain arrays (the same data a VecGeom user feeds

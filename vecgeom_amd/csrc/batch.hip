@@ -1,0 +1,307 @@
+// One-launch batched queries over many shapes with uneven point counts.
+//
+// The points of shape s occupy [offset_s, offset_s + count_s) of the SOA arrays.
+// The range is cut into tiles of `tile` points; tiles are sorted by the cost class
+// of their shape type (most expensive first, longest-processing-time order) and
+// consumed by a persistent grid through one atomic work counter. A tile belongs to
+// exactly one shape, so every wavefront of a workgroup evaluates one shape type:
+// the type switch is wave-uniform, the shape record is read through scalar loads,
+// and no lane waits on another lane's more expensive shape.
+//
+// The launch sequence (counter reset kernel + persistent kernel) can be captured
+// once into a hipGraph and replayed; the reset is a kernel, not a memset node.
+#include "dev_shapes.h"
+#include "vgmi_internal.h"
+#include <algorithm>
+#include <vector>
+
+namespace vgmi {
+
+struct Tile {
+  int32_t shape; // index into the batch's record table
+  int32_t pad_;
+  int64_t begin, end;
+};
+
+struct BatchOut {
+  const double *px, *py, *pz, *dx, *dy, *dz;
+  uint8_t *contains;
+  int32_t *inside;
+  double *din, *dout, *sin, *sout;
+};
+
+template <class Data>
+VGMI_DEV void run_tile(Data const &s, int64_t b, int64_t e, BatchOut const &o, int nthreads)
+{
+  using S = ShapeOps<Data>;
+  for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
+    Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
+                   __builtin_nontemporal_load(o.pz + i));
+    if (o.contains) __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), o.contains + i);
+    if (o.inside) __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), o.inside + i);
+    if (o.sin) __builtin_nontemporal_store(S::template SafeIn<double>(s, p), o.sin + i);
+    if (o.sout) __builtin_nontemporal_store(S::template SafeOut<double>(s, p), o.sout + i);
+    if (o.din || o.dout) {
+      Vec3<double> d(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
+                     __builtin_nontemporal_load(o.dz + i));
+      if (o.din) __builtin_nontemporal_store(S::template DistIn<double>(s, p, d), o.din + i);
+      if (o.dout) __builtin_nontemporal_store(S::template DistOut<double>(s, p, d), o.dout + i);
+    }
+  }
+}
+
+__global__ void k_batch_reset(uint32_t *counter) { *counter = 0u; }
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__ recs, const Tile *__restrict__ tiles,
+                                                 int ntiles, uint32_t *counter, BatchOut o)
+{
+  __shared__ int s_tile;
+  while (true) {
+    if (threadIdx.x == 0) s_tile = int(atomicAdd(counter, 1u));
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_tile);
+    __syncthreads();
+    if (t >= ntiles) break; // every wave of every block reaches this exit
+    const Tile tile        = tiles[t];
+    const int si           = __builtin_amdgcn_readfirstlane(tile.shape);
+    ShapeRecord const &rec = recs[si];
+    switch (rec.type) {
+#define X(E, T, M, NS)                                                                                               \
+  case E:                                                                                                            \
+    run_tile<T>(rec.u.M, tile.begin, tile.end, o, BLOCK);                                                           \
+    break;
+      VGMI_FOR_EACH_SHAPE(X)
+#undef X
+    default:
+      break;
+    }
+  }
+}
+
+// Relative cost class of one query per shape type (sort key for the tile queue).
+static int cost_class(int type)
+{
+  switch (type) {
+  case kBox:
+  case kOrb:
+    return 1;
+  case kTrapezoid:
+  case kTube:
+    return 2;
+  case kCone:
+    return 4;
+  case kPolycone:
+    return 8;
+  default:
+    return 16;
+  }
+}
+
+} // namespace vgmi
+
+using namespace vgmi;
+
+struct vgmi_batch {
+  std::vector<ShapeRecord> recs;
+  std::vector<int64_t> counts, offsets;
+  int64_t total = 0;
+  int tile = 4096, block = 256, blocksPerCU = 8, numCU = 256;
+  ShapeRecord *d_recs = nullptr;
+  Tile *d_tiles = nullptr;
+  uint32_t *d_counter = nullptr;
+  int ntiles = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graphExec = nullptr;
+};
+
+static vgmi_status batch_build_tiles(vgmi_batch *b)
+{
+  std::vector<Tile> tiles;
+  for (size_t s = 0; s < b->recs.size(); ++s)
+    for (int64_t beg = b->offsets[s]; beg < b->offsets[s] + b->counts[s]; beg += b->tile)
+      tiles.push_back(Tile{int32_t(s), 0, beg, std::min(beg + b->tile, b->offsets[s] + b->counts[s])});
+  std::stable_sort(tiles.begin(), tiles.end(), [&](Tile const &x, Tile const &y) {
+    return cost_class(b->recs[x.shape].type) > cost_class(b->recs[y.shape].type);
+  });
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
+  b->d_tiles = nullptr;
+  b->ntiles  = int(tiles.size());
+  if (tiles.empty()) return VGMI_OK;
+  hipError_t e = hipMalloc(&b->d_tiles, tiles.size() * sizeof(Tile));
+  if (e == hipSuccess) e = hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_batch: tile table upload: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
+static void batch_drop_graph(vgmi_batch *b)
+{
+  if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graphExec = nullptr;
+  b->graph     = nullptr;
+}
+
+static vgmi_status batch_launch(vgmi_batch *b, BatchOut const &o, hipStream_t st)
+{
+  if (b->ntiles == 0) return VGMI_OK;
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, st, b->d_counter);
+  const int grid = std::max(1, std::min(b->ntiles, b->numCU * b->blocksPerCU));
+  switch (b->block) {
+  case 64:
+    hipLaunchKernelGGL(k_batch<64>, dim3(grid), dim3(64), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    break;
+  case 128:
+    hipLaunchKernelGGL(k_batch<128>, dim3(grid), dim3(128), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    break;
+  case 512:
+    hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    break;
+  default:
+    hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_batch launch: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
+static vgmi_status batch_outputs(vgmi_batch *b, int precision, const double *px, const double *py, const double *pz,
+                                 const double *dx, const double *dy, const double *dz, uint8_t *contains,
+                                 int32_t *inside, double *din, double *dout, double *sin, double *sout, BatchOut &o)
+{
+  if (!b) return set_error(VGMI_ERR_INVALID, "vgmi_batch: null batch");
+  if (precision != VGMI_FP64) return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_batch: only VGMI_FP64 is implemented");
+  if (b->total > 0 && (!px || !py || !pz)) return set_error(VGMI_ERR_INVALID, "vgmi_batch: null point pointer");
+  if ((din || dout) && (!dx || !dy || !dz)) return set_error(VGMI_ERR_INVALID, "vgmi_batch: distances need directions");
+  o = BatchOut{px, py, pz, dx, dy, dz, contains, inside, din, dout, sin, sout};
+  return VGMI_OK;
+}
+
+extern "C" {
+
+vgmi_status vgmi_batch_create(const vgmi_shape *const *shapes, const int64_t *counts, int nshapes, vgmi_batch **out)
+{
+  if (!out) return set_error(VGMI_ERR_INVALID, "vgmi_batch_create: null output handle");
+  *out = nullptr;
+  if (nshapes <= 0 || !shapes || !counts) return set_error(VGMI_ERR_INVALID, "vgmi_batch_create: empty batch");
+  auto *b = new vgmi_batch();
+  for (int s = 0; s < nshapes; ++s) {
+    if (!shapes[s] || counts[s] < 0) {
+      delete b;
+      return set_error(VGMI_ERR_INVALID, "vgmi_batch_create: bad shape or count at %d", s);
+    }
+    b->recs.push_back(shapes[s]->rec);
+    b->offsets.push_back(b->total);
+    b->counts.push_back(counts[s]);
+    b->total += counts[s];
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    b->numCU = prop.multiProcessorCount;
+  hipError_t e = hipMalloc(&b->d_recs, b->recs.size() * sizeof(ShapeRecord));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_recs, b->recs.data(), b->recs.size() * sizeof(ShapeRecord), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&b->d_counter, 256);
+  if (e != hipSuccess) {
+    vgmi_batch_destroy(b);
+    return set_error(VGMI_ERR_DEVICE, "vgmi_batch_create: %s", hipGetErrorString(e));
+  }
+  if (vgmi_status s = batch_build_tiles(b)) {
+    vgmi_batch_destroy(b);
+    return s;
+  }
+  *out = b;
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_batch_destroy(vgmi_batch *b)
+{
+  if (!b) return VGMI_OK;
+  batch_drop_graph(b);
+  if (b->d_recs) (void)hipFree(b->d_recs);
+  if (b->d_tiles) (void)hipFree(b->d_tiles);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  delete b;
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_batch_set_tuning(vgmi_batch *b, int tile_points, int block, int blocks_per_cu)
+{
+  if (!b) return set_error(VGMI_ERR_INVALID, "vgmi_batch_set_tuning: null batch");
+  if (tile_points < 64 || (block != 64 && block != 128 && block != 256 && block != 512) || blocks_per_cu < 1 ||
+      blocks_per_cu > 64)
+    return set_error(VGMI_ERR_INVALID, "vgmi_batch_set_tuning: tile >= 64, block in {64,128,256,512}, 1..64 blocks/CU");
+  b->tile        = tile_points;
+  b->block       = block;
+  b->blocksPerCU = blocks_per_cu;
+  batch_drop_graph(b);
+  return batch_build_tiles(b);
+}
+
+vgmi_status vgmi_batch_info(const vgmi_batch *b, int64_t info[6])
+{
+  if (!b || !info) return set_error(VGMI_ERR_INVALID, "vgmi_batch_info: null argument");
+  info[0] = int64_t(b->recs.size());
+  info[1] = b->total;
+  info[2] = b->ntiles;
+  info[3] = b->tile;
+  info[4] = b->block;
+  info[5] = b->blocksPerCU;
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_batch_query_all(vgmi_batch *b, int precision, const double *px, const double *py, const double *pz,
+                                 const double *dx, const double *dy, const double *dz, uint8_t *contains,
+                                 int32_t *inside, double *dist_in, double *dist_out, double *safety_in,
+                                 double *safety_out, void *stream)
+{
+  BatchOut o;
+  if (vgmi_status s = batch_outputs(b, precision, px, py, pz, dx, dy, dz, contains, inside, dist_in, dist_out,
+                                    safety_in, safety_out, o))
+    return s;
+  return batch_launch(b, o, (hipStream_t)stream);
+}
+
+vgmi_status vgmi_batch_graph_capture(vgmi_batch *b, int precision, const double *px, const double *py,
+                                     const double *pz, const double *dx, const double *dy, const double *dz,
+                                     uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,
+                                     double *safety_in, double *safety_out, void *stream)
+{
+  BatchOut o;
+  if (vgmi_status s = batch_outputs(b, precision, px, py, pz, dx, dy, dz, contains, inside, dist_in, dist_out,
+                                    safety_in, safety_out, o))
+    return s;
+  batch_drop_graph(b);
+  hipStream_t cap = nullptr;
+  hipError_t e    = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_batch_graph_capture: stream: %s", hipGetErrorString(e));
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  vgmi_status ls = VGMI_OK;
+  if (e == hipSuccess) {
+    ls = batch_launch(b, o, cap);
+    hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
+    if (e == hipSuccess) e = e2;
+  }
+  if (e == hipSuccess && ls == VGMI_OK) e = hipGraphInstantiate(&b->graphExec, b->graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cap);
+  (void)stream;
+  if (ls != VGMI_OK) return ls;
+  if (e != hipSuccess) {
+    batch_drop_graph(b);
+    return set_error(VGMI_ERR_DEVICE, "vgmi_batch_graph_capture: %s", hipGetErrorString(e));
+  }
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_batch_graph_launch(vgmi_batch *b, void *stream)
+{
+  if (!b) return set_error(VGMI_ERR_INVALID, "vgmi_batch_graph_launch: null batch");
+  if (!b->graphExec) return set_error(VGMI_ERR_INVALID, "vgmi_batch_graph_launch: no captured graph");
+  hipError_t e = hipGraphLaunch(b->graphExec, (hipStream_t)stream);
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_batch_graph_launch: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
+} // extern "C"

@@ -63,3 +63,44 @@ def near_surface(g, n, lo, hi, contains_fn, dist_out_fn, band=1e-3):
         delta = (2.0 * g.random(k) - 1.0) * band
         out = np.concatenate([out, s + delta * e], axis=1)
     return np.ascontiguousarray(out[:, :n])
+
+
+# Batched shape set evaluated in one launch (the shapes implemented so far; each
+# entry is (kind, constructor parameters)). Tube: rmin 2, rmax 8, dz 10, phi 30-270 deg;
+# polycone with 10 z planes; general trapezoid; cone with an inner radius and slopes.
+def mix_shapes():
+    r = np.radians
+    return [
+        ("box", [10.0, 20.0, 30.0]),
+        ("tube", [2.0, 8.0, 10.0, float(r(30)), float(r(240))]),
+        ("cone", [5.0, 10.0, 7.0, 15.0, 10.0, float(r(20)), float(r(300))]),
+        ("polycone", [float(r(30)), float(r(270)), 10,
+                      -50, -40, -25, -10, 0, 5, 20, 30, 40, 50,
+                      0, 5, 5, 8, 8, 3, 3, 10, 10, 0,
+                      20, 25, 25, 30, 40, 40, 30, 35, 35, 20]),
+        ("trapezoid", [60.0, float(r(20)), float(r(5)), 40.0, 30.0, 40.0, float(r(10)), 16.0, 10.0, 14.0,
+                       float(r(10))]),
+        ("orb", [10.0]),
+    ]
+
+
+def mix_counts(nshapes, seed=SEED_BASE + 2, lo=14, hi=20):
+    """Per-shape query counts drawn log-uniform in [2^lo, 2^hi]."""
+    g = rng(seed)
+    return [int(2.0 ** (lo + (hi - lo) * g.random())) for _ in range(nshapes)]
+
+
+def mix_inputs(shapes, counts, contains_fns, dist_out_fns, seed=SEED_BASE + 3):
+    """Concatenated SOA inputs: per shape, half uniform in 1.2x the bounding box and half
+    within 1e-3 mm of the surface; isotropic directions. extents[i] = (lo, hi)."""
+    ps, ds = [], []
+    for i, (ext, n) in enumerate(zip(shapes, counts)):
+        g = rng(seed + 7919 * i)
+        lo, hi = ext
+        n_near = n // 2
+        p_uni = uniform_in_box(g, n - n_near, lo, hi)
+        p_near = near_surface(g, n_near, lo, hi, contains_fns[i], dist_out_fns[i])
+        p = np.concatenate([p_uni, p_near], axis=1)
+        ps.append(p)
+        ds.append(isotropic_dirs(g, p.shape[1]))
+    return np.ascontiguousarray(np.concatenate(ps, axis=1)), np.ascontiguousarray(np.concatenate(ds, axis=1))
