@@ -30,6 +30,8 @@ Shape *MakeShape(int type, const double *a, int n)
     if (nz < 2 || n < 3 + 3 * nz) return nullptr;
     return new Polycone(a[0], a[1], nz, a + 3, a + 3 + nz, a + 3 + 2 * nz);
   }
+  case 6:
+    return n >= 6 ? new Sphere(a[0], a[1], a[2], a[3], a[4], a[5]) : nullptr;
   case 5:
     return n >= 11 ? new Trapezoid(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]) : nullptr;
   default:

@@ -32,6 +32,34 @@ struct Wedge {
   void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
   double SafetyToOut(V3 const &p) const;
   double SafetyToIn(V3 const &p) const;
+  // Wedge_Evolution.h:293-303 (Contains), :235-247 (IsPointOnSurfaceAndMovingOut), :394-436 (DistanceToIn/Out)
+  bool Contains(V3 const &p) const
+  {
+    bool ci = false, co = false;
+    ContainsInside(p, false, ci, co);
+    return !co;
+  }
+  bool OnSurfaceMoving(bool start, bool movingOut, V3 const &p, V3 const &d) const;
+  void DistanceToIn(V3 const &p, V3 const &d, double &d1, double &d2) const;
+  void DistanceToOut(V3 const &p, V3 const &d, double &d1, double &d2) const;
+};
+
+// VecGeom/volumes/ThetaCone_Evolution.h:42-820 (theta section of the sphere)
+struct ThetaCone {
+  double sTheta = 0, dTheta = 0, eTheta = 0, angTol = 0, halfAngTol = 0;
+  double tanS = 0, tanE = 0, tanBis = 0, slope1 = 0, slope2 = 0, tanS2 = 0, tanE2 = 0;
+  void Set(double s, double d);
+  V3 Normal(bool start, V3 const &p) const;
+  bool IsOnSurface(bool start, V3 const &p) const;
+  bool OnSurfaceMoving(bool start, bool movingOut, V3 const &p, V3 const &d) const;
+  bool IsCompletelyInside(V3 const &p) const;
+  bool IsCompletelyOutside(V3 const &p) const;
+  bool Contains(V3 const &p) const { return !IsCompletelyOutside(p); }
+  double DistanceToLine(double slope, double x, double y) const;
+  double SafetyToIn(V3 const &p) const;
+  double SafetyToOut(V3 const &p) const;
+  void DistanceToIn(V3 const &p, V3 const &d, double &d1, double &d2, bool &i1, bool &i2) const;
+  void DistanceToOut(V3 const &p, V3 const &d, double &d1, double &d2, bool &i1, bool &i2) const;
 };
 
 // ---------------------------------------------------------------- Box
@@ -172,6 +200,32 @@ public:
   int SectionIndex(double z) const;
   void SectionInOut(int isec, V3 const &p, bool forInside, bool &cin, bool &cout) const;
   void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+};
+} // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Sphere
+// VecGeom/volumes/SphereStruct.h:25-300, VecGeom/volumes/kernel/SphereImplementation.h:65-417,
+// VecGeom/volumes/SphereUtilities.h:55-195, extent source/UnplacedSphere.cpp:76-165
+class Sphere : public Shape {
+public:
+  double rmin, rmax, sphi, dphi, stheta, dtheta, etheta;
+  bool fullPhi = true, fullTheta = true, full = true;
+  Wedge wedge;     // built from the raw (sphi, dphi) parameters
+  ThetaCone theta; // built from the raw (stheta, dtheta) parameters
+  Sphere(double rmin, double rmax, double sphi, double dphi, double stheta, double dtheta);
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  bool CompletelyInside(V3 const &p) const;
+  bool CompletelyOutside(V3 const &p) const;
+  bool OnSurfaceMoving(bool movingOut, V3 const &p, V3 const &d) const;
+  void MinDistFromPhi(bool toIn, V3 const &p, V3 const &d, double &distance) const;
   bool Contains(V3 const &p) const override;
   int Inside(V3 const &p) const override;
   double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;

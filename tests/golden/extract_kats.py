@@ -11,7 +11,8 @@ record per recognised assertion:
 
 Expectation kinds: "approx" (ApproxEqual, test/unit_tests/ApproxEqual.h),
 "anyof" (a disjunction of ApproxEqual), "inside" (EInside enum),
-"negative" (assert(Dist < 0) / assert(Dist <= 0)).
+"negative" (assert(Dist < 0)), "nonpositive" (assert(Dist <= 0),
+TestWrongSidePoint).
 
 Run in a container that holds the reference tree:
     python tests/golden/extract_kats.py /root/reference
@@ -31,7 +32,7 @@ CONSTS = {
     "kInfLength": DBL_MAX, "kInfinity": DBL_MAX, "kTolerance": 1e-9,
     "kHalfTolerance": 0.5e-9, "kCarTolerance": 1e-9, "kDegToRad": math.pi / 180.0,
     "kRadToDeg": 180.0 / math.pi, "M_PI": math.pi, "deg": math.pi / 180.0,
-    "kRadTolerance": 1e-9, "kAngTolerance": 1e-9,
+    "kRadTolerance": 1e-9, "kAngTolerance": 1e-9, "PI": math.pi,
 }
 FUNCS = {"sqrt": math.sqrt, "cos": math.cos, "sin": math.sin, "tan": math.tan,
          "atan": math.atan, "atan2": math.atan2, "fabs": math.fabs, "abs": abs,
@@ -218,6 +219,18 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
             except Unsupported:
                 scalars.pop(m.group(1), None)
             continue
+        # several scalar definitions in one statement: double a = 1, b = 2 * PI
+        m = re.match(r"(?:Precision|double)\s+(\w+\s*=.*,\s*\w+\s*=.*)$", st)
+        if m and not re.search(r"\.(%s)\(" % METHODS, st):
+            for decl in split_args(m.group(1)):
+                dm = re.fullmatch(r"(\w+)\s*=\s*(.+)", decl.strip(), re.S)
+                if not dm:
+                    continue
+                try:
+                    scalars[dm.group(1)] = eval_scalar(dm.group(2), scalars)
+                except Unsupported:
+                    scalars.pop(dm.group(1), None)
+            continue
         # scalar definitions
         m = re.match(r"(?:Precision|double)\s+(\w+)\s*=\s*([^=]+)$", st)
         if m and not re.search(r"\.(%s)\(" % METHODS, st):
@@ -287,7 +300,12 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                     mm = re.fullmatch(r"(\w+)\s*(<|<=)\s*0(?:\.0*)?", body)
                     if mm and mm.group(1) in pending:
                         sh, meth, p, d, _ = pending[mm.group(1)]
-                        rec = (sh, meth, p, d, ln, {"kind": "negative"})
+                        rec = (sh, meth, p, d, ln, {"kind": "negative" if mm.group(2) == "<" else "nonpositive"})
+                    # TestWrongSidePoint(Dist, msg) is Dist <= 0 (test/unit_tests/TestSphere.cpp:24-29)
+                    mm = re.fullmatch(r"TestWrongSidePoint\s*\(\s*(\w+)\s*,.*\)", body)
+                    if mm and mm.group(1) in pending:
+                        sh, meth, p, d, _ = pending[mm.group(1)]
+                        rec = (sh, meth, p, d, ln, {"kind": "nonpositive"})
             if rec:
                 sh, meth, p, d, line, exp = rec
                 records.append({"line": "%s:%d" % (fname, line), "shape": shapes[sh], "method": meth,
@@ -326,6 +344,7 @@ SHAPES = {
     "cone": ("TestCons.cpp", r"Cone_t", (7,)),
     "orb": ("TestOrb.cpp", r"Orb_t", (1,)),
     "polycone": ("TestPolycone.cpp", r"Polycone_t", None),
+    "sphere": ("TestSphere.cpp", r"Sphere_t", (6,)),
 }
 
 if __name__ == "__main__":

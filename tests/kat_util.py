@@ -38,6 +38,8 @@ def check(rec, got):
         return int(got) == INSIDE[e["value"]]
     if e["kind"] == "negative":
         return got < 0
+    if e["kind"] == "nonpositive":
+        return got <= 0
     raise ValueError(e)
 
 

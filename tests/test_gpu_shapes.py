@@ -37,6 +37,14 @@ CASES = [
                   0, 5, 5, 8, 8, 3, 3, 10, 10, 0,
                   20, 25, 25, 30, 40, 40, 30, 35, 35, 20]),
     ("polycone", [0.0, 2 * math.pi, 4, -1, -0.5, 0.5, 2, 0.1, 0.0, 0.0, 0.4, 1.0, 2.0, 2.0, 1.5]),
+    ("sphere", [6.0, 8.0, 0.0, 2 * math.pi, 0.0, math.pi]),
+    ("sphere", [0.0, 10.0, math.radians(20), math.radians(270), math.radians(30), math.radians(100)]),
+    ("sphere", [45.0, 50.0, -math.pi / 4, 0.5 * math.pi, math.pi / 4, 0.5 * math.pi]),
+    ("sphere", [0.0, 50.0, 0.0, 2 * math.pi, 0.0, math.pi / 4]),
+    ("sphere", [5.0, 50.0, -math.pi, 1.5 * math.pi, 0.0, math.pi]),
+    ("sphere", [0.0, 50.0, 0.0, 2 * math.pi, 0.75 * math.pi, 0.25 * math.pi]),
+    ("sphere", [6.0, 8.0, math.pi / 6, math.pi / 3, 0.0, math.pi]),
+    ("sphere", [0.0, 410.0, 0.0, 2 * math.pi, 0.5 * math.pi, 0.5 * math.pi]),
 ]
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 

@@ -12,12 +12,18 @@
 
 namespace vgmi {
 
+#ifndef VGMI_DEV
 #define VGMI_DEV __device__ __forceinline__
+#endif
 
 constexpr double kTolerance     = 1e-9;
 constexpr double kHalfTolerance = 0.5 * kTolerance;
 constexpr double kPushTolerance = 1e-6;
 constexpr double kConeTolerance = 1e-7;
+// Mixed precision: points whose FP32 safeties put them within this distance (mm) of a
+// surface, and FP32 section hand-offs this close to an internal plane, are evaluated in FP64.
+constexpr double kMixedBand      = 1e-3;
+constexpr double kMixedPlaneBand = 1e-3;
 constexpr double kInfLength     = DBL_MAX;
 constexpr double kPi            = 3.14159265358979323846;
 constexpr double kTwoPi         = 2. * kPi;
@@ -98,6 +104,27 @@ template <class R>
 VGMI_DEV R Sign(R x)
 {
   return vcopysign(R(1), x);
+}
+
+// Band for re-checking an intersection point a kernel has just computed (is the hit
+// on this plane / half-plane?). FP64 uses the reference's band; in FP32 the point is
+// only known to ~1e-7 of its coordinates, so the FP32 pass uses kMixedBand, which
+// accepts every genuine hit of that surface.
+template <class R>
+VGMI_DEV R HitTol(double fp64_band)
+{
+  return sizeof(R) == sizeof(float) ? R(kMixedBand) : R(fp64_band);
+}
+
+// Root (-b + sgn*sqrt|D|) / a of a*t^2 + 2*b*t + c = 0. FP64 keeps the reference's form
+// (multiplying by the given 1/a); the FP32 pass switches to the equal c / (-b - sgn*sqrt|D|)
+// where -b and sgn*sqrt|D| would cancel.
+template <class R>
+VGMI_DEV R QuadRoot(R b, R D, R ainv, R c, int sgn)
+{
+  const R sq = vsqrt(vabs(D));
+  if (sizeof(R) == sizeof(float) && ((sgn > 0) == (b > R(0)))) return c / NonZero(-b - R(sgn) * sq);
+  return (-b + R(sgn) * sq) * ainv;
 }
 
 template <class R>

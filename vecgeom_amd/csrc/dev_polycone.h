@@ -122,6 +122,16 @@ VGMI_DEV R DistanceToOut(PolyconeData const &P, Vec3<R> const &point, Vec3<R> co
   Vec3<R> pn = point;
   do {
     ConeData const &S = sec(P, index);
+    if constexpr (sizeof(R) == sizeof(float)) {
+      // FP32 pass of the mixed path: the hand-off below tests the exit point against
+      // the next section with the 1e-9 mm band, but in FP32 that point lies ~1e-6 of
+      // its magnitude off the shared z plane. Exits within kMixedPlaneBand of an internal
+      // plane are answered NaN, which sends the point to the FP64 re-evaluation.
+      if (istep > 0) {
+        const R plane = R(zb(P, inc > 0 ? index : index + 1));
+        if (vabs(point.z + total * dir.z - plane) < R(kMixedPlaneBand)) return R(__builtin_nanf(""));
+      }
+    }
     if (total != R(0) || istep < 2) {
       pn = point + total * dir;
       pn.z -= R(shift(P, index));

@@ -5,6 +5,7 @@
 #include "dev_cone.h"
 #include "dev_orb.h"
 #include "dev_polycone.h"
+#include "dev_sphere.h"
 #include "dev_trapezoid.h"
 #include "dev_tube.h"
 
@@ -15,7 +16,8 @@
   X(kCone, ConeData, cone, cone)                                                                                     \
   X(kTrapezoid, TrapezoidData, trap, trap)                                                                           \
   X(kOrb, OrbData, orb, orb)                                                                                         \
-  X(kPolycone, PolyconeData, pcon, pcon)
+  X(kPolycone, PolyconeData, pcon, pcon)                                                                             \
+  X(kSphere, SphereData, sph, sph)
 
 namespace vgmi {
 

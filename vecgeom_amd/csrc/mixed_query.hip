@@ -7,14 +7,16 @@
 // that depends on the tolerance bands (Inside/Contains, on-surface conventions,
 // wrong-side -1) is the FP64 one; elsewhere the FP32 rounding error (about 1e-7
 // of the local coordinates) is far below the distance to any surface, so
-// Inside/Contains cannot flip and distances carry only the FP32 error.
+// Inside/Contains cannot flip and distances carry only the FP32 error. An FP32
+// kernel that cannot decide within FP32 rounding (a polycone section hand-off)
+// answers NaN, and that point is evaluated in FP64 as well.
 #include "dev_shapes.h"
 #include "vgmi_internal.h"
 
 namespace vgmi {
 
 constexpr int kMixBlock  = 256;
-constexpr float kBand    = 1e-3f; // mm
+constexpr float kBand    = float(kMixedBand);
 constexpr float kFltBig  = 1e30f;
 
 template <class Data, int OP>
@@ -50,13 +52,14 @@ __global__ __launch_bounds__(kMixBlock) void k_query_mixed(Data s, const double 
     const Vec3<double> d(__builtin_nontemporal_load(dx + i), __builtin_nontemporal_load(dy + i),
                          __builtin_nontemporal_load(dz + i));
     double r;
-    if (near) {
-      r = OP == kOpDistIn ? S::template DistIn<double>(s, p, d) : S::template DistOut<double>(s, p, d);
-    } else {
+    bool fp64 = near;
+    if (!near) {
       const Vec3<float> df(float(d.x), float(d.y), float(d.z));
       const float f = OP == kOpDistIn ? S::template DistIn<float>(s, pf, df) : S::template DistOut<float>(s, pf, df);
       r             = f >= kFltBig ? kInfLength : double(f);
+      fp64          = f != f;
     }
+    if (fp64) r = OP == kOpDistIn ? S::template DistIn<double>(s, p, d) : S::template DistOut<double>(s, p, d);
     __builtin_nontemporal_store(r, (double *)out + i);
   }
 }

@@ -96,6 +96,19 @@ struct PolyconeData {
   double rmax, zlo, zhi;
 };
 
+// Sphere (VecGeom/volumes/SphereStruct.h:25-266): shell radii, the normalised phi
+// range and clamped theta range, plus the wedge vectors and theta-cone cache built
+// from the raw constructor angles (Wedge_Evolution.cpp:17-108, ThetaCone_Evolution.h:62-83).
+struct SphereData {
+  double rmin, rmax, sphi, dphi, stheta, dtheta, etheta;
+  double w1x, w1y, w2x, w2y;      // wedge along vectors
+  double n1x, n1y, n2x, n2y;      // wedge inward normals
+  double tS, tE;                  // theta-cone start and end angles (raw)
+  double tanS, tanE, tanS2, tanE2, tanBis, slope1, slope2;
+  int32_t fullPhi, fullTheta, full;
+  int32_t wedgeLtPi;              // raw dphi < pi (Wedge kernels)
+};
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -109,6 +122,7 @@ struct ShapeRecord {
     OrbData orb;
     ConeData cone;
     PolyconeData pcon;
+    SphereData sph;
     double raw[64];
   } u;
 };

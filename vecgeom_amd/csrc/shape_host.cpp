@@ -301,6 +301,133 @@ int MakePolycone(double const *a, int n, ShapeRecord &r, std::vector<double> &ex
   return 0;
 }
 
+// Sphere: SphereStruct.h:245-266 (constructor), CheckPhiAngles :128-150 with :180-199,
+// CheckThetaAngles :100-126; wedge from the raw (sphi, dphi), theta cones from the raw
+// (stheta, dtheta) (ThetaCone_Evolution.h:62-83).
+int MakeSphere(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 6) return err = "sphere needs rmin rmax sphi dphi stheta dtheta", -1;
+  if (!(a[1] > 0) || a[0] < 0 || !(a[0] < a[1])) return err = "sphere needs 0 <= rmin < rmax", -1;
+  r              = ShapeRecord();
+  r.type         = kSphere;
+  SphereData &s  = r.u.sph;
+  const double sphi = a[2], dphi = a[3], sth = a[4], dth = a[5];
+  s.rmin         = a[0];
+  s.rmax         = a[1];
+  s.sphi         = sphi;
+  s.dphi         = dphi;
+  s.stheta       = sth;
+  s.dtheta       = dth;
+  // wedge (Wedge_Evolution.cpp:100-108)
+  s.w1x = std::cos(sphi);
+  s.w1y = std::sin(sphi);
+  s.w2x = std::cos(sphi + dphi);
+  s.w2y = std::sin(sphi + dphi);
+  s.n1x = -std::sin(sphi);
+  s.n1y = std::cos(sphi);
+  s.n2x = std::sin(sphi + dphi);
+  s.n2y = -std::cos(sphi + dphi);
+  s.wedgeLtPi = dphi < hPi;
+  // theta cones
+  s.tS      = sth;
+  s.tE      = sth + dth;
+  double ts = s.tS, te = s.tE;
+  if (s.tS > hPi / 2) ts = hPi - s.tS;
+  if (s.tE > hPi / 2) te = hPi - s.tE;
+  s.tanS   = std::tan(ts);
+  s.tanS2  = s.tanS * s.tanS;
+  s.tanE   = std::tan(te);
+  s.tanE2  = s.tanE * s.tanE;
+  s.tanBis = std::tan(ts + (dth / 2));
+  if (s.tS > hPi / 2 && s.tE > hPi / 2) s.tanBis = std::tan(ts - (dth / 2));
+  s.slope1 = std::tan(hPi / 2 - s.tS);
+  s.slope2 = std::tan(hPi / 2 - s.tE);
+  // phi range checks
+  bool fullPhi;
+  if (dphi >= 2 * hPi - 1e-9 * 0.5) {
+    s.dphi  = 2 * hPi;
+    s.sphi  = 0;
+    fullPhi = true;
+  } else {
+    fullPhi = false;
+    if (dphi > 0) s.dphi = dphi;
+    if (sphi < 0)
+      s.sphi = 2 * hPi - std::fmod(std::fabs(sphi), 2 * hPi);
+    else
+      s.sphi = std::fmod(sphi, 2 * hPi);
+    if (s.sphi + s.dphi > 2 * hPi) s.sphi -= 2 * hPi;
+  }
+  // theta range checks
+  if (dth + sth >= hPi)
+    s.dtheta = hPi - sth;
+  else if (dth > 0)
+    s.dtheta = dth;
+  const bool fullTheta = !(s.dtheta - s.stheta < hPi);
+  s.fullPhi   = fullPhi;
+  s.fullTheta = fullTheta;
+  s.full      = fullPhi && fullTheta;
+  s.etheta    = s.stheta + s.dtheta;
+  return 0;
+}
+
+static bool SphereWedgeContains(SphereData const &s, double x, double y)
+{
+  // Wedge::Contains (Wedge_Evolution.h:293-336 without the surface test)
+  const double startCheck = (-x * s.w1y + y * s.w1x);
+  const double endCheck   = (-s.w2x * y + s.w2y * x);
+  bool out                = startCheck < 0.;
+  if (s.wedgeLtPi)
+    out = out || (endCheck < 0.);
+  else
+    out = out && (endCheck < 0.);
+  return !out;
+}
+
+// source/UnplacedSphere.cpp:76-165
+static void SphereExtent(SphereData const &s, double lo[3], double hi[3])
+{
+  for (int i = 0; i < 3; ++i) lo[i] = -s.rmax, hi[i] = s.rmax;
+  if (s.full) return;
+  double st1 = 0., st2 = 0.;
+  if (!s.fullTheta) {
+    hi[2] = ((s.stheta <= hPi / 2.) ? s.rmax : s.rmin) * std::cos(s.stheta);
+    lo[2] = ((s.etheta <= hPi / 2.) ? s.rmin : s.rmax) * std::cos(s.etheta);
+    st1   = std::sin(s.stheta);
+    st2   = std::sin(s.etheta);
+    double sx = 0;
+    bool set  = false;
+    if (s.stheta <= hPi / 2.) {
+      if (s.etheta < hPi / 2.) sx = s.rmax * st2, set = true;
+    } else {
+      sx = s.rmax * st1, set = true;
+    }
+    if (set) hi[0] = sx, lo[0] = -sx, hi[1] = sx, lo[1] = -sx;
+  }
+  if (!s.fullPhi) {
+    double Rmax = s.rmax;
+    if (s.stheta > hPi / 2.) Rmax *= st1;
+    if (s.etheta < hPi / 2.) Rmax *= st2;
+    const double Rmin = s.rmin * std::min(st1, st2);
+    const double Rin  = 0.5 * (Rmax + Rmin);
+    const bool o0 = !SphereWedgeContains(s, Rin, 0), o90 = !SphereWedgeContains(s, 0, Rin);
+    const bool o180 = !SphereWedgeContains(s, -Rin, 0), o270 = !SphereWedgeContains(s, 0, -Rin);
+    if (!(o0 || o90 || o180 || o270)) return;
+    const double Cx = Rmax * std::cos(s.sphi), Dx = Rmax * std::cos(s.sphi + s.dphi);
+    const double Cy = Rmax * std::sin(s.sphi), Dy = Rmax * std::sin(s.sphi + s.dphi);
+    if (o0) hi[0] = std::max(Cx, Dx);
+    if (o90) hi[1] = std::max(Cy, Dy);
+    if (o180) lo[0] = std::min(Cx, Dx);
+    if (o270) lo[1] = std::min(Cy, Dy);
+    if (s.dphi >= hPi) return;
+    const double Ax = Rmin * std::cos(s.sphi), Bx = Rmin * std::cos(s.sphi + s.dphi);
+    const double Ay = Rmin * std::sin(s.sphi), By = Rmin * std::sin(s.sphi + s.dphi);
+    hi[0] = std::max(std::max(Ax, Bx), hi[0]);
+    hi[1] = std::max(std::max(Ay, By), hi[1]);
+    lo[0] = std::min(std::min(Ax, Bx), lo[0]);
+    lo[1] = std::min(std::min(Ay, By), lo[1]);
+  }
+}
+
 int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::vector<double> &extra,
                     std::string &err)
 {
@@ -318,6 +445,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeCone(params, nparams, r, err);
   case kPolycone:
     return MakePolycone(params, nparams, r, extra, err);
+  case kSphere:
+    return MakeSphere(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -346,6 +475,9 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     return;
   case kOrb:
     for (int i = 0; i < 3; ++i) lo[i] = -r.u.orb.r, hi[i] = r.u.orb.r;
+    return;
+  case kSphere:
+    SphereExtent(r.u.sph, lo, hi);
     return;
   case kPolycone:
     lo[0] = lo[1] = -r.u.pcon.rmax;

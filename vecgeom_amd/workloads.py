@@ -67,7 +67,8 @@ def near_surface(g, n, lo, hi, contains_fn, dist_out_fn, band=1e-3):
 
 # Batched shape set evaluated in one launch (the shapes implemented so far; each
 # entry is (kind, constructor parameters)). Tube: rmin 2, rmax 8, dz 10, phi 30-270 deg;
-# polycone with 10 z planes; general trapezoid; cone with an inner radius and slopes.
+# polycone with 10 z planes; general trapezoid; cone with an inner radius and slopes;
+# sphere shell with theta (30-130 deg) and phi (20-290 deg) cuts.
 def mix_shapes():
     r = np.radians
     return [
@@ -80,6 +81,7 @@ def mix_shapes():
                       20, 25, 25, 30, 40, 40, 30, 35, 35, 20]),
         ("trapezoid", [60.0, float(r(20)), float(r(5)), 40.0, 30.0, 40.0, float(r(10)), 16.0, 10.0, 14.0,
                        float(r(10))]),
+        ("sphere", [5.0, 10.0, float(r(20)), float(r(270)), float(r(30)), float(r(100))]),
         ("orb", [10.0]),
     ]
 
