@@ -30,6 +30,12 @@ Shape *MakeShape(int type, const double *a, int n)
     if (nz < 2 || n < 3 + 3 * nz) return nullptr;
     return new Polycone(a[0], a[1], nz, a + 3, a + 3 + nz, a + 3 + 2 * nz);
   }
+  case 4: {
+    if (n < 4) return nullptr;
+    int sides = int(a[2]), nz = int(a[3]);
+    if (sides < 1 || nz < 2 || n < 4 + 3 * nz) return nullptr;
+    return new Polyhedron(a[0], a[1], sides, nz, a + 4, a + 4 + nz, a + 4 + 2 * nz);
+  }
   case 6:
     return n >= 6 ? new Sphere(a[0], a[1], a[2], a[3], a[4], a[5]) : nullptr;
   case 5:

@@ -235,3 +235,73 @@ public:
   void Extent(V3 &lo, V3 &hi) const override;
 };
 } // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Polyhedron
+// VecGeom/volumes/Quadrilaterals.h:105-630 with source/Quadrilaterals.cpp:36-118 and
+// source/Planes.cpp:30-52 (plane sets); VecGeom/volumes/PolyhedronStruct.h:91-430 (segments);
+// VecGeom/volumes/kernel/PolyhedronImplementation.h:298-1200 (kernels, generic
+// EInnerRadii/EPhiCutout specialisation); extent source/UnplacedPolyhedron.cpp:273-303.
+struct Quad {
+  V3 n;
+  double d = 0;
+  V3 sn[4];
+  double sd[4] = {0, 0, 0, 0};
+  V3 c[4];
+};
+
+struct Quads {
+  std::vector<Quad> q;
+  int size() const { return int(q.size()); }
+  void Set(int i, V3 const &c0, V3 const &c1, V3 const &c2, V3 const &c3);
+  void FlipSign(int i) { q[i].n = -q[i].n, q[i].d = -q[i].d; }
+  bool Contains(V3 const &p) const;
+  int Inside(V3 const &p) const;
+  int InsideAt(V3 const &p, int i) const;
+  bool RayHits(int i, V3 const &x) const;
+  double DistanceToIn(bool behind, V3 const &p, V3 const &d) const;
+  double DistanceToOut(V3 const &p, V3 const &d, double zMin, double zMax) const;
+  double DistanceSquared(int i, V3 const &p) const;
+};
+
+struct ZSeg {
+  Quads outer, phi, inner;
+  bool hasInner() const { return inner.size() > 0; }
+};
+
+class Polyhedron : public Shape {
+public:
+  int sides = 0, nz = 0;
+  bool hasInnerRadii = false, hasPhiCutout = false, hasLargePhiCutout = false;
+  double phiStart = 0, phiDelta = 0;
+  Wedge wedge;
+  std::vector<ZSeg> segs;
+  std::vector<double> zs, rmin, rmax;
+  std::vector<char> sameZ;
+  std::vector<V3> phiSections;
+  Tube bounding{0, 1, 1, 0, kTwoPi};
+  double boundingOffset = 0;
+  Polyhedron(double phiStart, double phiDelta, int sides, int nz, const double *z, const double *rmin,
+             const double *rmax);
+  int NSeg() const { return int(segs.size()); }
+  int FindZSegment(double z) const;
+  int FindPhiSegment(V3 const &p) const;
+  double DistanceToInZSegment(int i, V3 const &p, V3 const &d) const;
+  double DistanceToOutZSegment(int i, double zMin, double zMax, V3 const &p, V3 const &d) const;
+  double SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bool ptInside) const;
+  void DistanceToEndcaps(bool inside, V3 const &p, V3 const &d, double &distance) const;
+  void SafetyToEndcapsSquared(V3 const &p, double &dsq) const;
+  bool InPhiCutoutWedge(ZSeg const &s, V3 const &p) const;
+  int InsideSegBorder(V3 const &p, int zIndex) const;
+  int InsideSegPhi(V3 const &p, int zIndex, int phiIndex) const;
+  double Safety(V3 const &p, bool ptInside) const;
+  int WedgeInside(V3 const &p) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override { return Safety(p, false); }
+  double SafetyToOut(V3 const &p) const override { return Safety(p, true); }
+  void Extent(V3 &lo, V3 &hi) const override;
+};
+} // namespace vgo

@@ -12,7 +12,8 @@ record per recognised assertion:
 Expectation kinds: "approx" (ApproxEqual, test/unit_tests/ApproxEqual.h),
 "anyof" (a disjunction of ApproxEqual), "inside" (EInside enum),
 "negative" (assert(Dist < 0)), "nonpositive" (assert(Dist <= 0),
-TestWrongSidePoint).
+TestWrongSidePoint), "absdiff" (assert(fabs(s->M(..) - v) < tol)), "bool"
+(assert(s->Contains(p)) / assert(!s->Contains(p))).
 
 Run in a container that holds the reference tree:
     python tests/golden/extract_kats.py /root/reference
@@ -182,12 +183,21 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
         for a in args:
             a = a.strip()
             if a in arrays:
+                if any(v is None for v in arrays[a]):
+                    raise Unsupported(a)
                 vals.extend(arrays[a])
             else:
                 vals.append(eval_scalar(a, scalars))
         return vals
 
     for ln, st in statements(text):
+        # drop block braces and function headers joined in front of a statement
+        while True:
+            st2 = re.sub(r"^[^=;]*?[{}]\s*(?=[A-Za-z_!])", "", st)
+            st2 = re.sub(r"^.*?\)\s*\{\s*(?=[A-Za-z_!])", "", st2)
+            if st2 == st:
+                break
+            st = st2
         st = re.sub(r"^\s*(?:const\s+)?", "", st)
         # shape definitions
         m = re.match(r"(?:%s)\s*\*?\s*(\w+)\s*(?:=\s*new\s+(?:%s)\s*)?\((.*)\)$" % (shape_t, shape_t), st)
@@ -209,6 +219,21 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
             try:
                 arrays[m.group(1)] = [eval_scalar(a, scalars) for a in split_args(m.group(2))]
             except Unsupported:
+                arrays.pop(m.group(1), None)
+            continue
+        # arrays declared with a size and filled element by element: T name[N]; name[i] = v;
+        m = re.match(r"(?:Precision|double)\s+(\w+)\s*\[\s*(\w+)\s*\]$", st)
+        if m:
+            try:
+                arrays[m.group(1)] = [None] * int(eval_scalar(m.group(2), scalars))
+            except (Unsupported, ValueError):
+                arrays.pop(m.group(1), None)
+            continue
+        m = re.match(r"(\w+)\s*\[\s*(\d+)\s*\]\s*=\s*([^=].*)$", st)
+        if m and m.group(1) in arrays:
+            try:
+                arrays[m.group(1)][int(m.group(2))] = eval_scalar(m.group(3), scalars)
+            except (Unsupported, IndexError):
                 arrays.pop(m.group(1), None)
             continue
         # integer scalars
@@ -274,8 +299,30 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
         if m:
             body = m.group(1).strip()
             rec = None
+            # fabs(shape->Method(args) - value) < tol, fabs(shape->Method(args)) < tol
+            mq = re.fullmatch(r"(?:std::)?fabs\s*\(\s*\(?\s*(\w+)\s*(?:\.|->)\s*(%s)\s*\(([^()]*(?:\([^()]*\))?[^()]*)\)"
+                              r"\s*(?:-\s*([-+\w.()*/ ]+?))?\s*\)?\s*\)\s*<\s*(\w+)" % METHODS, body)
+            # shape->Contains(p) / !shape->Contains(p)
+            mc = re.fullmatch(r"(!?)\s*(\w+)\s*(?:\.|->)\s*Contains\s*\((.*)\)", body)
             mm = re.fullmatch(r"(\w+)\s*(?:\.|->)\s*Inside\s*\((.*)\)\s*==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", body)
-            if mm and mm.group(1) in shapes:
+            if mq and mq.group(1) in shapes and "||" not in body:
+                try:
+                    a = split_args(mq.group(3))
+                    p = eval_vec(a[0], vecs, scalars, vec_type)
+                    meth = mq.group(2)
+                    d = eval_vec(a[1], vecs, scalars, vec_type) if meth in ("DistanceToIn", "DistanceToOut") else None
+                    val = eval_scalar(mq.group(4), scalars) if mq.group(4) else 0.0
+                    tol = eval_scalar(mq.group(5), scalars)
+                    rec = (mq.group(1), meth, p, d, ln, {"kind": "absdiff", "value": val, "tol": tol})
+                except (Unsupported, IndexError):
+                    rec = None
+            elif mc and mc.group(2) in shapes:
+                try:
+                    p = eval_vec(mc.group(3), vecs, scalars, vec_type)
+                    rec = (mc.group(2), "Contains", p, None, ln, {"kind": "bool", "value": mc.group(1) != "!"})
+                except Unsupported:
+                    rec = None
+            elif mm and mm.group(1) in shapes:
                 try:
                     p = eval_vec(mm.group(2), vecs, scalars, vec_type)
                     rec = (mm.group(1), "Inside", p, None, ln, {"kind": "inside", "value": mm.group(3)})
@@ -345,6 +392,7 @@ SHAPES = {
     "orb": ("TestOrb.cpp", r"Orb_t", (1,)),
     "polycone": ("TestPolycone.cpp", r"Polycone_t", None),
     "sphere": ("TestSphere.cpp", r"Sphere_t", (6,)),
+    "polyhedron": ("TestPolyhedra.cpp", r"Polyhedra_t", "polyhedron"),
 }
 
 if __name__ == "__main__":
@@ -356,6 +404,8 @@ if __name__ == "__main__":
         recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat)
         if counts is None:  # sphi dphi nz z[nz] rmin[nz] rmax[nz]
             recs = [r for r in recs if len(r["shape"]) >= 3 and len(r["shape"]) == 3 + 3 * int(r["shape"][2])]
+        elif counts == "polyhedron":  # sphi dphi nsides nz z[nz] rmin[nz] rmax[nz]
+            recs = [r for r in recs if len(r["shape"]) >= 4 and len(r["shape"]) == 4 + 3 * int(r["shape"][3])]
         else:
             recs = [r for r in recs if len(r["shape"]) in counts]
         with open(os.path.join(out_dir, "kat_%s.json" % name), "w") as f:

@@ -40,6 +40,10 @@ def check(rec, got):
         return got < 0
     if e["kind"] == "nonpositive":
         return got <= 0
+    if e["kind"] == "absdiff":
+        return abs(got - e["value"]) < e["tol"]
+    if e["kind"] == "bool":
+        return bool(got) == e["value"]
     raise ValueError(e)
 
 

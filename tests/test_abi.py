@@ -57,3 +57,12 @@ def test_extent_matches_oracle():
         lo, hi = make_shape(kind, params).Extent()
         olo, ohi = OracleShape(kind, params).extent()
         assert list(lo) == list(olo) and list(hi) == list(ohi), (kind, params, lo, hi, olo, ohi)
+
+
+def test_batch_rejects_mismatched_counts_before_native_call():
+    import pytest
+    from vecgeom_amd import make_shape
+    from vecgeom_amd.batch import ShapeBatch
+    box = make_shape("box", [1.0, 2.0, 3.0])
+    with pytest.raises(ValueError):
+        ShapeBatch([box], [10, 20])

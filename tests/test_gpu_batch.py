@@ -11,6 +11,10 @@ from vecgeom_amd.batch import METHODS, ShapeBatch
 pytestmark = pytest.mark.gpu
 
 
+# uneven per-shape counts (one per shape of the mix), including a single-point shape
+COUNTS = [3000, 5000, 777, 4096, 1, 2500, 1500, 333, 2048, 640, 100, 4000]
+
+
 def _mix(counts):
     shapes = workloads.mix_shapes()
     gpu = [make_shape(k, p) for k, p in shapes]
@@ -22,7 +26,7 @@ def _mix(counts):
 
 
 def test_batch_matches_oracle_and_single_launches():
-    counts = [3000, 5000, 777, 4096, 1, 2500]
+    counts = COUNTS[:len(workloads.mix_shapes())]
     shapes, gpu, orc, p, d = _mix(counts)
     P, D = SOA3D.from_array(p), SOA3D.from_array(d)
     b = ShapeBatch(gpu, counts)
@@ -42,7 +46,7 @@ def test_batch_matches_oracle_and_single_launches():
 
 
 def test_batch_graph_replay():
-    counts = [2048, 1024, 512, 4000, 100, 3000]
+    counts = COUNTS[::-1][:len(workloads.mix_shapes())]
     shapes, gpu, orc, p, d = _mix(counts)
     P, D = SOA3D.from_array(p), SOA3D.from_array(d)
     b = ShapeBatch(gpu, counts)

@@ -47,6 +47,9 @@ class ShapeBatch:
         L = _declare()
         self.shapes = list(shapes)  # keep the shapes alive: the batch references their device data
         self.counts = [int(c) for c in counts]
+        if len(self.shapes) != len(self.counts) or not self.shapes:
+            raise ValueError("ShapeBatch needs one count per shape (%d shapes, %d counts)"
+                             % (len(self.shapes), len(self.counts)))
         arr = (ctypes.c_void_p * len(self.shapes))(*[s.handle.value for s in self.shapes])
         cnt = (ctypes.c_int64 * len(self.counts))(*self.counts)
         h = ctypes.c_void_p()
