@@ -45,6 +45,12 @@ CASES = [
     ("sphere", [0.0, 50.0, 0.0, 2 * math.pi, 0.75 * math.pi, 0.25 * math.pi]),
     ("sphere", [6.0, 8.0, math.pi / 6, math.pi / 3, 0.0, math.pi]),
     ("sphere", [0.0, 410.0, 0.0, 2 * math.pi, 0.5 * math.pi, 0.5 * math.pi]),
+    ("polyhedron", [0.0, 2 * math.pi, 8, 6, -20, -10, -5, 5, 10, 20, 0, 5, 5, 3, 0, 0, 10, 15, 15, 12, 12, 8]),
+    ("polyhedron", [0.0, math.pi / 4, 5, 8, -30, -20, -10, 0, 10, 20, 30, 40, 30, 30, 0, 0, 0, 0, 40, 40,
+                    70, 70, 70, 40, 40, 80, 80, 60]),
+    ("polyhedron", [0.5, 4.0, 8, 6, -20, -10, -5, 5, 10, 20, 0, 5, 5, 3, 0, 0, 10, 15, 15, 12, 12, 8]),
+    ("polyhedron", [-0.5, 1.2, 6, 4, -10, 0, 0, 10, 2, 2, 4, 4, 8, 8, 12, 12]),
+    ("polyhedron", [0.0, math.pi, 2, 2, -1, 1, 1, 1, 2, 2]),
 ]
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 

@@ -92,6 +92,8 @@ static int cost_class(int type)
   case kCone:
     return 4;
   case kPolycone:
+  case kPolyhedron:
+  case kSphere:
     return 8;
   default:
     return 16;

@@ -5,6 +5,7 @@
 #include "dev_cone.h"
 #include "dev_orb.h"
 #include "dev_polycone.h"
+#include "dev_polyhedron.h"
 #include "dev_sphere.h"
 #include "dev_trapezoid.h"
 #include "dev_tube.h"
@@ -17,7 +18,8 @@
   X(kTrapezoid, TrapezoidData, trap, trap)                                                                           \
   X(kOrb, OrbData, orb, orb)                                                                                         \
   X(kPolycone, PolyconeData, pcon, pcon)                                                                             \
-  X(kSphere, SphereData, sph, sph)
+  X(kSphere, SphereData, sph, sph)                                                                                   \
+  X(kPolyhedron, PolyhedronData, phed, phed)
 
 namespace vgmi {
 

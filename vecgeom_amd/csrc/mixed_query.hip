@@ -35,7 +35,10 @@ __global__ __launch_bounds__(kMixBlock) void k_query_mixed(Data s, const double 
   const Vec3<float> pf(float(p.x), float(p.y), float(p.z));
   const float sin32  = S::template SafeIn<float>(s, pf);
   const float sout32 = S::template SafeOut<float>(s, pf);
-  const bool near    = vabs(sin32) < kBand || vabs(sout32) < kBand;
+  // A consistent FP32 state has safeties of opposite sign (inside: SafetyToIn < 0 < SafetyToOut,
+  // outside: the reverse). Equal signs mean the FP32 tolerance bands collapsed onto the surface
+  // (a point within FP32 rounding of it is then both inside and outside): treat as near.
+  const bool near = vabs(sin32) < kBand || vabs(sout32) < kBand || ((sin32 < 0.f) == (sout32 < 0.f));
   if constexpr (OP == kOpContains) {
     const bool r = near ? S::template Contains<double>(s, p) : S::template Contains<float>(s, pf);
     __builtin_nontemporal_store(uint8_t(r), (uint8_t *)out + i);

@@ -109,6 +109,22 @@ struct SphereData {
   int32_t wedgeLtPi;              // raw dphi < pi (Wedge kernels)
 };
 
+// Polyhedron (VecGeom/volumes/PolyhedronStruct.h:45-60): z segments of quadrilateral
+// plane sets. The z planes, radii, same-z flags, phi-section vectors, a 6-entry header per
+// segment (outer, phi, inner: offset and count) and 32 doubles per quadrilateral (plane
+// normal and distance, four side planes, four corners) live in the variable-length block.
+struct PolyhedronData {
+  int32_t sides, nz, nseg;
+  int32_t hasPhiCutout, hasLargePhiCutout, wedgeLtPi;
+  int32_t zsOff, rminOff, rmaxOff, sameZOff, phiSecOff, segOff;
+  const double *base;
+  double phiStart, phiDelta, boundingOffset;
+  double w1x, w1y, w2x, w2y, n1x, n1y, n2x, n2y; // phi wedge
+  TubeData bounding;                             // bounding tube (UniversalTube kernels)
+  double ext[6];                                 // extent lo[3], hi[3]
+};
+constexpr int kQuadDoubles = 32;
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -123,8 +139,12 @@ struct ShapeRecord {
     ConeData cone;
     PolyconeData pcon;
     SphereData sph;
+    PolyhedronData phed;
     double raw[64];
   } u;
 };
+static_assert(sizeof(PolyhedronData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
+static_assert(sizeof(SphereData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
+static_assert(sizeof(ConeData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 
 } // namespace vgmi

@@ -428,6 +428,226 @@ static void SphereExtent(SphereData const &s, double lo[3], double hi[3])
   }
 }
 
+// ---------------------------------------------------------------- Polyhedron
+// VecGeom/volumes/PolyhedronStruct.h:91-103 and :248-429 (segments, phi sections, bounding
+// tube, vertices), source/Quadrilaterals.cpp:36-110 (plane of each quadrilateral from its
+// first three distinct corners, side planes), source/Planes.cpp:30-52.
+namespace {
+struct HV {
+  double v[3];
+  double &operator[](int i) { return v[i]; }
+  double operator[](int i) const { return v[i]; }
+};
+inline HV hv(double x, double y, double z) { return HV{{x, y, z}}; }
+inline double hdot(HV const &a, HV const &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline HV hsub(HV const &a, HV const &b) { return hv(a[0] - b[0], a[1] - b[1], a[2] - b[2]); }
+inline HV hscale(HV const &a, double s) { return hv(a[0] * s, a[1] * s, a[2] * s); }
+inline HV hcross(HV const &l, HV const &r)
+{
+  return hv(l[1] * r[2] - l[2] * r[1], l[2] * r[0] - l[0] * r[2], l[0] * r[1] - l[1] * r[0]);
+}
+inline double hmag(HV const &a) { return std::sqrt(hdot(a, a)); }
+inline HV hnormalized(HV const &a) { return hscale(a, 1. / hmag(a)); }
+inline HV hfix(HV a)
+{
+  for (int i = 0; i < 3; ++i)
+    if (std::fabs(a[i]) < hTol) a[i] = 0.0;
+  return a;
+}
+inline bool htolEqual(HV const &a, HV const &b)
+{
+  return std::fabs(a[0] - b[0]) < hTol && std::fabs(a[1] - b[1]) < hTol && std::fabs(a[2] - b[2]) < hTol;
+}
+inline double hnormAngle(double a) { return a + 2. * hPi * (int(a < 0) - int(a / (2. * hPi))); }
+
+// One quadrilateral into 32 doubles: normal, distance, 4 x (side normal, side distance), 4 corners.
+void PutQuad(double *q, HV const &c0, HV const &c1, HV const &c2, HV const &c3)
+{
+  HV const cs[4] = {c0, c1, c2, c3};
+  HV ch[4];
+  ch[0]        = cs[0];
+  int assigned = 1;
+  for (int t = 1; t < 4; ++t) {
+    bool chosen = true;
+    for (int j = 0; j < assigned; ++j)
+      if (htolEqual(ch[j], cs[t])) {
+        chosen = false;
+        break;
+      }
+    if (chosen) ch[assigned++] = cs[t];
+  }
+  double a = ch[0][1] * (ch[1][2] - ch[2][2]) + ch[1][1] * (ch[2][2] - ch[0][2]) + ch[2][1] * (ch[0][2] - ch[1][2]);
+  double b = ch[0][2] * (ch[1][0] - ch[2][0]) + ch[1][2] * (ch[2][0] - ch[0][0]) + ch[2][2] * (ch[0][0] - ch[1][0]);
+  double c = ch[0][0] * (ch[1][1] - ch[2][1]) + ch[1][0] * (ch[2][1] - ch[0][1]) + ch[2][0] * (ch[0][1] - ch[1][1]);
+  double d = -ch[0][0] * (ch[1][1] * ch[2][2] - ch[2][1] * ch[1][2]) -
+             ch[1][0] * (ch[2][1] * ch[0][2] - ch[0][1] * ch[2][2]) -
+             ch[2][0] * (ch[0][1] * ch[1][2] - ch[1][1] * ch[0][2]);
+  HV n             = hv(a, b, c);
+  const double inv = 1. / hmag(n);
+  n                = hscale(n, inv);
+  d *= inv;
+  n = hfix(n);
+  q[0] = n[0], q[1] = n[1], q[2] = n[2], q[3] = d;
+  for (int j = 0; j < 4; ++j) {
+    HV const &s0 = cs[j], &s1 = cs[(j + 1) % 4];
+    double *sp   = q + 4 + 4 * j;
+    if (!htolEqual(s1, s0)) {
+      const HV sv       = hfix(hnormalized(hcross(n, hsub(s1, s0))));
+      const double invl = 1. / hmag(sv);
+      sp[0] = invl * sv[0], sp[1] = invl * sv[1], sp[2] = invl * sv[2];
+      sp[3] = invl * -hdot(sv, s0);
+    } else {
+      sp[0] = sp[1] = sp[2] = sp[3] = 0.0;
+    }
+  }
+  for (int j = 0; j < 4; ++j)
+    for (int k = 0; k < 3; ++k) q[20 + 3 * j + k] = cs[j][k];
+}
+inline void FlipQuad(double *q)
+{
+  for (int k = 0; k < 4; ++k) q[k] = -q[k];
+}
+} // namespace
+
+// source/UnplacedPolyhedron.cpp:273-303 (corners at rmin/rmax of every plane and phi step)
+static void PolyhedronExtent(PolyhedronData const &P, std::vector<double> const &extra, double lo[3], double hi[3])
+{
+  for (int i = 0; i < 3; ++i) lo[i] = 1.7976931348623157e308, hi[i] = -1.7976931348623157e308;
+  const double *b      = extra.data();
+  const double sidePhi = P.phiDelta / P.sides;
+  const double conv    = 1. / std::cos(0.5 * sidePhi);
+  for (int iphi = 0; iphi <= P.sides; ++iphi) {
+    const double phi  = P.phiStart + iphi * sidePhi;
+    const double corx = conv * std::cos(phi), cory = conv * std::sin(phi);
+    for (int k = 0; k < P.nz; ++k) {
+      const double rn = b[P.rminOff + k], rx = b[P.rmaxOff + k], zk = b[P.zsOff + k];
+      const double pa[3] = {rn * corx, rn * cory, zk}, pb[3] = {rx * corx, rx * cory, zk};
+      for (int i = 0; i < 3; ++i) lo[i] = std::min(lo[i], pa[i]), hi[i] = std::max(hi[i], pa[i]);
+      for (int i = 0; i < 3; ++i) lo[i] = std::min(lo[i], pb[i]), hi[i] = std::max(hi[i], pb[i]);
+    }
+  }
+}
+
+int MakePolyhedron(double const *a, int n, ShapeRecord &r, std::vector<double> &extra, std::string &err)
+{
+  if (n < 4) return err = "polyhedron needs sphi dphi nsides nz z[nz] rmin[nz] rmax[nz]", -1;
+  const int sides = int(a[2]), nz = int(a[3]);
+  if (sides < 1 || double(sides) != a[2] || nz < 2 || double(nz) != a[3] || n < 4 + 3 * nz)
+    return err = "polyhedron needs nsides >= 1, nz >= 2 and 3*nz plane values", -1;
+  const double phiStart0 = a[0], phiDelta0 = a[1];
+  if (!(phiDelta0 > 0)) return err = "polyhedron dphi must be positive", -1;
+  const double *z = a + 4, *rmn = a + 4 + nz, *rmx = a + 4 + 2 * nz;
+  for (int i = 0; i < nz; ++i) {
+    if (rmn[i] < 0 || rmx[i] < rmn[i]) return err = "polyhedron needs 0 <= rmin <= rmax at every plane", -1;
+    if (i > 0 && z[i] < z[i - 1]) return err = "polyhedron z planes must not decrease", -1;
+  }
+  r                 = ShapeRecord();
+  r.type            = kPolyhedron;
+  PolyhedronData &P = r.u.phed;
+  P.sides           = sides;
+  P.nz              = nz;
+  P.nseg            = nz - 1;
+  P.hasPhiCutout      = phiDelta0 < 2. * hPi;
+  P.hasLargePhiCutout = phiDelta0 < hPi;
+  P.phiStart          = hnormAngle(phiStart0);
+  P.phiDelta          = phiDelta0 > 2. * hPi ? 2. * hPi : phiDelta0;
+  // wedge from (phiDelta, phiStart), source/Wedge_Evolution.cpp:100-108
+  P.w1x       = std::cos(P.phiStart);
+  P.w1y       = std::sin(P.phiStart);
+  P.w2x       = std::cos(P.phiStart + P.phiDelta);
+  P.w2y       = std::sin(P.phiStart + P.phiDelta);
+  P.n1x       = -std::sin(P.phiStart);
+  P.n1y       = std::cos(P.phiStart);
+  P.n2x       = std::sin(P.phiStart + P.phiDelta);
+  P.n2y       = -std::cos(P.phiStart + P.phiDelta);
+  P.wedgeLtPi = P.phiDelta < hPi;
+
+  extra.clear();
+  auto put = [&](double v) { extra.push_back(v); };
+  P.zsOff = 0;
+  for (int i = 0; i < nz; ++i) put(z[i]);
+  P.rminOff = int(extra.size());
+  for (int i = 0; i < nz; ++i) put(rmn[i]);
+  P.rmaxOff = int(extra.size());
+  for (int i = 0; i < nz; ++i) put(rmx[i]);
+  P.sameZOff = int(extra.size());
+  for (int i = 0; i < nz; ++i) put((i > 0 && i < nz - 1 && z[i] == z[i + 1]) ? 1.0 : 0.0);
+
+  const double ps = hnormAngle(phiStart0), pd = phiDelta0 > 2. * hPi ? 2. * hPi : phiDelta0;
+  const double sidePhi = pd / sides;
+  const bool cut       = P.hasPhiCutout != 0;
+  auto getPhi = [&](int side) {
+    if (!cut && side == sides) side = 0;
+    return hnormAngle(ps + side * sidePhi);
+  };
+  P.phiSecOff = int(extra.size());
+  std::vector<HV> phiSec(sides + 1);
+  for (int i = 0; i < sides + 1; ++i) {
+    const double ph = getPhi(i);
+    const HV corner = hfix(hnormalized(hv(1. * std::cos(ph), 1. * std::sin(ph), 0)));
+    phiSec[i]       = hcross(hnormalized(corner), hv(0, 0, -1));
+    put(phiSec[i][0]), put(phiSec[i][1]), put(phiSec[i][2]);
+  }
+  const double cosHalf = std::cos(0.5 * sidePhi);
+  double innerR = 1.7976931348623157e308, outerR = -1.7976931348623157e308;
+  for (int i = 0; i < nz; ++i) {
+    if (rmn[i] < innerR) innerR = rmn[i];
+    if (rmx[i] > outerR) outerR = rmx[i];
+  }
+  outerR /= cosHalf;
+  const double bz     = 0.5 * (z[nz - 1] - z[0]) + hTol;
+  const double phiTol = 100 * hTol;
+  const double bt[5]  = {innerR - hHTol, outerR + hHTol, bz, !cut ? 0 : ps - phiTol, !cut ? 2. * hPi : pd + 2 * phiTol};
+  ShapeRecord tr;
+  if (MakeTube(bt, 5, tr, err) != 0) return -1;
+  P.bounding       = tr.u.tube;
+  P.boundingOffset = 0.5 * (z[0] + z[nz - 1]);
+
+  auto vertex = [&](const double *rr, int i, int j) {
+    if (!cut && j == sides) j = 0;
+    const double ph = getPhi(j), rad = rr[i] / cosHalf;
+    return hfix(hv(rad * std::cos(ph), rad * std::sin(ph), z[i]));
+  };
+  // segment headers, then quadrilaterals
+  P.segOff        = int(extra.size());
+  const int nseg  = nz - 1;
+  extra.resize(extra.size() + 6 * nseg, 0.0);
+  for (int i = 0; i < nseg; ++i) {
+    const bool hasInner = rmn[i] > 0 || rmn[i + 1] > 0;
+    const int nOuter    = (z[i] == z[i + 1] && rmx[i] == rmx[i + 1]) ? 0 : sides;
+    const int nPhi      = cut ? ((z[i] == z[i + 1]) ? 0 : 2) : 0;
+    const int nInner    = (hasInner && !(z[i] == z[i + 1] && rmn[i] == rmn[i + 1])) ? sides : 0;
+    const int oOff = int(extra.size()), pOff = oOff + kQuadDoubles * nOuter, iOff = pOff + kQuadDoubles * nPhi;
+    extra.resize(iOff + kQuadDoubles * nInner, 0.0);
+    double *hd = &extra[P.segOff + 6 * i];
+    hd[0] = oOff, hd[1] = nOuter, hd[2] = pOff, hd[3] = nPhi, hd[4] = iOff, hd[5] = nInner;
+    for (int is = 0; is < nOuter; ++is) {
+      double *q = &extra[oOff + kQuadDoubles * is];
+      PutQuad(q, vertex(rmx, i, is), vertex(rmx, i, is + 1), vertex(rmx, i + 1, is + 1), vertex(rmx, i + 1, is));
+      const HV c = vertex(rmx, i, is);
+      if (q[0] * c[0] + q[1] * c[1] < 0) FlipQuad(q);
+    }
+    for (int is = 0; is < nInner; ++is) {
+      double *q = &extra[iOff + kQuadDoubles * is];
+      PutQuad(q, vertex(rmn, i, is), vertex(rmn, i, is + 1), vertex(rmn, i + 1, is + 1), vertex(rmn, i + 1, is));
+      const HV c = vertex(rmn, i, is);
+      if (q[0] * c[0] + q[1] * c[1] < 0) FlipQuad(q);
+    }
+    if (cut && nPhi == 2) {
+      double *q0 = &extra[pOff];
+      PutQuad(q0, vertex(rmn, i, 0), vertex(rmn, i + 1, 0), vertex(rmx, i + 1, 0), vertex(rmx, i, 0));
+      if (hdot(hv(q0[0], q0[1], q0[2]), phiSec[0]) > 0) FlipQuad(q0);
+      double *q1 = &extra[pOff + kQuadDoubles];
+      PutQuad(q1, vertex(rmx, i, sides), vertex(rmx, i + 1, sides), vertex(rmn, i + 1, sides),
+              vertex(rmn, i, sides));
+      if (hdot(hv(q1[0], q1[1], q1[2]), phiSec[sides]) < 0) FlipQuad(q1);
+    }
+  }
+  P.base = extra.data();
+  PolyhedronExtent(P, extra, P.ext, P.ext + 3);
+  return 0;
+}
+
 int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::vector<double> &extra,
                     std::string &err)
 {
@@ -447,6 +667,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakePolycone(params, nparams, r, extra, err);
   case kSphere:
     return MakeSphere(params, nparams, r, err);
+  case kPolyhedron:
+    return MakePolyhedron(params, nparams, r, extra, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -456,6 +678,7 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
 void SetShapeExtraBase(ShapeRecord &r, const double *device_base)
 {
   if (r.type == kPolycone) r.u.pcon.base = device_base;
+  if (r.type == kPolyhedron) r.u.phed.base = device_base;
 }
 
 void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
@@ -478,6 +701,9 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     return;
   case kSphere:
     SphereExtent(r.u.sph, lo, hi);
+    return;
+  case kPolyhedron:
+    for (int i = 0; i < 3; ++i) lo[i] = r.u.phed.ext[i], hi[i] = r.u.phed.ext[3 + i];
     return;
   case kPolycone:
     lo[0] = lo[1] = -r.u.pcon.rmax;

@@ -68,7 +68,8 @@ def near_surface(g, n, lo, hi, contains_fn, dist_out_fn, band=1e-3):
 # Batched shape set evaluated in one launch (the shapes implemented so far; each
 # entry is (kind, constructor parameters)). Tube: rmin 2, rmax 8, dz 10, phi 30-270 deg;
 # polycone with 10 z planes; general trapezoid; cone with an inner radius and slopes;
-# sphere shell with theta (30-130 deg) and phi (20-290 deg) cuts.
+# 8-sided polyhedron with 6 z planes (inner radii in the middle planes); sphere shell
+# with theta (30-130 deg) and phi (20-290 deg) cuts.
 def mix_shapes():
     r = np.radians
     return [
@@ -79,6 +80,8 @@ def mix_shapes():
                       -50, -40, -25, -10, 0, 5, 20, 30, 40, 50,
                       0, 5, 5, 8, 8, 3, 3, 10, 10, 0,
                       20, 25, 25, 30, 40, 40, 30, 35, 35, 20]),
+        ("polyhedron", [0.0, float(2 * np.pi), 8, 6, -20, -10, -5, 5, 10, 20, 0, 5, 5, 3, 0, 0,
+                        10, 15, 15, 12, 12, 8]),
         ("trapezoid", [60.0, float(r(20)), float(r(5)), 40.0, 30.0, 40.0, float(r(10)), 16.0, 10.0, 14.0,
                        float(r(10))]),
         ("sphere", [5.0, 10.0, float(r(20)), float(r(270)), float(r(30)), float(r(100))]),
