@@ -123,10 +123,40 @@ Tube::Tube(double rmin_, double rmax_, double dz, double sphi_, double dphi_)
   maxVal    = std::max(rmax, z);
 }
 
+// source/UnplacedTube.cpp:214-262
 void Tube::Extent(V3 &lo, V3 &hi) const
 {
   lo = V3(-rmax, -rmax, -z);
   hi = V3(rmax, rmax, z);
+  if (dphi == kTwoPi) return;
+  PhiExtent(wedge, rmax, rmin, sphi, dphi, lo, hi);
+}
+
+void PhiExtent(Wedge const &w, double rOut, double rIn, double sphi, double dphi, V3 &lo, V3 &hi)
+{
+  const double Rin = 0.5 * (rOut + rIn);
+  const bool o0    = !w.Contains(V3(Rin, 0, 0));
+  const bool o90   = !w.Contains(V3(0, Rin, 0));
+  const bool o180  = !w.Contains(V3(-Rin, 0, 0));
+  const bool o270  = !w.Contains(V3(0, -Rin, 0));
+  if (!(o0 || o90 || o180 || o270)) return;
+  const double Cx = rOut * std::cos(sphi), Dx = rOut * std::cos(sphi + dphi);
+  const double Cy = rOut * std::sin(sphi), Dy = rOut * std::sin(sphi + dphi);
+  if (o0) hi.x = std::max(Cx, Dx);
+  if (o90) hi.y = std::max(Cy, Dy);
+  if (o180) lo.x = std::min(Cx, Dx);
+  if (o270) lo.y = std::min(Cy, Dy);
+  if (dphi >= kPi) return;
+  const double Ax = rIn * std::cos(sphi), Bx = rIn * std::cos(sphi + dphi);
+  const double Ay = rIn * std::sin(sphi), By = rIn * std::sin(sphi + dphi);
+  double t = std::max(Ax, Bx);
+  hi.x     = t > hi.x ? t : hi.x;
+  t        = std::max(Ay, By);
+  hi.y     = t > hi.y ? t : hi.y;
+  t        = std::min(Ax, Bx);
+  lo.x     = t < lo.x ? t : lo.x;
+  t        = std::min(Ay, By);
+  lo.y     = t < lo.y ? t : lo.y;
 }
 
 // TubeImplementation.h:58-99 PointInCyclicalSector (onSurface=false)

@@ -98,6 +98,8 @@ Cone::Cone(double rmin1_, double rmax1_, double rmin2_, double rmax2_, double dz
   // CalculateCached
   origRmax1 = rmax1;
   origRmax2 = rmax2;
+  origRmin1 = rmin1_;
+  origRmin2 = rmin2_;
   if (rmin1 == rmax1) rmax1 += kConeTolerance;
   if (rmin2 == rmax2) rmax2 += kConeTolerance;
   sqRmin1     = rmin1 * rmin1;
@@ -142,11 +144,15 @@ Cone::Cone(double rmin1_, double rmax1_, double rmin2_, double rmax2_, double dz
   tolOz      = dz + kHalfTolerance;
 }
 
+// source/UnplacedCone.cpp:149-213 (constructor radii, phi-reduced box)
 void Cone::Extent(V3 &lo, V3 &hi) const
 {
-  double r = std::max(origRmax1, origRmax2);
-  lo       = V3(-r, -r, -dz);
-  hi       = V3(r, r, dz);
+  const double mx = origRmax1 > origRmax2 ? origRmax1 : origRmax2;
+  const double mn = origRmin1 > origRmin2 ? origRmin2 : origRmin1;
+  lo              = V3(-mx, -mx, -dz);
+  hi              = V3(mx, mx, dz);
+  if (dphi == kTwoPi) return;
+  PhiExtent(wedge, mx, mn, sphi, dphi, lo, hi);
 }
 
 // ConeUtilities.h:118-135 GetRadiusOfConeAtPoint

@@ -32,14 +32,40 @@ Polycone::Polycone(double sphi_, double dphi_, int nz, const double *z, const do
   }
   zlo = zs.front();
   zhi = zs.back();
+  // fStartPhi / fDeltaPhi: SetAndCheckDPhiAngle, SetAndCheckSPhiAngle, then the open/closed
+  // conversion (PolyconeStruct.h:152-184, :355-390)
+  deltaPhi = dphi_;
+  startPhi = sphi_;
+  if (dphi_ >= kTwoPi - 0.5 * kAngTolerance) {
+    deltaPhi = kTwoPi;
+    startPhi = 0;
+  } else if (dphi_ > 0) {
+    deltaPhi = dphi_;
+  }
+  startPhi = sphi_ < 0 ? kTwoPi - std::fmod(std::fabs(sphi_), kTwoPi) : std::fmod(sphi_, kTwoPi);
+  if (startPhi + deltaPhi > kTwoPi) startPhi -= kTwoPi;
+  if (dphi_ <= 0. || dphi_ > kTwoPi - kTolerance) {
+    startPhi = 0;
+    deltaPhi = kTwoPi;
+  } else {
+    startPhi = sphi_;
+    while (startPhi < 0) startPhi += kTwoPi;
+  }
 }
 
+// source/UnplacedPolycone.cpp:865-898: z from the planes, x/y from the extent of the cone
+// (minR, maxR, minR, maxR, 1, fStartPhi, fDeltaPhi) over the sections' constructor radii
 void Polycone::Extent(V3 &lo, V3 &hi) const
 {
-  double r = 0;
-  for (auto const &c : secs) r = std::max(r, std::max(c.rmax1, c.rmax2));
-  lo = V3(-r, -r, zlo);
-  hi = V3(r, r, zhi);
+  double maxR = 0, minR = kInfLength;
+  for (auto const &c : secs) {
+    maxR = std::max(maxR, std::max(c.origRmax1, c.origRmax2));
+    minR = std::min(minR, std::min(c.origRmin1, c.origRmin2));
+  }
+  const Cone tmp(minR, maxR, minR, maxR, 1, startPhi, deltaPhi);
+  tmp.Extent(lo, hi);
+  lo.z = zlo;
+  hi.z = zhi;
 }
 
 // PolyconeStruct.h:299-308 GetSectionIndex

@@ -24,6 +24,11 @@ public:
 };
 
 // Phi wedge: VecGeom/volumes/Wedge_Evolution.h, ctor source/Wedge_Evolution.cpp:17-128
+struct Wedge;
+// Box reduced to the phi range of a wedge: source/UnplacedTube.cpp:220-262 (the same rule is used by
+// the cone, polycone and sphere extents). C, D at rOut and A, B at rIn, at the start and end angles.
+void PhiExtent(Wedge const &w, double rOut, double rIn, double sphi, double dphi, V3 &lo, V3 &hi);
+
 struct Wedge {
   double dphi = 0;
   V3 along1, along2, normal1, normal2;
@@ -155,7 +160,7 @@ public:
 class Cone : public Shape {
 public:
   double rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi;
-  double origRmax1, origRmax2;
+  double origRmax1, origRmax2, origRmin1, origRmin2; // constructor values (_frmax1.. in ConeStruct)
   double a1x, a1y, a2x, a2y;
   Wedge wedge;
   double innerSlope, outerSlope, innerOffset, outerOffset, innerTol, outerTol;
@@ -195,6 +200,7 @@ public:
   std::vector<double> shift;     // per section
   std::vector<Cone> secs;
   double rmaxExtent = 0, zlo = 0, zhi = 0;
+  double startPhi = 0, deltaPhi = 0; // fStartPhi, fDeltaPhi (PolyconeStruct.h:152-184)
   Polycone(double sphi, double dphi, int nz, const double *z, const double *rmin, const double *rmax);
   int NSec() const { return int(secs.size()); }
   int SectionIndex(double z) const;

@@ -13,7 +13,8 @@ Expectation kinds: "approx" (ApproxEqual, test/unit_tests/ApproxEqual.h),
 "anyof" (a disjunction of ApproxEqual), "inside" (EInside enum),
 "negative" (assert(Dist < 0)), "nonpositive" (assert(Dist <= 0),
 TestWrongSidePoint), "absdiff" (assert(fabs(s->M(..) - v) < tol)), "bool"
-(assert(s->Contains(p)) / assert(!s->Contains(p))).
+(assert(s->Contains(p)) / assert(!s->Contains(p))), "extent" (s.Extent(lo, hi) then
+assert(ApproxEqual(lo, Vec_t(..))), method "Extent").
 
 Run in a container that holds the reference tree:
     python tests/golden/extract_kats.py /root/reference
@@ -169,7 +170,7 @@ METHODS = "DistanceToIn|DistanceToOut|SafetyToIn|SafetyToOut|Inside|Contains"
 
 def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
     text = open(path).read()
-    vecs, scalars, shapes, pending, arrays = {}, {}, {}, {}, {}
+    vecs, scalars, shapes, pending, arrays, extents = {}, {}, {}, {}, {}, {}
     global CURRENT_VECS
     CURRENT_VECS = vecs
     records = []
@@ -294,11 +295,31 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                 except (Unsupported, IndexError):
                     pass
             continue
+        # shape->Extent(lo, hi): remember which vector holds which corner
+        m = re.fullmatch(r"(\w+)\s*(?:\.|->)\s*Extent\s*\(\s*(\w+)\s*,\s*(\w+)\s*\)", st)
+        if m:
+            if m.group(1) in shapes:
+                extents[m.group(2)] = (m.group(1), "lo")
+                extents[m.group(3)] = (m.group(1), "hi")
+            else:
+                extents.pop(m.group(2), None)
+                extents.pop(m.group(3), None)
+            continue
         # assertions on a stored result
         m = re.match(r"assert\s*\((.*)\)$", st)
         if m:
             body = m.group(1).strip()
             rec = None
+            me = re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,\s*(.+)\)", body)
+            if me and me.group(1) in extents:
+                try:
+                    sh, which = extents[me.group(1)]
+                    v = eval_vec(me.group(2), vecs, scalars, vec_type)
+                    records.append({"line": "%s:%d" % (fname, ln), "shape": shapes[sh], "method": "Extent",
+                                    "p": None, "d": None, "expect": {"kind": "extent", "which": which, "value": v}})
+                except Unsupported:
+                    pass
+                continue
             # fabs(shape->Method(args) - value) < tol, fabs(shape->Method(args)) < tol
             mq = re.fullmatch(r"(?:std::)?fabs\s*\(\s*\(?\s*(\w+)\s*(?:\.|->)\s*(%s)\s*\(([^()]*(?:\([^()]*\))?[^()]*)\)"
                               r"\s*(?:-\s*([-+\w.()*/ ]+?))?\s*\)?\s*\)\s*<\s*(\w+)" % METHODS, body)

@@ -44,6 +44,9 @@ def check(rec, got):
         return abs(got - e["value"]) < e["tol"]
     if e["kind"] == "bool":
         return bool(got) == e["value"]
+    if e["kind"] == "extent":  # got = (lo, hi); ApproxEqual on each component
+        v = got[0] if e["which"] == "lo" else got[1]
+        return all(approx_equal(float(a), float(b)) for a, b in zip(v, e["value"]))
     raise ValueError(e)
 
 

@@ -73,7 +73,7 @@ struct OrbData {
 // slopes, offsets, apex data and tolerances of CalculateCached().
 struct ConeData {
   double rmin1, rmax1, rmin2, rmax2, dz, sphi, dphi;
-  double origRmax1, origRmax2;
+  double origRmax1, origRmax2, origRmin1, origRmin2; // constructor radii (extent)
   double a1x, a1y, a2x, a2y;             // phi-plane vectors of the normalised angles
   double w1x, w1y, w2x, w2y;             // wedge along vectors (raw input angles)
   double n1x, n1y, n2x, n2y;             // wedge inward normals (raw input angles)
@@ -93,7 +93,7 @@ struct PolyconeData {
   int32_t shiftOff; // double[nsec]
   int32_t zsOff;    // double[nsec + 1]
   const double *base;
-  double rmax, zlo, zhi;
+  double ext[6]; // extent lo[3], hi[3]
 };
 
 // Sphere (VecGeom/volumes/SphereStruct.h:25-266): shell radii, the normalised phi

@@ -10,6 +10,62 @@ static constexpr double hTol  = 1e-9;
 static constexpr double hHTol = 0.5e-9;
 static constexpr double hPi   = 3.14159265358979323846;
 
+// Wedge::Contains for a wedge given by its along vectors (Wedge_Evolution.h:293-336, no surface test)
+static bool WedgeContainsXY(double a1x, double a1y, double a2x, double a2y, bool ltPi, double x, double y)
+{
+  const double startCheck = (-x * a1y + y * a1x);
+  const double endCheck   = (-a2x * y + a2y * x);
+  bool out                = startCheck < 0.;
+  if (ltPi)
+    out = out || (endCheck < 0.);
+  else
+    out = out && (endCheck < 0.);
+  return !out;
+}
+
+// Box reduced to the phi range (source/UnplacedTube.cpp:220-262; the cone, polycone and sphere
+// extents use the same rule): C, D at rOut and A, B at rIn, at the start and end angles.
+static void PhiExtent(double a1x, double a1y, double a2x, double a2y, bool ltPi, double rOut, double rIn, double sphi,
+                      double dphi, double lo[3], double hi[3])
+{
+  const double Rin = 0.5 * (rOut + rIn);
+  const bool o0    = !WedgeContainsXY(a1x, a1y, a2x, a2y, ltPi, Rin, 0);
+  const bool o90   = !WedgeContainsXY(a1x, a1y, a2x, a2y, ltPi, 0, Rin);
+  const bool o180  = !WedgeContainsXY(a1x, a1y, a2x, a2y, ltPi, -Rin, 0);
+  const bool o270  = !WedgeContainsXY(a1x, a1y, a2x, a2y, ltPi, 0, -Rin);
+  if (!(o0 || o90 || o180 || o270)) return;
+  const double Cx = rOut * std::cos(sphi), Dx = rOut * std::cos(sphi + dphi);
+  const double Cy = rOut * std::sin(sphi), Dy = rOut * std::sin(sphi + dphi);
+  if (o0) hi[0] = std::max(Cx, Dx);
+  if (o90) hi[1] = std::max(Cy, Dy);
+  if (o180) lo[0] = std::min(Cx, Dx);
+  if (o270) lo[1] = std::min(Cy, Dy);
+  if (dphi >= hPi) return;
+  const double Ax = rIn * std::cos(sphi), Bx = rIn * std::cos(sphi + dphi);
+  const double Ay = rIn * std::sin(sphi), By = rIn * std::sin(sphi + dphi);
+  double t = std::max(Ax, Bx);
+  hi[0]    = t > hi[0] ? t : hi[0];
+  t        = std::max(Ay, By);
+  hi[1]    = t > hi[1] ? t : hi[1];
+  t        = std::min(Ax, Bx);
+  lo[0]    = t < lo[0] ? t : lo[0];
+  t        = std::min(Ay, By);
+  lo[1]    = t < lo[1] ? t : lo[1];
+}
+
+// source/UnplacedCone.cpp:149-213
+static void ConeExtent(ConeData const &c, double lo[3], double hi[3])
+{
+  const double mx = c.origRmax1 > c.origRmax2 ? c.origRmax1 : c.origRmax2;
+  const double mn = c.origRmin1 > c.origRmin2 ? c.origRmin2 : c.origRmin1;
+  lo[0] = lo[1] = -mx;
+  hi[0] = hi[1] = mx;
+  lo[2]         = -c.dz;
+  hi[2]         = c.dz;
+  if (c.dphi == 2. * hPi) return;
+  PhiExtent(c.w1x, c.w1y, c.w2x, c.w2y, c.wedgeLtPi != 0, mx, mn, c.sphi, c.dphi, lo, hi);
+}
+
 // VecGeom/volumes/BoxStruct.h:26
 int MakeBox(double const *a, int n, ShapeRecord &r, std::string &err)
 {
@@ -188,6 +244,8 @@ int MakeCone(double const *a, int n, ShapeRecord &r, std::string &err)
   c.a2y = std::sin(c.sphi + c.dphi);
   c.origRmax1 = c.rmax1;
   c.origRmax2 = c.rmax2;
+  c.origRmin1 = a[0];
+  c.origRmin2 = a[2];
   if (c.rmin1 == c.rmax1) c.rmax1 += kConeTol;
   if (c.rmin2 == c.rmax2) c.rmax2 += kConeTol;
   c.sqRmin1     = c.rmin1 * c.rmin1;
@@ -292,11 +350,31 @@ int MakePolycone(double const *a, int n, ShapeRecord &r, std::vector<double> &ex
   P.zsOff = int(extra.size());
   extra.insert(extra.end(), zs.begin(), zs.end());
   P.base = nullptr;
-  double rm = 0;
-  for (auto const &c : cones) rm = std::max(rm, std::max(c.u.cone.rmax1, c.u.cone.rmax2));
-  P.rmax = rm;
-  P.zlo  = zs.front();
-  P.zhi  = zs.back();
+  // extent (source/UnplacedPolycone.cpp:865-898): z from the planes; x/y from the cone
+  // (minR, maxR, minR, maxR, 1, fStartPhi, fDeltaPhi) over the sections' constructor radii,
+  // with fStartPhi/fDeltaPhi as PolyconeStruct.h:152-184 leaves them
+  double maxR = 0, minR = 1.7976931348623157e308;
+  for (auto const &c : cones) {
+    maxR = std::max(maxR, std::max(c.u.cone.origRmax1, c.u.cone.origRmax2));
+    minR = std::min(minR, std::min(c.u.cone.origRmin1, c.u.cone.origRmin2));
+  }
+  const double sphi0 = a[0], dphi0 = a[1];
+  double startPhi, deltaPhi = dphi0;
+  if (dphi0 <= 0. || dphi0 > 2. * hPi - hTol) {
+    startPhi = 0;
+    deltaPhi = 2. * hPi;
+  } else {
+    if (dphi0 >= 2. * hPi - 0.5 * 1e-9) deltaPhi = 2. * hPi;
+    startPhi = sphi0;
+    while (startPhi < 0) startPhi += 2. * hPi;
+  }
+  const double tp[7] = {minR, maxR, minR, maxR, 1, startPhi, deltaPhi};
+  ShapeRecord tmp;
+  std::string e;
+  if (MakeCone(tp, 7, tmp, e) != 0) return err = "polycone extent: " + e, -1;
+  ConeExtent(tmp.u.cone, P.ext, P.ext + 3);
+  P.ext[2] = zs.front();
+  P.ext[5] = zs.back();
   (void)rmaxAll;
   return 0;
 }
@@ -687,12 +765,17 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
   case kBox:
     for (int i = 0; i < 3; ++i) lo[i] = -r.u.box.h[i], hi[i] = r.u.box.h[i];
     return;
-  case kTube:
-    lo[0] = lo[1] = -r.u.tube.rmax;
-    hi[0] = hi[1] = r.u.tube.rmax;
-    lo[2]         = -r.u.tube.z;
-    hi[2]         = r.u.tube.z;
+  case kTube: {
+    // source/UnplacedTube.cpp:214-262
+    TubeData const &t = r.u.tube;
+    lo[0] = lo[1] = -t.rmax;
+    hi[0] = hi[1] = t.rmax;
+    lo[2]         = -t.z;
+    hi[2]         = t.z;
+    if (t.dphi == 2. * hPi) return;
+    PhiExtent(t.a1x, t.a1y, t.a2x, t.a2y, t.wedgeLtPi != 0, t.rmax, t.rmin, t.sphi, t.dphi, lo, hi);
     return;
+  }
   case kTrapezoid:
     TrapezoidExtent(r.u.trap, lo, hi);
     return;
@@ -706,19 +789,11 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     for (int i = 0; i < 3; ++i) lo[i] = r.u.phed.ext[i], hi[i] = r.u.phed.ext[3 + i];
     return;
   case kPolycone:
-    lo[0] = lo[1] = -r.u.pcon.rmax;
-    hi[0] = hi[1] = r.u.pcon.rmax;
-    lo[2]         = r.u.pcon.zlo;
-    hi[2]         = r.u.pcon.zhi;
+    for (int i = 0; i < 3; ++i) lo[i] = r.u.pcon.ext[i], hi[i] = r.u.pcon.ext[3 + i];
     return;
-  case kCone: {
-    const double rm = std::max(r.u.cone.origRmax1, r.u.cone.origRmax2);
-    lo[0] = lo[1] = -rm;
-    hi[0] = hi[1] = rm;
-    lo[2] = -r.u.cone.dz;
-    hi[2] = r.u.cone.dz;
+  case kCone:
+    ConeExtent(r.u.cone, lo, hi);
     return;
-  }
   default:
     lo[0] = lo[1] = lo[2] = 0;
     hi[0] = hi[1] = hi[2] = 0;
