@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-nav", action="store_true")
     ap.add_argument("--no-box-tube", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="tile,block,blocks_per_cu for the batched launch (skips the autotuner)")
     return ap.parse_args()
 
 
@@ -102,7 +104,11 @@ def run_mix(args):
     batch = ShapeBatch(gpu, counts)
     outs = batch.alloc_outputs()
     tuning = None
-    if not args.no_tune:
+    if args.tuning:
+        tile, block, bpc = (int(x) for x in args.tuning.split(","))
+        batch.set_tuning(tile, block, bpc)
+        tuning = {"tile": tile, "block": block, "blocks_per_cu": bpc, "source": "--tuning"}
+    elif not args.no_tune:
         tuning, _ = autotune(batch, P, D, outs)
     batch.capture(P, D, outs)
     for _ in range(args.warmup):
@@ -116,7 +122,9 @@ def run_mix(args):
            "config": {"workload": "batched shape mix, one launch (HIP graph replay), all six methods",
                       "shapes": [[k, c] for (k, _), c in zip(shapes, counts)], "points": total,
                       "queries_per_step": queries, "near_surface_fraction": 0.5,
-                      "not_yet_in_mix": ["polyhedron", "sphere", "torus2", "tessellated"],
+                      "not_yet_in_mix": [k for k in ("box", "tube", "cone", "polycone", "polyhedron", "trapezoid",
+                                                      "sphere", "torus2", "orb", "tessellated")
+                                         if k not in {s for s, _ in shapes}],
                       "tuning": tuning or batch.info()},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_batch",

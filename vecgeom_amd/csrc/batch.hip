@@ -117,6 +117,16 @@ struct vgmi_batch {
   hipGraphExec_t graphExec = nullptr;
 };
 
+// Default points per tile: at least four tiles per resident block so the atomic queue can
+// balance the expensive shapes, within [256, 4096] and a multiple of 64.
+static int default_tile(vgmi_batch const *b)
+{
+  const int64_t blocks = int64_t(b->numCU) * b->blocksPerCU;
+  int64_t t            = b->total / std::max<int64_t>(1, 4 * blocks);
+  t                    = std::max<int64_t>(256, std::min<int64_t>(4096, t));
+  return int(t / 64 * 64);
+}
+
 static vgmi_status batch_build_tiles(vgmi_batch *b)
 {
   std::vector<Tile> tiles;
@@ -210,6 +220,7 @@ vgmi_status vgmi_batch_create(const vgmi_shape *const *shapes, const int64_t *co
     vgmi_batch_destroy(b);
     return set_error(VGMI_ERR_DEVICE, "vgmi_batch_create: %s", hipGetErrorString(e));
   }
+  b->tile = default_tile(b);
   if (vgmi_status s = batch_build_tiles(b)) {
     vgmi_batch_destroy(b);
     return s;
