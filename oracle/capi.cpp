@@ -36,6 +36,8 @@ Shape *MakeShape(int type, const double *a, int n)
     if (sides < 1 || nz < 2 || n < 4 + 3 * nz) return nullptr;
     return new Polyhedron(a[0], a[1], sides, nz, a + 4, a + 4 + nz, a + 4 + 2 * nz);
   }
+  case 7:
+    return n >= 5 ? new Torus2(a[0], a[1], a[2], a[3], a[4]) : nullptr;
   case 6:
     return n >= 6 ? new Sphere(a[0], a[1], a[2], a[3], a[4], a[5]) : nullptr;
   case 5:

@@ -311,3 +311,30 @@ public:
   void Extent(V3 &lo, V3 &hi) const override;
 };
 } // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Torus2
+// VecGeom/volumes/TorusStruct2.h:22-45, VecGeom/volumes/kernel/TorusImplementation2.h:14-660 (quartic
+// via the resolvent cubic, Newton polishing), bounding tube with UniversalTube kernels (the
+// VECGEOM_NO_SPECIALIZATION branch of DistanceToIn, :560-575).
+class Torus2 : public Shape {
+public:
+  double rmin, rmax, rtor, sphi, dphi, rmin2, rmax2, rtor2;
+  Wedge wedge;
+  Tube bounding{0, 1, 1, 0, kTwoPi};
+  Torus2(double rmin, double rmax, double rtor, double sphi, double dphi);
+  void ContainsInside(V3 const &p, bool forInside, bool notForDisk, bool &ci, bool &co) const;
+  double ToBoundary(V3 const &pt, V3 const &dir, double radius, bool out) const;
+  double DistSqrToTorusR(V3 const &p, V3 const &d, double dist) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override { bounding.Extent(lo, hi); }
+};
+// TorusImplementation2.h:14-190
+unsigned SolveCubic(double a, double b, double c, double *x);
+int SolveQuartic(double a, double b, double c, double d, double *x);
+} // namespace vgo

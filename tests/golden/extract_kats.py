@@ -414,6 +414,7 @@ SHAPES = {
     "polycone": ("TestPolycone.cpp", r"Polycone_t", None),
     "sphere": ("TestSphere.cpp", r"Sphere_t", (6,)),
     "polyhedron": ("TestPolyhedra.cpp", r"Polyhedra_t", "polyhedron"),
+    "torus2": ("TestTorus2.cpp", r"Torus_t", (5,)),
 }
 
 if __name__ == "__main__":

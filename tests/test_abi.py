@@ -68,3 +68,19 @@ def test_batch_rejects_mismatched_counts_before_native_call():
     box = make_shape("box", [1.0, 2.0, 3.0])
     with pytest.raises(ValueError):
         ShapeBatch([box], [10, 20])
+
+
+def test_product_extent_matches_reference_kats():
+    """Host-side extents of the product (shapes without a variable-length block) against the
+    Extent asserts of the reference's unit tests."""
+    from tests import kat_util
+    from vecgeom_amd import make_shape
+    fails, n = [], 0
+    for kind in ["box", "tube", "trapezoid", "cone", "orb", "sphere", "torus2"]:
+        for r in kat_util.load(kind):
+            if r["method"] == "Extent":
+                n += 1
+                got = make_shape(kind, r["shape"]).Extent()
+                if not kat_util.check(r, got):
+                    fails.append((kind, r["line"], got))
+    assert n >= 10 and not fails, fails

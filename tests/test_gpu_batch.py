@@ -7,6 +7,8 @@ import torch
 from oracle.pyoracle import OracleShape
 from vecgeom_amd import SOA3D, make_shape, workloads
 from vecgeom_amd.batch import METHODS, ShapeBatch
+from tests import gpu_util
+from tests.test_gpu_shapes import TOL_BY_KIND
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +43,11 @@ def test_batch_matches_oracle_and_single_launches():
             for m in METHODS:
                 ref = o.query(m, ps, dsl if m.startswith("Distance") else None)
                 got = out[m][off:off + n].cpu().numpy()
-                assert np.array_equal(ref, got), (kind, m, tuning)
+                if kind in TOL_BY_KIND and m not in ("Contains", "Inside"):
+                    bad, _, maxerr = gpu_util.compare_float(ref, got, TOL_BY_KIND[kind])
+                    assert bad == 0, (kind, m, tuning, maxerr)
+                else:
+                    assert np.array_equal(ref, got), (kind, m, tuning)
             off += n
 
 

@@ -51,7 +51,16 @@ CASES = [
     ("polyhedron", [0.5, 4.0, 8, 6, -20, -10, -5, 5, 10, 20, 0, 5, 5, 3, 0, 0, 10, 15, 15, 12, 12, 8]),
     ("polyhedron", [-0.5, 1.2, 6, 4, -10, 0, 0, 10, 2, 2, 4, 4, 8, 8, 12, 12]),
     ("polyhedron", [0.0, math.pi, 2, 2, -1, 1, 1, 1, 2, 2]),
+    ("torus2", [0.0, 10.0, 50.0, 0.0, 2 * math.pi]),
+    ("torus2", [2.0, 10.0, 50.0, 0.3, 2.0]),
+    ("torus2", [5.0, 10.0, 50.0, 1.0, 4.5]),
+    ("torus2", [100.0, 150.0, 200.0, 0.0, math.pi / 3]),
 ]
+# Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
+# from the host C library in the last bit, and the Newton polishing stops once a step is below
+# kTolerance, so torus results agree within a few kTolerance rather than bit for bit.
+TOL_BY_KIND = {"torus2": 1e-8}
+EXACT_FRACTION_BY_KIND = {"torus2": 0.5}
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 
 
@@ -69,9 +78,9 @@ def test_fp64_parity(kind, params):
         if m in ("Contains", "Inside"):
             assert np.array_equal(ref, got), (m, int((ref != got).sum()))
         else:
-            bad, exact, maxerr = gpu_util.compare_float(ref, got, TOL)
+            bad, exact, maxerr = gpu_util.compare_float(ref, got, TOL_BY_KIND.get(kind, TOL))
             assert bad == 0, (m, bad, maxerr)
-            assert exact >= 0.999 * len(ref), (m, exact, len(ref))
+            assert exact >= EXACT_FRACTION_BY_KIND.get(kind, 0.999) * len(ref), (m, exact, len(ref))
 
 
 def test_query_all_matches_single_methods():
@@ -86,7 +95,8 @@ def test_query_all_matches_single_methods():
         assert torch.equal(res[m], single), m
 
 
-@pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone"])
+@pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
+                                   "torus2"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape
@@ -94,6 +104,11 @@ def test_gpu_reference_kats(shape):
     fails = []
     for r in recs:
         s = make_shape(shape, r["shape"])
+        if r["method"] == "Extent":  # the product's extent (ABBoxes, sampling boxes)
+            got = s.Extent()
+            if not kat_util.check(r, got):
+                fails.append((r["line"], "Extent", r["expect"], got))
+            continue
         P = gpu_util.to_dev(kat_util.soa([r["p"]]))
         if r["method"].startswith("Distance"):
             got = getattr(s, r["method"])(P, gpu_util.to_dev(kat_util.soa([r["d"]])))

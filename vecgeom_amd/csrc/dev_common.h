@@ -71,6 +71,15 @@ VGMI_DEV R vabs(R a)
 VGMI_DEV double vsqrt(double a) { return __builtin_sqrt(a); }
 VGMI_DEV float vsqrt(float a) { return __builtin_sqrtf(a); }
 VGMI_DEV double vcopysign(double a, double b) { return __builtin_copysign(a, b); }
+// library transcendental functions (Torus2 cubic solver)
+VGMI_DEV double vcbrt(double a) { return ::cbrt(a); }
+VGMI_DEV float vcbrt(float a) { return ::cbrtf(a); }
+VGMI_DEV double vpow(double a, double b) { return ::pow(a, b); }
+VGMI_DEV float vpow(float a, float b) { return ::powf(a, b); }
+VGMI_DEV double vcos(double a) { return ::cos(a); }
+VGMI_DEV float vcos(float a) { return ::cosf(a); }
+VGMI_DEV double vatan2(double a, double b) { return ::atan2(a, b); }
+VGMI_DEV float vatan2(float a, float b) { return ::atan2f(a, b); }
 VGMI_DEV float vcopysign(float a, float b) { return __builtin_copysignf(a, b); }
 
 template <class R>

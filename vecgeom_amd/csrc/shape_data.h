@@ -125,6 +125,15 @@ struct PolyhedronData {
 };
 constexpr int kQuadDoubles = 32;
 
+// Torus2 (VecGeom/volumes/TorusStruct2.h:22-45): radii and squares, the phi wedge of the
+// raw angles and the bounding tube (rtor -/+ (rmax + kTolerance), half length rmax).
+struct TorusData {
+  double rmin, rmax, rtor, sphi, dphi, rmin2, rmax2, rtor2;
+  double w1x, w1y, w2x, w2y, n1x, n1y, n2x, n2y;
+  int32_t hasPhi, wedgeLtPi;
+  TubeData bounding;
+};
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -140,11 +149,13 @@ struct ShapeRecord {
     PolyconeData pcon;
     SphereData sph;
     PolyhedronData phed;
+    TorusData tor;
     double raw[64];
   } u;
 };
 static_assert(sizeof(PolyhedronData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 static_assert(sizeof(SphereData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
+static_assert(sizeof(TorusData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 static_assert(sizeof(ConeData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 
 } // namespace vgmi

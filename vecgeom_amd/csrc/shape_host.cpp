@@ -726,6 +726,34 @@ int MakePolyhedron(double const *a, int n, ShapeRecord &r, std::vector<double> &
   return 0;
 }
 
+// Torus2: TorusStruct2.h:36-43 and UnplacedTorus2.h:64-75 (wedge of the raw angles, bounding tube)
+int MakeTorus2(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 5) return err = "torus2 needs rmin rmax rtor sphi dphi", -1;
+  if (!(a[1] > 0 && a[2] > 0 && a[4] > 0) || a[0] < 0 || !(a[0] < a[1]))
+    return err = "torus2 needs 0 <= rmin < rmax, rtor > 0 and dphi > 0", -1;
+  r            = ShapeRecord();
+  r.type       = kTorus2;
+  TorusData &t = r.u.tor;
+  t.rmin = a[0], t.rmax = a[1], t.rtor = a[2], t.sphi = a[3], t.dphi = a[4];
+  t.rmin2 = t.rmin * t.rmin, t.rmax2 = t.rmax * t.rmax, t.rtor2 = t.rtor * t.rtor;
+  t.w1x       = std::cos(t.sphi);
+  t.w1y       = std::sin(t.sphi);
+  t.w2x       = std::cos(t.sphi + t.dphi);
+  t.w2y       = std::sin(t.sphi + t.dphi);
+  t.n1x       = -std::sin(t.sphi);
+  t.n1y       = std::cos(t.sphi);
+  t.n2x       = std::sin(t.sphi + t.dphi);
+  t.n2y       = -std::cos(t.sphi + t.dphi);
+  t.hasPhi    = t.dphi < 2. * hPi;
+  t.wedgeLtPi = t.dphi < hPi;
+  const double bt[5] = {t.rtor - t.rmax - hTol, t.rtor + t.rmax + hTol, t.rmax, t.sphi, t.dphi};
+  ShapeRecord tr;
+  if (MakeTube(bt, 5, tr, err) != 0) return -1;
+  t.bounding = tr.u.tube;
+  return 0;
+}
+
 int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r, std::vector<double> &extra,
                     std::string &err)
 {
@@ -747,6 +775,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeSphere(params, nparams, r, err);
   case kPolyhedron:
     return MakePolyhedron(params, nparams, r, extra, err);
+  case kTorus2:
+    return MakeTorus2(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -788,6 +818,14 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
   case kPolyhedron:
     for (int i = 0; i < 3; ++i) lo[i] = r.u.phed.ext[i], hi[i] = r.u.phed.ext[3 + i];
     return;
+  case kTorus2: {
+    // UnplacedTorus2::Extent is the bounding tube's extent (UnplacedTorus2.h:210)
+    ShapeRecord tr;
+    tr.type   = kTube;
+    tr.u.tube = r.u.tor.bounding;
+    ShapeExtent(tr, lo, hi);
+    return;
+  }
   case kPolycone:
     for (int i = 0; i < 3; ++i) lo[i] = r.u.pcon.ext[i], hi[i] = r.u.pcon.ext[3 + i];
     return;

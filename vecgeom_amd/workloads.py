@@ -69,7 +69,7 @@ def near_surface(g, n, lo, hi, contains_fn, dist_out_fn, band=1e-3):
 # entry is (kind, constructor parameters)). Tube: rmin 2, rmax 8, dz 10, phi 30-270 deg;
 # polycone with 10 z planes; general trapezoid; cone with an inner radius and slopes;
 # 8-sided polyhedron with 6 z planes (inner radii in the middle planes); sphere shell
-# with theta (30-130 deg) and phi (20-290 deg) cuts.
+# with theta (30-130 deg) and phi (20-290 deg) cuts; Torus2 with rtor 50, rmax 10.
 def mix_shapes():
     r = np.radians
     return [
@@ -85,6 +85,7 @@ def mix_shapes():
         ("trapezoid", [60.0, float(r(20)), float(r(5)), 40.0, 30.0, 40.0, float(r(10)), 16.0, 10.0, 14.0,
                        float(r(10))]),
         ("sphere", [5.0, 10.0, float(r(20)), float(r(270)), float(r(30)), float(r(100))]),
+        ("torus2", [0.0, 10.0, 50.0, 0.0, float(2 * np.pi)]),
         ("orb", [10.0]),
     ]
 
