@@ -134,13 +134,44 @@ VGMI_DEV double box_safety(AABBRec const &b, V3 const &p)
   return vmax(s0, vmax(s1, s2));
 }
 
-// BVH::CheckDaughterIntersections
+// Lower bound of the distance from a local point to a shape (zero or negative inside): the
+// largest signed distance to a half-space or cylinder containing the solid. -inf where no
+// cheap bound is used.
+VGMI_DEV double lower_bound_in(ShapeRecord const &s, V3 const &p)
+{
+  switch (s.type) {
+  case kBox:
+    return vmax(vabs(p.x) - s.u.box.h[0], vmax(vabs(p.y) - s.u.box.h[1], vabs(p.z) - s.u.box.h[2]));
+  case kTube: {
+    TubeData const &t = s.u.tube;
+    const double rho  = vsqrt(p.x * p.x + p.y * p.y);
+    return vmax(vmax(rho - t.rmax, t.rmin - rho), vabs(p.z) - t.z);
+  }
+  case kTrapezoid: {
+    TrapezoidData const &t = s.u.trap;
+    double b               = vabs(p.z) - t.dz;
+    for (int i = 0; i < 4; ++i) b = vmax(b, t.a[i] * p.x + t.b[i] * p.y + t.c[i] * p.z + t.d[i]);
+    return b;
+  }
+  case kOrb:
+    return vsqrt(p.Mag2()) - s.u.orb.r;
+  default:
+    return -kInfLength;
+  }
+}
+
+// BVH::CheckDaughterIntersections. `cap` (the step limit when the mother distance is finite,
+// else kInfLength) prunes daughters that cannot change the outcome of the step: one whose
+// distance exceeds both the current step and the limit ends the step at the limit either way.
+// A daughter is skipped when its box entry distance or its distance lower bound exceeds
+// cap + 1e-6, which leaves every distance below the limit, and the visiting order, unchanged.
 VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V3 const &ld, double &step, int last,
-                              int &hit, Stack stk)
+                              int &hit, double cap, Stack stk)
 {
   const V3 inv(1.0 / NonZero(ld.x), 1.0 / NonZero(ld.y), 1.0 / NonZero(ld.z));
-  int sp       = 1;
-  stk.at(0)    = 0;
+  const double capm = cap + 1e-6;
+  int sp            = 1;
+  stk.at(0)         = 0;
   do {
     const uint32_t id = stk.at(--sp);
     const int nc      = g.nchild[L.nodeBase + id];
@@ -150,10 +181,13 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
         const int prim = g.prim[L.primBase + off + i];
         double tmin, tmax;
         slab(g.dAABB[L.dOffset + prim], lp, inv, tmin, tmax);
-        if (tmin <= tmax && tmax >= 0.0 && tmin < step) {
-          const int pv      = g.daughters[L.dOffset + prim];
-          PVolRec const &P  = g.pvols[pv];
-          const double dist = dist_in(shape_of_pvol(g, pv), to_local(P.t, P.r, lp), rotate(P.r, ld));
+        if (tmin <= tmax && tmax >= 0.0 && tmin < step && !(tmin > capm)) {
+          const int pv        = g.daughters[L.dOffset + prim];
+          PVolRec const &P    = g.pvols[pv];
+          ShapeRecord const &S = shape_of_pvol(g, pv);
+          const V3 loc        = to_local(P.t, P.r, lp);
+          if (lower_bound_in(S, loc) > capm) continue;
+          const double dist = dist_in(S, loc, rotate(P.r, ld));
           if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
         }
       }
@@ -162,8 +196,8 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
       double tminL, tmaxL, tminR, tmaxR;
       slab(g.nodes[L.nodeBase + l], lp, inv, tminL, tmaxL);
       slab(g.nodes[L.nodeBase + r], lp, inv, tminR, tmaxR);
-      const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step;
-      const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step;
+      const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step && !(tminL > capm);
+      const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step && !(tminR > capm);
       if (tminR < tminL) {
         if (tr) stk.at(sp++) = r;
         if (tl) stk.at(sp++) = l;
@@ -310,7 +344,8 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   double step       = dist_out(g.shapes[L.shape], lp, ld);
   if (step < 0.) step = 0.;
   int hit = -1;
-  if (L.ndaughters > 0) check_daughters(g, L, lp, ld, step, top(g, in.last), hit, stk);
+  if (L.ndaughters > 0)
+    check_daughters(g, L, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, stk);
   out = in;
   if (step == kInfLength && limit > 0.) {
     out.boundary = true;

@@ -2,8 +2,8 @@
 // resolvent cubic, Newton polishing to kTolerance, bounding-tube pre-step with the UniversalTube
 // kernels (the VECGEOM_NO_SPECIALIZATION branch of DistanceToIn).
 //
-// In the mixed path the FP32 pass answers only the early exits (bounding-tube miss, inside or
-// outside) and defers every quartic solve to the FP64 evaluation.
+// In the mixed path the FP32 pass answers only DistanceToOut's early exits (point outside) and
+// defers every quartic solve, and all of DistanceToIn, to the FP64 evaluation.
 //
 // The cubic uses the device math library's cbrt, pow, atan2 and cos where the scalar backend
 // uses the host C library's; the Newton polishing that follows makes both converge to the same
@@ -361,6 +361,9 @@ VGMI_DEV R DistSqrToTorusR(TorusData const &t, Vec3<R> const &point, Vec3<R> con
 template <class R>
 VGMI_DEV R DistanceToIn(TorusData const &t, Vec3<R> const &point, Vec3<R> const &dir)
 {
+  // FP32 pass of the mixed path: the bounding tube is tangent to the torus, so its FP32
+  // pre-test can miss where FP64 hits; the whole query is deferred to FP64 (NaN).
+  if (sizeof(R) == sizeof(float)) return R(__builtin_nanf(""));
   Vec3<R> lp          = point;
   R distance          = InfLen<R>();
   R tubeDist          = InfLen<R>();
@@ -374,7 +377,6 @@ VGMI_DEV R DistanceToIn(TorusData const &t, Vec3<R> const &point, Vec3<R> const 
   } else {
     if (tubeDist == InfLen<R>()) return InfLen<R>();
   }
-  if (sizeof(R) == sizeof(float)) return R(__builtin_nanf("")); // root solve in FP64, as in DistanceToOut
   lp = lp + dir * tubeDist;
   if (t.hasPhi) {
     // Wedge::DistanceToIn
