@@ -218,6 +218,43 @@ VGMI_API vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, do
                                         uint32_t *nav, uint32_t *last, int32_t *steps_taken, size_t n, int nsteps,
                                         void *stream);
 
+/* ------------------------------------------------------------------------
+ * Serving loop (continuous batching). No reference counterpart: the reference
+ * is called synchronously by a transport engine; this is the request queue a
+ * tracking service puts in front of vgmi_nav_propagate-style work.
+ *
+ * Each request is a set of tracks given as host arrays: global points and
+ * directions plus a per-track step count. A track is located from the world
+ * (vgmi_nav_locate), then stepped up to nsteps[i] times
+ * (ComputeStepAndPropagatedState, navigation/VNavigator.h:436-540) or until it
+ * leaves the world. Results go to the caller's host arrays, which must stay
+ * valid (inputs unmodified) until vgmi_server_wait returns for the ticket.
+ *
+ * Requests from many threads are packed into launches of at most lane_tracks
+ * tracks; `lanes` launches (each with its own stream, pinned staging and device
+ * buffers) are in flight at once. Admission: a request with more than
+ * lane_tracks tracks, or one that would put more than queue_tracks tracks in
+ * the queue, is refused with VGMI_ERR_CAPACITY; creation fails with
+ * VGMI_ERR_CAPACITY when the lanes would take more than half of the free HBM.
+ * Only VGMI_FP64 is implemented (VGMI_ERR_UNSUPPORTED otherwise).
+ * ------------------------------------------------------------------------ */
+typedef struct vgmi_server vgmi_server;
+
+VGMI_API vgmi_status vgmi_server_create(const vgmi_geom *geom, int precision, double step_limit,
+                                        int64_t lane_tracks, int lanes, int64_t queue_tracks, vgmi_server **out);
+/* nav_out and steps_out may be NULL. *ticket identifies the request for vgmi_server_wait. */
+VGMI_API vgmi_status vgmi_server_submit(vgmi_server *server, const double *px, const double *py, const double *pz,
+                                        const double *dx, const double *dy, const double *dz, const int32_t *nsteps,
+                                        int64_t n, double *px_out, double *py_out, double *pz_out,
+                                        uint32_t *nav_out, int32_t *steps_out, uint64_t *ticket);
+/* Blocks until the request is done; latency_us (may be NULL) = submit-to-completion time.
+ * Each ticket is waited for exactly once. */
+VGMI_API vgmi_status vgmi_server_wait(vgmi_server *server, uint64_t ticket, double *latency_us);
+/* stats[0..3] = launches, tracks done, steps done, requests refused */
+VGMI_API vgmi_status vgmi_server_stats(vgmi_server *server, int64_t stats[4]);
+/* Stops the threads after the launches in flight complete; queued requests are dropped. */
+VGMI_API vgmi_status vgmi_server_destroy(vgmi_server *server);
+
 #ifdef __cplusplus
 }
 #endif

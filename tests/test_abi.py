@@ -84,3 +84,18 @@ def test_product_extent_matches_reference_kats():
                 if not kat_util.check(r, got):
                     fails.append((kind, r["line"], got))
     assert n >= 10 and not fails, fails
+
+
+def test_server_rejects_bad_arguments_without_gpu():
+    """vgmi_server_* validate their arguments on the host before touching a device."""
+    import ctypes as C
+    from vecgeom_amd import lib, serving
+    L = serving._declare()
+    h = C.c_void_p()
+    assert L.vgmi_server_create(None, 0, 10.0, 1024, 2, 4096, C.byref(h)) == -1
+    assert b"geometry" in lib().vgmi_last_error()
+    assert L.vgmi_server_create(None, 0, 10.0, 1024, 2, 4096, None) == -1
+    t = C.c_uint64()
+    assert L.vgmi_server_submit(None, *([None] * 7), 1, *([None] * 5), C.byref(t)) == -1
+    assert L.vgmi_server_wait(None, 1, None) == -1
+    assert L.vgmi_server_destroy(None) == 0

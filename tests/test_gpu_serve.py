@@ -1,0 +1,109 @@
+"""GPU tests of the continuous-batching server (vgmi_server_*).
+
+Bar: every request's result is bit-identical whether it runs alone or packed with
+others into shared launches (tracks are independent and the FP64 kernels are
+deterministic), and matches the oracle's transport loop (points within kTolerance,
+expected bit-identical; NavStateIndex values and step counts exact).
+"""
+import threading
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import OracleGeometry
+from vecgeom_amd import FP64, MIXED, VgmiError, geometry
+from vecgeom_amd.serving import Server, poisson_load, summarize
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def geo():
+    from vecgeom_amd.navigation import Geometry
+    g = geometry.small_calorimeter()
+    return g, Geometry(g)
+
+
+def _requests(g, sizes, seed):
+    p, d = geometry.world_points(g, sum(sizes), seed=seed)
+    rng = np.random.default_rng(seed)
+    out, o = [], 0
+    for n in sizes:
+        out.append((p[:, o:o + n], d[:, o:o + n], rng.integers(1, 101, size=n, dtype=np.int32)))
+        o += n
+    return out
+
+
+def test_alone_equals_in_the_mix(geo):
+    g, G = geo
+    reqs = _requests(g, [1, 63, 1000, 4097, 20000, 300, 65536, 17], seed=31)
+    with Server(G, lane_tracks=1 << 17, lanes=3) as S:
+        alone = [S.wait(S.submit(*r)) for r in reqs]
+        # the same requests from 8 threads at once, twice over, so launches carry several requests
+        mixed = [None] * (2 * len(reqs))
+
+        def run(k):
+            rr = [S.submit(*reqs[i % len(reqs)]) for i in range(k, 2 * len(reqs), 8)]
+            for i, r in zip(range(k, 2 * len(reqs), 8), rr):
+                mixed[i] = S.wait(r)
+
+        th = [threading.Thread(target=run, args=(k,)) for k in range(8)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        st = S.stats()
+    assert st["launches"] < len(reqs) + 2 * len(reqs), st
+    assert st["tracks"] == 3 * sum(len(r[2]) for r in reqs)
+    for i, m in enumerate(mixed):
+        a = alone[i % len(reqs)]
+        assert np.array_equal(a.p_out, m.p_out), i
+        assert np.array_equal(a.nav, m.nav), i
+        assert np.array_equal(a.taken, m.taken), i
+        assert m.latency_us > 0
+
+
+def test_matches_oracle_transport(geo):
+    g, G = geo
+    O = OracleGeometry(g)
+    (p, d, nsteps), = _requests(g, [3000], seed=32)
+    with Server(G) as S:
+        r = S.wait(S.submit(p, d, nsteps))
+    nav0 = O.locate(p, threads=8)
+    for k in np.unique(nsteps):
+        sel = np.nonzero(nsteps == k)[0]
+        rp, rn, rl, rtaken = O.propagate(np.ascontiguousarray(p[:, sel]), np.ascontiguousarray(d[:, sel]), 10.0,
+                                         nav0[sel], np.zeros(len(sel), np.uint32), int(k), threads=8)
+        assert np.array_equal(r.nav[sel], rn), k
+        assert int(r.taken[sel].sum()) == rtaken, k
+        assert np.max(np.abs(r.p_out[:, sel] - rp)) <= 1e-9, k
+        assert np.array_equal(r.p_out[:, sel], rp), k
+
+
+def test_admission(geo):
+    g, G = geo
+    (p, d, nsteps), = _requests(g, [5000], seed=33)
+    with pytest.raises(VgmiError, match="UNSUPPORTED|only VGMI_FP64|failed"):
+        Server(G, precision=MIXED)
+    with Server(G, precision=FP64, lane_tracks=4096, lanes=1, queue_tracks=4096) as S:
+        with pytest.raises(VgmiError, match="not admitted"):
+            S.submit(p, d, nsteps)  # larger than a lane
+        r = S.wait(S.submit(p[:, :4096], d[:, :4096], nsteps[:4096]))
+        assert r.taken.min() >= 1
+        assert S.stats()["refused"] == 1
+    with pytest.raises(VgmiError, match="free HBM"):
+        Server(G, lane_tracks=1 << 40, lanes=16, queue_tracks=1 << 41)
+
+
+def test_poisson_load_completes(geo):
+    g, G = geo
+    pool = geometry.world_points(g, 1 << 17, seed=34)
+    with Server(G, lane_tracks=1 << 17, lanes=4) as S:
+        rec, refused, wall = poisson_load(S, pool, rate_hz=100.0, duration_s=1.0, producers=16, seed=1,
+                                          log2_max=16.0)
+        st = S.stats()
+    s = summarize(rec, refused, wall)
+    print(s, st)
+    assert s["requests"] > 20 and refused == 0
+    assert st["tracks"] == s["tracks"] and st["steps"] == s["nav_steps"]
+    assert s["p99_ms"] > 0

@@ -92,6 +92,7 @@ vgmi_status vgmi_geom_build(const vgmi_geom_desc *desc, int flags, vgmi_geom **o
     delete G;
     return set_error(VGMI_ERR_INVALID, "vgmi_geom_build: %s", err.c_str());
   }
+  for (auto const &r : G->host.shapes) G->typeMask |= 1u << unsigned(r.type);
   if (!(flags & VGMI_GEOM_HOST_ONLY)) {
     if (int rc = upload(*G)) {
       delete G;

@@ -87,12 +87,18 @@ VGMI_DEV void top_matrix(GeomView const &g, uint32_t nav, Mat &m)
     for (int i = 0; i < 9; ++i) m.r[i] = a[3 + i];
 }
 
-// ---- shape dispatch on a device shape record
-VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_in(s, p, d); }
-VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_out(s, p, d); }
-VGMI_DEV bool contains(ShapeRecord const &s, V3 const &p) { return rec_contains(s, p); }
-VGMI_DEV double safety_in(ShapeRecord const &s, V3 const &p) { return rec_safety_in(s, p); }
-VGMI_DEV double safety_out(ShapeRecord const &s, V3 const &p) { return rec_safety_out(s, p); }
+// ---- shape dispatch on a device shape record. Every function below is templated on TM,
+// the bit mask of shape types the kernel compiles in (bit E for ShapeType E).
+template <unsigned TM>
+VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_in<TM>(s, p, d); }
+template <unsigned TM>
+VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_out<TM>(s, p, d); }
+template <unsigned TM>
+VGMI_DEV bool contains(ShapeRecord const &s, V3 const &p) { return rec_contains<TM>(s, p); }
+template <unsigned TM>
+VGMI_DEV double safety_in(ShapeRecord const &s, V3 const &p) { return rec_safety_in<TM>(s, p); }
+template <unsigned TM>
+VGMI_DEV double safety_out(ShapeRecord const &s, V3 const &p) { return rec_safety_out<TM>(s, p); }
 
 VGMI_DEV ShapeRecord const &shape_of_pvol(GeomView const &g, int pvol) { return g.shapes[g.lvs[g.pvols[pvol].lv].shape]; }
 
@@ -165,6 +171,7 @@ VGMI_DEV double lower_bound_in(ShapeRecord const &s, V3 const &p)
 // distance exceeds both the current step and the limit ends the step at the limit either way.
 // A daughter is skipped when its box entry distance or its distance lower bound exceeds
 // cap + 1e-6, which leaves every distance below the limit, and the visiting order, unchanged.
+template <unsigned TM>
 VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V3 const &ld, double &step, int last,
                               int &hit, double cap, Stack stk)
 {
@@ -187,7 +194,7 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
           ShapeRecord const &S = shape_of_pvol(g, pv);
           const V3 loc        = to_local(P.t, P.r, lp);
           if (lower_bound_in(S, loc) > capm) continue;
-          const double dist = dist_in(S, loc, rotate(P.r, ld));
+          const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
           if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
         }
       }
@@ -210,6 +217,7 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
 }
 
 // BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint)
+template <unsigned TM>
 VGMI_DEV bool level_locate(GeomView const &g, LVRec const &L, int excl, V3 const &lp, int &pvol, V3 &dlp, Stack stk)
 {
   int sp    = 1;
@@ -226,7 +234,7 @@ VGMI_DEV bool level_locate(GeomView const &g, LVRec const &L, int excl, V3 const
           if (pv == excl) continue;
           PVolRec const &P = g.pvols[pv];
           const V3 loc     = to_local(P.t, P.r, lp);
-          if (contains(shape_of_pvol(g, pv), loc)) {
+          if (contains<TM>(shape_of_pvol(g, pv), loc)) {
             pvol = pv;
             dlp  = loc;
             return true;
@@ -243,6 +251,7 @@ VGMI_DEV bool level_locate(GeomView const &g, LVRec const &L, int excl, V3 const
 }
 
 // BVH::ComputeSafety
+template <unsigned TM>
 VGMI_DEV double safety_to_daughters(GeomView const &g, LVRec const &L, V3 const &lp, double safety, Stack stk)
 {
   int sp    = 1;
@@ -257,7 +266,7 @@ VGMI_DEV double safety_to_daughters(GeomView const &g, LVRec const &L, V3 const 
         if (box_safety(g.dAABB[L.dOffset + prim], lp) < safety) {
           const int pv     = g.daughters[L.dOffset + prim];
           PVolRec const &P = g.pvols[pv];
-          const double s   = safety_in(shape_of_pvol(g, pv), to_local(P.t, P.r, lp));
+          const double s   = safety_in<TM>(shape_of_pvol(g, pv), to_local(P.t, P.r, lp));
           if (s < safety) safety = s;
         }
       }
@@ -278,6 +287,7 @@ VGMI_DEV double safety_to_daughters(GeomView const &g, LVRec const &L, V3 const 
 }
 
 // GlobalLocator::LocateGlobalPoint (top = false) / LocateGlobalPointExclVolume (excl >= 0)
+template <unsigned TM>
 VGMI_DEV uint32_t locate_down(GeomView const &g, int pvol, int excl, V3 p, uint32_t path, Stack stk)
 {
   path              = push(g, path, pvol);
@@ -286,7 +296,7 @@ VGMI_DEV uint32_t locate_down(GeomView const &g, int pvol, int excl, V3 p, uint3
   while (L->ndaughters > 0 && deeper) {
     int next = -1;
     V3 dlp;
-    deeper = level_locate(g, *L, excl, p, next, dlp, stk);
+    deeper = level_locate<TM>(g, *L, excl, p, next, dlp, stk);
     if (deeper) {
       path = push(g, path, next);
       L    = &g.lvs[g.pvols[next].lv];
@@ -296,13 +306,15 @@ VGMI_DEV uint32_t locate_down(GeomView const &g, int pvol, int excl, V3 p, uint3
   return path;
 }
 
+template <unsigned TM>
 VGMI_DEV uint32_t locate_global(GeomView const &g, V3 const &p, Stack stk)
 {
-  if (!contains(shape_of_pvol(g, g.worldPvol), p)) return 0;
-  return locate_down(g, g.worldPvol, -1, p, 0, stk);
+  if (!contains<TM>(shape_of_pvol(g, g.worldPvol), p)) return 0;
+  return locate_down<TM>(g, g.worldPvol, -1, p, 0, stk);
 }
 
 // GlobalLocator::RelocatePointFromPathForceDifferent
+template <unsigned TM>
 VGMI_DEV uint32_t relocate_force_different(GeomView const &g, V3 const &lp, uint32_t path, Stack stk)
 {
   int mother      = top(g, path);
@@ -310,7 +322,7 @@ VGMI_DEV uint32_t relocate_force_different(GeomView const &g, V3 const &lp, uint
   if (mother < 0) return path;
   V3 tmp = lp;
   while (mother >= 0) {
-    if (mother == entry || !contains(shape_of_pvol(g, mother), tmp)) {
+    if (mother == entry || !contains<TM>(shape_of_pvol(g, mother), tmp)) {
       path             = pop(g, path);
       PVolRec const &P = g.pvols[mother];
       tmp              = to_master(P.t, P.r, tmp);
@@ -321,7 +333,7 @@ VGMI_DEV uint32_t relocate_force_different(GeomView const &g, V3 const &lp, uint
   }
   if (mother >= 0) {
     path = pop(g, path);
-    return locate_down(g, mother, entry, tmp, path, stk);
+    return locate_down<TM>(g, mother, entry, tmp, path, stk);
   }
   return path;
 }
@@ -332,6 +344,7 @@ struct State {
 };
 
 // VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState / ComputeStep
+template <unsigned TM>
 VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, double limit, State const &in,
                              State &out, bool relocate, Stack stk)
 {
@@ -341,11 +354,11 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   const V3 ld       = rotate(m.r, gd);
   const int pvol    = top(g, in.nav);
   LVRec const &L    = g.lvs[g.pvols[pvol].lv];
-  double step       = dist_out(g.shapes[L.shape], lp, ld);
+  double step       = dist_out<TM>(g.shapes[L.shape], lp, ld);
   if (step < 0.) step = 0.;
   int hit = -1;
   if (L.ndaughters > 0)
-    check_daughters(g, L, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, stk);
+    check_daughters<TM>(g, L, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, stk);
   out = in;
   if (step == kInfLength && limit > 0.) {
     out.boundary = true;
@@ -363,24 +376,25 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   if (!relocate) return step;
   const V3 after = lp + (step + 1E-6) * ld;
   if (top(g, out.nav) == pvol) {
-    out.nav = relocate_force_different(g, after, out.nav, stk);
+    out.nav = relocate_force_different<TM>(g, after, out.nav, stk);
   } else {
     PVolRec const &N = g.pvols[hit];
     out.nav          = pop(g, out.nav);
-    out.nav          = locate_down(g, hit, -1, to_local(N.t, N.r, after), out.nav, stk);
+    out.nav          = locate_down<TM>(g, hit, -1, to_local(N.t, N.r, after), out.nav, stk);
   }
   return step;
 }
 
 // VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
+template <unsigned TM>
 VGMI_DEV double compute_safety(GeomView const &g, V3 const &gp, uint32_t nav, Stack stk)
 {
   Mat m;
   top_matrix(g, nav, m);
   const V3 lp    = to_local(m.t, m.r, gp);
   LVRec const &L = g.lvs[g.pvols[top(g, nav)].lv];
-  double s       = safety_out(g.shapes[L.shape], lp);
-  if (s > 0.0 && L.ndaughters > 0) s = safety_to_daughters(g, L, lp, s, stk);
+  double s       = safety_out<TM>(g.shapes[L.shape], lp);
+  if (s > 0.0 && L.ndaughters > 0) s = safety_to_daughters<TM>(g, L, lp, s, stk);
   return s;
 }
 

@@ -67,68 +67,82 @@ struct ShapeOps;
 VGMI_FOR_EACH_SHAPE(VGMI_SHAPE_OPS)
 #undef VGMI_SHAPE_OPS
 
-// ---- run-time dispatch on a record (FP64)
+// ---- run-time dispatch on a record (FP64). TypeMask is a bit mask of the shape types compiled
+// in (bit E for enum value E); a kernel that serves a geometry made of a few simple types
+// leaves the large solvers out of its register allocation.
+template <unsigned TypeMask = ~0u>
 VGMI_DEV double rec_dist_in(ShapeRecord const &s, Vec3<double> const &p, Vec3<double> const &d)
 {
   switch (s.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    return NS::DistanceToIn(s.u.M, p, d);
+    if constexpr ((TypeMask >> E) & 1u) return NS::DistanceToIn(s.u.M, p, d);                                        \
+    break;
     VGMI_FOR_EACH_SHAPE(X)
 #undef X
   }
   return kInfLength;
 }
+template <unsigned TypeMask = ~0u>
 VGMI_DEV double rec_dist_out(ShapeRecord const &s, Vec3<double> const &p, Vec3<double> const &d)
 {
   switch (s.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    return NS::DistanceToOut(s.u.M, p, d);
+    if constexpr ((TypeMask >> E) & 1u) return NS::DistanceToOut(s.u.M, p, d);                                       \
+    break;
     VGMI_FOR_EACH_SHAPE(X)
 #undef X
   }
   return -1.;
 }
+template <unsigned TypeMask = ~0u>
 VGMI_DEV bool rec_contains(ShapeRecord const &s, Vec3<double> const &p)
 {
   switch (s.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    return NS::Contains(s.u.M, p);
+    if constexpr ((TypeMask >> E) & 1u) return NS::Contains(s.u.M, p);                                               \
+    break;
     VGMI_FOR_EACH_SHAPE(X)
 #undef X
   }
   return false;
 }
+template <unsigned TypeMask = ~0u>
 VGMI_DEV int rec_inside(ShapeRecord const &s, Vec3<double> const &p)
 {
   switch (s.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    return NS::Inside(s.u.M, p);
+    if constexpr ((TypeMask >> E) & 1u) return NS::Inside(s.u.M, p);                                                 \
+    break;
     VGMI_FOR_EACH_SHAPE(X)
 #undef X
   }
   return kOutside;
 }
+template <unsigned TypeMask = ~0u>
 VGMI_DEV double rec_safety_in(ShapeRecord const &s, Vec3<double> const &p)
 {
   switch (s.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    return NS::SafetyToIn(s.u.M, p);
+    if constexpr ((TypeMask >> E) & 1u) return NS::SafetyToIn(s.u.M, p);                                             \
+    break;
     VGMI_FOR_EACH_SHAPE(X)
 #undef X
   }
   return kInfLength;
 }
+template <unsigned TypeMask = ~0u>
 VGMI_DEV double rec_safety_out(ShapeRecord const &s, Vec3<double> const &p)
 {
   switch (s.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    return NS::SafetyToOut(s.u.M, p);
+    if constexpr ((TypeMask >> E) & 1u) return NS::SafetyToOut(s.u.M, p);                                            \
+    break;
     VGMI_FOR_EACH_SHAPE(X)
 #undef X
   }

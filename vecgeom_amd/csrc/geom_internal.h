@@ -34,4 +34,5 @@ struct vgmi_geom {
   void *device_block = nullptr;
   size_t device_bytes = 0;
   int device = -1;
+  unsigned typeMask = 0; // bit t set when a shape of ShapeType t is present
 };

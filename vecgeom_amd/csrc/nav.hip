@@ -6,10 +6,21 @@
 namespace vgmi {
 namespace nav {
 
+// Shape-type sets the kernels are instantiated for: a geometry built only from the simple
+// solids gets kernels without the polycone/polyhedron/sphere/torus solvers, whose register
+// demand would otherwise set the occupancy of every navigation kernel.
+constexpr unsigned kSimpleTypes = (1u << kBox) | (1u << kTube) | (1u << kCone) | (1u << kTrapezoid) | (1u << kOrb);
+constexpr unsigned kAllTypes    = ~0u;
+// Waves per SIMD the transport-loop kernels are compiled for: the simple-solid variants are
+// held to 128 VGPRs (4 waves, a few bytes of spill); the all-solid variants keep every register.
+template <unsigned TM>
+constexpr unsigned kLoopWaves = TM == kAllTypes ? 1u : 4u;
+
 #define VGMI_NAV_STACK                                                                                               \
   __shared__ uint32_t lds_stack[kStackDepth * kNavBlock];                                                           \
   Stack stk{lds_stack + threadIdx.x}
 
+template <unsigned TM>
 __global__ __launch_bounds__(kNavBlock) void k_locate(GeomView g, const double *__restrict__ px,
                                                       const double *__restrict__ py, const double *__restrict__ pz,
                                                       uint32_t *__restrict__ out, size_t n)
@@ -17,7 +28,7 @@ __global__ __launch_bounds__(kNavBlock) void k_locate(GeomView g, const double *
   VGMI_NAV_STACK;
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= n) return;
-  out[i] = locate_global(g, V3(px[i], py[i], pz[i]), stk);
+  out[i] = locate_global<TM>(g, V3(px[i], py[i], pz[i]), stk);
 }
 
 struct StepArgs {
@@ -30,6 +41,7 @@ struct StepArgs {
   int relocate;
 };
 
+template <unsigned TM>
 __global__ __launch_bounds__(kNavBlock) void k_step(GeomView g, StepArgs a)
 {
   VGMI_NAV_STACK;
@@ -38,7 +50,7 @@ __global__ __launch_bounds__(kNavBlock) void k_step(GeomView g, StepArgs a)
   State in{a.nav_in[i], a.last_in ? a.last_in[i] : 0u, false}, out = in;
   double step = 0.;
   if (in.nav != 0) {
-    step = compute_step(g, V3(a.px[i], a.py[i], a.pz[i]), V3(a.dx[i], a.dy[i], a.dz[i]), a.limit[i], in, out,
+    step = compute_step<TM>(g, V3(a.px[i], a.py[i], a.pz[i]), V3(a.dx[i], a.dy[i], a.dz[i]), a.limit[i], in, out,
                         a.relocate != 0, stk);
   } else {
     out.nav = 0;
@@ -49,6 +61,7 @@ __global__ __launch_bounds__(kNavBlock) void k_step(GeomView g, StepArgs a)
   if (a.bnd_out) a.bnd_out[i] = out.boundary;
 }
 
+template <unsigned TM>
 __global__ __launch_bounds__(kNavBlock) void k_safety(GeomView g, const double *__restrict__ px,
                                                       const double *__restrict__ py, const double *__restrict__ pz,
                                                       const uint32_t *__restrict__ nav, double *__restrict__ out,
@@ -58,7 +71,7 @@ __global__ __launch_bounds__(kNavBlock) void k_safety(GeomView g, const double *
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = nav[i];
-  out[i]           = s ? compute_safety(g, V3(px[i], py[i], pz[i]), s, stk) : 0.;
+  out[i]           = s ? compute_safety<TM>(g, V3(px[i], py[i], pz[i]), s, stk) : 0.;
 }
 
 struct PropArgs {
@@ -73,7 +86,8 @@ struct PropArgs {
 
 // The transport loop of one track stays in registers for all its steps; only the
 // final state goes back to HBM.
-__global__ __launch_bounds__(kNavBlock) void k_propagate(GeomView g, PropArgs a)
+template <unsigned TM>
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoopWaves<TM>))) void k_propagate(GeomView g, PropArgs a)
 {
   VGMI_NAV_STACK;
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
@@ -84,7 +98,7 @@ __global__ __launch_bounds__(kNavBlock) void k_propagate(GeomView g, PropArgs a)
   int s = 0;
   for (; s < a.nsteps && st.nav != 0; ++s) {
     State out;
-    const double step = compute_step(g, p, d, a.limit, st, out, true, stk);
+    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk);
     p                 = p + step * d;
     st                = out;
   }
@@ -96,13 +110,64 @@ __global__ __launch_bounds__(kNavBlock) void k_propagate(GeomView g, PropArgs a)
   if (a.taken) a.taken[i] = s;
 }
 
+// Serving-loop kernel: locate each track from the world, then run its own number of
+// ComputeStepAndPropagatedState steps (per-track step counts), all in registers.
+template <unsigned TM>
+__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoopWaves<TM>))) void k_locate_propagate(GeomView g, ServeArgs a)
+{
+  VGMI_NAV_STACK;
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  V3 p(a.px[i], a.py[i], a.pz[i]);
+  const V3 d(a.dx[i], a.dy[i], a.dz[i]);
+  State st{locate_global<TM>(g, p, stk), 0u, false};
+  const int nsteps = a.nsteps[i];
+  int s            = 0;
+  for (; s < nsteps && st.nav != 0; ++s) {
+    State out;
+    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk);
+    p                 = p + step * d;
+    st                = out;
+  }
+  a.opx[i]    = p.x;
+  a.opy[i]    = p.y;
+  a.opz[i]    = p.z;
+  a.onav[i]   = st.nav;
+  a.otaken[i] = s;
+}
+
 } // namespace nav
+
+vgmi_status launch_locate_propagate(const vgmi_geom *geom, ServeArgs const &a, hipStream_t st)
+{
+  if (a.n == 0) return VGMI_OK;
+  const dim3 grid(unsigned((a.n + nav::kNavBlock - 1) / nav::kNavBlock));
+  if ((geom->typeMask & ~nav::kSimpleTypes) == 0u)
+    hipLaunchKernelGGL(nav::k_locate_propagate<nav::kSimpleTypes>, grid, dim3(nav::kNavBlock), 0, st, geom->view, a);
+  else
+    hipLaunchKernelGGL(nav::k_locate_propagate<nav::kAllTypes>, grid, dim3(nav::kNavBlock), 0, st, geom->view, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "serving launch: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
 } // namespace vgmi
 
 using namespace vgmi;
 using namespace vgmi::nav;
 
 static unsigned nav_grid(size_t n) { return unsigned((n + kNavBlock - 1) / kNavBlock); }
+
+// Launches KERNEL<kSimpleTypes> when the geometry has only simple solids, else KERNEL<kAllTypes>.
+#define VGMI_NAV_LAUNCH(KERNEL, GEOM, N, STREAM, ...)                                                                \
+  do {                                                                                                               \
+    if (((GEOM)->typeMask & ~kSimpleTypes) == 0u)                                                                    \
+      hipLaunchKernelGGL(KERNEL<kSimpleTypes>, dim3(nav_grid(N)), dim3(kNavBlock), 0, (hipStream_t)(STREAM),         \
+                         __VA_ARGS__);                                                                               \
+    else                                                                                                             \
+      hipLaunchKernelGGL(KERNEL<kAllTypes>, dim3(nav_grid(N)), dim3(kNavBlock), 0, (hipStream_t)(STREAM),            \
+                         __VA_ARGS__);                                                                               \
+  } while (0)
 
 static vgmi_status check_geom(const vgmi_geom *g, int precision, const char *who)
 {
@@ -128,8 +193,7 @@ vgmi_status vgmi_nav_locate(const vgmi_geom *geom, int precision, const double *
   if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_locate")) return s;
   if (n == 0) return VGMI_OK;
   if (!px || !py || !pz || !nav_out) return set_error(VGMI_ERR_INVALID, "vgmi_nav_locate: null pointer");
-  hipLaunchKernelGGL(k_locate, dim3(nav_grid(n)), dim3(kNavBlock), 0, (hipStream_t)stream, geom->view, px, py, pz,
-                     nav_out, n);
+  VGMI_NAV_LAUNCH(k_locate, geom, n, stream, geom->view, px, py, pz, nav_out, n);
   return launched("vgmi_nav_locate");
 }
 
@@ -144,7 +208,7 @@ vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const double *px
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_step: null pointer");
   StepArgs a{px, py, pz, dx, dy, dz, step_limit, nav_in, last_in, nav_out, last_out, boundary_out, step_out, n,
              relocate};
-  hipLaunchKernelGGL(k_step, dim3(nav_grid(n)), dim3(kNavBlock), 0, (hipStream_t)stream, geom->view, a);
+  VGMI_NAV_LAUNCH(k_step, geom, n, stream, geom->view, a);
   return launched("vgmi_nav_step");
 }
 
@@ -154,8 +218,7 @@ vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *
   if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_safety")) return s;
   if (n == 0) return VGMI_OK;
   if (!px || !py || !pz || !nav_in || !safety_out) return set_error(VGMI_ERR_INVALID, "vgmi_nav_safety: null pointer");
-  hipLaunchKernelGGL(k_safety, dim3(nav_grid(n)), dim3(kNavBlock), 0, (hipStream_t)stream, geom->view, px, py, pz,
-                     nav_in, safety_out, n);
+  VGMI_NAV_LAUNCH(k_safety, geom, n, stream, geom->view, px, py, pz, nav_in, safety_out, n);
   return launched("vgmi_nav_safety");
 }
 
@@ -168,7 +231,7 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
   if (!px || !py || !pz || !dx || !dy || !dz || !nav || !last)
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_propagate: null pointer");
   PropArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, n, nsteps};
-  hipLaunchKernelGGL(k_propagate, dim3(nav_grid(n)), dim3(kNavBlock), 0, (hipStream_t)stream, geom->view, a);
+  VGMI_NAV_LAUNCH(k_propagate, geom, n, stream, geom->view, a);
   return launched("vgmi_nav_propagate");
 }
 

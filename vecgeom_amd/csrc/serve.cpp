@@ -1,0 +1,357 @@
+// Serving loop: continuous batching of track-transport requests onto the GPU.
+//
+// Producers submit requests (host arrays of track positions, directions and per-track step
+// counts) from any thread. A dispatcher thread packs every pending request that fits into
+// the next free lane's pinned staging buffer and issues, on that lane's HIP stream, one
+// host-to-device copy, one locate-and-propagate launch and one device-to-host copy; a
+// completion thread unpacks a lane's results when its event has fired and marks the
+// requests done. Lanes run concurrently, so copies of one launch overlap the kernels of
+// another. Admission: a request larger than a lane is rejected, and the queue holds at
+// most `queue_tracks` tracks (back-pressure); lane sizes are checked against the free HBM
+// reported by the runtime at creation.
+#include "geom_internal.h"
+#include "vgmi_internal.h"
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+using namespace vgmi;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+// Bytes per track in a lane: 6 doubles in, step count in, 3 doubles + state + steps taken out.
+constexpr size_t kTrackBytes = 6 * 8 + 4 + 3 * 8 + 4 + 4;
+
+struct Request {
+  const double *px, *py, *pz, *dx, *dy, *dz;
+  const int32_t *nsteps;
+  int64_t n;
+  double *opx, *opy, *opz;
+  uint32_t *onav;
+  int32_t *otaken;
+  Clock::time_point tSubmit, tDone;
+  int state  = 0; // 0 queued, 1 in flight, 2 done
+  int status = 0; // vgmi_status of the launch that carried it
+};
+
+struct Lane {
+  hipStream_t stream = nullptr;
+  hipEvent_t done    = nullptr;
+  char *hstage       = nullptr; // pinned
+  char *dstage       = nullptr;
+  int64_t cap        = 0;
+  bool busy          = false; // owns a batch (from packing until its results are unpacked)
+  bool issued        = false; // its copies, launch and event record are on the stream
+  int status         = 0;
+  int64_t ntracks    = 0;
+  std::vector<Request *> batch;
+  std::vector<int64_t> offset;
+  // SoA sections of the staging buffers for the current batch (same layout on host and
+  // device): inputs packed contiguously from offset 0, outputs from outBegin.
+  size_t oPx, oPy, oPz, oDx, oDy, oDz, oSteps, inEnd, oOpx, oOpy, oOpz, oNav, oTaken, outBegin, outEnd;
+};
+
+} // namespace
+
+struct vgmi_server {
+  const vgmi_geom *geom = nullptr;
+  double limit          = 0;
+  int64_t queueCap      = 0;
+  std::vector<Lane> lanes;
+  std::mutex m;
+  std::condition_variable cvWork, cvDone;
+  std::deque<Request *> queue;
+  int64_t queuedTracks = 0;
+  std::unordered_map<uint64_t, Request *> requests;
+  uint64_t nextTicket = 1;
+  bool stop           = false;
+  std::thread dispatcher, completer;
+  int device = 0;
+  // statistics
+  int64_t launches = 0, tracksDone = 0, stepsDone = 0, rejected = 0;
+};
+
+namespace {
+
+size_t lane_bytes(int64_t cap) { return size_t(cap) * kTrackBytes + 16 * 256; }
+
+// Section offsets for a batch of n tracks, each section 256-byte aligned.
+void layout(Lane &L, size_t n)
+{
+  size_t o  = 0;
+  auto take = [&](size_t bytes) {
+    size_t at = o;
+    o += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  L.oPx = take(8 * n), L.oPy = take(8 * n), L.oPz = take(8 * n);
+  L.oDx = take(8 * n), L.oDy = take(8 * n), L.oDz = take(8 * n);
+  L.oSteps   = take(4 * n);
+  L.inEnd    = o;
+  L.outBegin = o;
+  L.oOpx = take(8 * n), L.oOpy = take(8 * n), L.oOpz = take(8 * n);
+  L.oNav   = take(4 * n);
+  L.oTaken = take(4 * n);
+  L.outEnd = o;
+}
+
+void dispatch_loop(vgmi_server *S)
+{
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    Lane *L = nullptr;
+    {
+      std::unique_lock<std::mutex> lk(S->m);
+      S->cvWork.wait(lk, [&] {
+        if (S->stop) return true;
+        if (S->queue.empty()) return false;
+        for (auto &l : S->lanes)
+          if (!l.busy) return true;
+        return false;
+      });
+      if (S->stop) return;
+      for (auto &l : S->lanes)
+        if (!l.busy) {
+          L = &l;
+          break;
+        }
+      L->batch.clear();
+      L->offset.clear();
+      L->ntracks = 0;
+      while (!S->queue.empty() && L->ntracks + S->queue.front()->n <= L->cap) {
+        Request *r = S->queue.front();
+        S->queue.pop_front();
+        S->queuedTracks -= r->n;
+        r->state = 1;
+        L->offset.push_back(L->ntracks);
+        L->batch.push_back(r);
+        L->ntracks += r->n;
+      }
+      L->busy   = true;
+      L->issued = false;
+    }
+    layout(*L, size_t(L->ntracks));
+    // pack inputs (outside the lock)
+    char *h = L->hstage;
+    for (size_t k = 0; k < L->batch.size(); ++k) {
+      Request const *r = L->batch[k];
+      const size_t o = size_t(L->offset[k]);
+      std::memcpy(h + L->oPx + 8 * o, r->px, 8 * r->n);
+      std::memcpy(h + L->oPy + 8 * o, r->py, 8 * r->n);
+      std::memcpy(h + L->oPz + 8 * o, r->pz, 8 * r->n);
+      std::memcpy(h + L->oDx + 8 * o, r->dx, 8 * r->n);
+      std::memcpy(h + L->oDy + 8 * o, r->dy, 8 * r->n);
+      std::memcpy(h + L->oDz + 8 * o, r->dz, 8 * r->n);
+      std::memcpy(h + L->oSteps + 4 * o, r->nsteps, 4 * r->n);
+    }
+    char *d = L->dstage;
+    ServeArgs a{reinterpret_cast<const double *>(d + L->oPx),  reinterpret_cast<const double *>(d + L->oPy),
+                reinterpret_cast<const double *>(d + L->oPz),  reinterpret_cast<const double *>(d + L->oDx),
+                reinterpret_cast<const double *>(d + L->oDy),  reinterpret_cast<const double *>(d + L->oDz),
+                reinterpret_cast<const int32_t *>(d + L->oSteps), S->limit,
+                reinterpret_cast<double *>(d + L->oOpx),      reinterpret_cast<double *>(d + L->oOpy),
+                reinterpret_cast<double *>(d + L->oOpz),      reinterpret_cast<uint32_t *>(d + L->oNav),
+                reinterpret_cast<int32_t *>(d + L->oTaken),   size_t(L->ntracks)};
+    hipError_t e = hipMemcpyAsync(d, h, L->inEnd, hipMemcpyHostToDevice, L->stream);
+    int st       = e == hipSuccess ? launch_locate_propagate(S->geom, a, L->stream) : VGMI_ERR_DEVICE;
+    if (st == VGMI_OK)
+      e = hipMemcpyAsync(h + L->outBegin, d + L->outBegin, L->outEnd - L->outBegin, hipMemcpyDeviceToHost, L->stream);
+    if (st == VGMI_OK && e == hipSuccess) e = hipEventRecord(L->done, L->stream);
+    if (st == VGMI_OK && e != hipSuccess) st = VGMI_ERR_DEVICE;
+    if (st != VGMI_OK) (void)hipStreamSynchronize(L->stream); // the event is not used then
+    {
+      std::lock_guard<std::mutex> lk(S->m);
+      L->status = st;
+      L->issued = true;
+      S->launches++;
+    }
+    S->cvWork.notify_all(); // wakes the completion thread
+  }
+}
+
+void complete_loop(vgmi_server *S)
+{
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    std::vector<Lane *> ready;
+    {
+      std::unique_lock<std::mutex> lk(S->m);
+      bool anyBusy = false;
+      for (auto &l : S->lanes) anyBusy = anyBusy || l.busy;
+      if (S->stop && !anyBusy) return;
+      if (!anyBusy) {
+        S->cvWork.wait_for(lk, std::chrono::milliseconds(1));
+        continue;
+      }
+      for (auto &l : S->lanes)
+        if (l.issued && (l.status != VGMI_OK || hipEventQuery(l.done) != hipErrorNotReady)) ready.push_back(&l);
+    }
+    if (ready.empty()) {
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+      continue;
+    }
+    for (Lane *L : ready) {
+      char const *h = L->hstage;
+      int64_t steps = 0;
+      if (L->status == VGMI_OK && hipEventSynchronize(L->done) != hipSuccess) L->status = VGMI_ERR_DEVICE;
+      for (size_t k = 0; L->status == VGMI_OK && k < L->batch.size(); ++k) {
+        Request *r     = L->batch[k];
+        const size_t o = size_t(L->offset[k]);
+        std::memcpy(r->opx, h + L->oOpx + 8 * o, 8 * r->n);
+        std::memcpy(r->opy, h + L->oOpy + 8 * o, 8 * r->n);
+        std::memcpy(r->opz, h + L->oOpz + 8 * o, 8 * r->n);
+        if (r->onav) std::memcpy(r->onav, h + L->oNav + 4 * o, 4 * r->n);
+        const int32_t *tk = reinterpret_cast<const int32_t *>(h + L->oTaken) + o;
+        if (r->otaken) std::memcpy(r->otaken, tk, 4 * r->n);
+        for (int64_t i = 0; i < r->n; ++i) steps += tk[i];
+      }
+      const auto now = Clock::now();
+      {
+        std::lock_guard<std::mutex> lk(S->m);
+        for (Request *r : L->batch) {
+          r->tDone  = now;
+          r->state  = 2;
+          r->status = L->status;
+          if (L->status == VGMI_OK) S->tracksDone += r->n;
+        }
+        if (L->status == VGMI_OK) S->stepsDone += steps;
+        L->batch.clear();
+        L->busy   = false;
+        L->issued = false;
+      }
+      S->cvDone.notify_all();
+      S->cvWork.notify_all();
+    }
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+vgmi_status vgmi_server_create(const vgmi_geom *geom, int precision, double step_limit, int64_t lane_tracks,
+                               int lanes, int64_t queue_tracks, vgmi_server **out)
+{
+  if (!out) return set_error(VGMI_ERR_INVALID, "vgmi_server_create: null output handle");
+  *out = nullptr;
+  if (!geom || !geom->device_block) return set_error(VGMI_ERR_INVALID, "vgmi_server_create: geometry not on a device");
+  if (precision != VGMI_FP64) return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_server_create: only VGMI_FP64 is implemented");
+  if (lane_tracks < 1 || lanes < 1 || lanes > 16 || queue_tracks < lane_tracks)
+    return set_error(VGMI_ERR_INVALID, "vgmi_server_create: need lane_tracks >= 1, 1..16 lanes, queue >= lane");
+  size_t freeB = 0, totalB = 0;
+  if (hipSetDevice(geom->device) != hipSuccess || hipMemGetInfo(&freeB, &totalB) != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_server_create: no device");
+  const size_t need = size_t(lanes) * lane_bytes(lane_tracks);
+  if (need > freeB / 2)
+    return set_error(VGMI_ERR_CAPACITY, "vgmi_server_create: %zu bytes of lanes exceed half of the free HBM (%zu)", need,
+                     freeB);
+  auto *S      = new vgmi_server();
+  S->geom      = geom;
+  S->limit     = step_limit;
+  S->queueCap  = queue_tracks;
+  S->device    = geom->device;
+  S->lanes.resize(lanes);
+  for (auto &L : S->lanes) {
+    L.cap = lane_tracks;
+    const size_t bytes = lane_bytes(lane_tracks);
+    hipError_t e = hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&L.hstage), bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&L.dstage), bytes);
+    if (e != hipSuccess) {
+      vgmi_server_destroy(S);
+      return set_error(VGMI_ERR_DEVICE, "vgmi_server_create: %s", hipGetErrorString(e));
+    }
+  }
+  S->dispatcher = std::thread(dispatch_loop, S);
+  S->completer  = std::thread(complete_loop, S);
+  *out          = S;
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_server_submit(vgmi_server *S, const double *px, const double *py, const double *pz, const double *dx,
+                               const double *dy, const double *dz, const int32_t *nsteps, int64_t n, double *px_out,
+                               double *py_out, double *pz_out, uint32_t *nav_out, int32_t *steps_out,
+                               uint64_t *ticket)
+{
+  if (!S || !ticket) return set_error(VGMI_ERR_INVALID, "vgmi_server_submit: null argument");
+  if (n < 1 || !px || !py || !pz || !dx || !dy || !dz || !nsteps || !px_out || !py_out || !pz_out)
+    return set_error(VGMI_ERR_INVALID, "vgmi_server_submit: empty request or null array");
+  auto *r = new Request{px,     py,      pz,     dx,        dy,           dz,
+                        nsteps, n,       px_out, py_out,    pz_out,       nav_out,
+                        steps_out, Clock::now(), Clock::time_point(), 0, 0};
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    if (n > S->lanes[0].cap || S->queuedTracks + n > S->queueCap) {
+      S->rejected++;
+      delete r;
+      return set_error(VGMI_ERR_CAPACITY, "vgmi_server_submit: request of %lld tracks not admitted (queue %lld/%lld)",
+                       (long long)n, (long long)S->queuedTracks, (long long)S->queueCap);
+    }
+    *ticket              = S->nextTicket++;
+    S->requests[*ticket] = r;
+    S->queue.push_back(r);
+    S->queuedTracks += n;
+  }
+  S->cvWork.notify_all();
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_server_wait(vgmi_server *S, uint64_t ticket, double *latency_us)
+{
+  if (!S) return set_error(VGMI_ERR_INVALID, "vgmi_server_wait: null server");
+  std::unique_lock<std::mutex> lk(S->m);
+  auto it = S->requests.find(ticket);
+  if (it == S->requests.end()) return set_error(VGMI_ERR_INVALID, "vgmi_server_wait: unknown ticket %llu",
+                                                (unsigned long long)ticket);
+  Request *r = it->second;
+  S->cvDone.wait(lk, [&] { return r->state == 2; });
+  if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->tDone - r->tSubmit).count();
+  const int st = r->status;
+  S->requests.erase(it);
+  delete r;
+  return st == VGMI_OK ? VGMI_OK : set_error(st, "vgmi_server_wait: the launch carrying ticket %llu failed",
+                                             (unsigned long long)ticket);
+}
+
+vgmi_status vgmi_server_stats(vgmi_server *S, int64_t stats[4])
+{
+  if (!S || !stats) return set_error(VGMI_ERR_INVALID, "vgmi_server_stats: null argument");
+  std::lock_guard<std::mutex> lk(S->m);
+  stats[0] = S->launches;
+  stats[1] = S->tracksDone;
+  stats[2] = S->stepsDone;
+  stats[3] = S->rejected;
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_server_destroy(vgmi_server *S)
+{
+  if (!S) return VGMI_OK;
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    S->stop = true;
+  }
+  S->cvWork.notify_all();
+  if (S->dispatcher.joinable()) S->dispatcher.join();
+  if (S->completer.joinable()) S->completer.join();
+  for (auto &L : S->lanes) {
+    if (L.stream) (void)hipStreamSynchronize(L.stream);
+    if (L.dstage) (void)hipFree(L.dstage);
+    if (L.hstage) (void)hipHostFree(L.hstage);
+    if (L.done) (void)hipEventDestroy(L.done);
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+  }
+  for (auto &kv : S->requests) delete kv.second;
+  delete S;
+  return VGMI_OK;
+}
+
+} // extern "C"
