@@ -14,8 +14,11 @@ Also reported in the same JSON line:
   box_tube: Box (10,20,30) and full-phi Tube (5,10,15), 2^20 points each, six methods;
   nav:      calorimeter (23k touchables), 2^20 tracks x 50 ComputeStepAndPropagatedState
             steps (10 mm limit), fused transport loop; nav-steps/s.
-Each carries a roofline entry (HIP events on the launch stream) and a CPU baseline
-(the oracle restatement of the scalar backend, 1 thread, bounded sample).
+  serve:    the continuous-batching server on the calorimeter: 16 producer threads with
+            Poisson arrivals (rates swept), 2^U(10,17.6) tracks per request, U{1..100}
+            steps per track; p50/p99 latency and nav-steps/s per rate.
+Each throughput figure carries a roofline entry (HIP events on the launch stream) and a
+CPU baseline (the oracle restatement of the scalar backend, 1 thread, bounded sample).
 """
 import argparse
 import json
@@ -47,6 +50,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-nav", action="store_true")
     ap.add_argument("--no-box-tube", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", default="50,100,200",
+                    help="Poisson request rates (requests/s, all producers together) of the serving sweep")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--copy-check", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (calibrates HBM byte counters) and exit")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="tile,block,blocks_per_cu for the batched launch (skips the autotuner)")
@@ -279,6 +288,52 @@ def run_nav(args):
     return res
 
 
+# ------------------------------------------------------------------ serving loop
+def run_serve(args):
+    import numpy as np
+
+    from vecgeom_amd import geometry
+    from vecgeom_amd.navigation import Geometry
+    from vecgeom_amd.serving import Server, poisson_load, summarize
+
+    g = geometry.calorimeter()
+    G = Geometry(g)
+    pool = geometry.world_points(g, 1 << 20, seed=0x5EED0005)
+    out = {"workload": "calorimeter; 16 producer threads, Poisson arrivals; batch 2^U(10,17.6) tracks; "
+                       "U{1..100} steps per track; 10 mm step limit; latency = submit to results in host memory",
+           "lanes": 4, "lane_tracks": 1 << 18}
+    with Server(G, lane_tracks=1 << 18, lanes=4, queue_tracks=1 << 23) as S:
+        rng = np.random.default_rng(0)
+        alone = {}
+        for n in (1024, 65536):
+            lat = []
+            for _ in range(5):
+                r = S.wait(S.submit(pool[0][:, :n], pool[1][:, :n], rng.integers(1, 101, size=n, dtype=np.int32)))
+                lat.append(r.latency_us / 1e3)
+            alone[str(n)] = round(float(np.median(lat)), 3)
+        out["alone_latency_ms"] = alone
+        sweep = []
+        for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+            rec, refused, wall = poisson_load(S, pool, rate_hz=rate, duration_s=args.serve_seconds, producers=16,
+                                              seed=int(rate))
+            s = summarize(rec, refused, wall)
+            s["rate_hz"] = rate
+            sweep.append(s)
+        out["sweep"] = sweep
+    return out
+
+
+def run_copy_check():
+    import torch
+    n = 1 << 30
+    a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_check": {"bytes_read": n, "bytes_written": n, "launches": 5}}), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -289,6 +344,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_check:
+        run_copy_check()
+        return
     print("bench: batched shape mix", file=sys.stderr, flush=True)
     mix = run_mix(args)
     extra = {}
@@ -298,6 +356,9 @@ def main():
     if not args.no_nav:
         print("bench: calorimeter navigation", file=sys.stderr, flush=True)
         extra["nav"] = run_nav(args)
+    if not args.no_serve:
+        print("bench: serving loop", file=sys.stderr, flush=True)
+        extra["serve"] = run_serve(args)
     line = {"metric": "shape queries/s (FP64 GPU, batched one-launch mix)", "value": mix["value"],
             "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": mix["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

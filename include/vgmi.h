@@ -218,6 +218,14 @@ VGMI_API vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, do
                                         uint32_t *nav, uint32_t *last, int32_t *steps_taken, size_t n, int nsteps,
                                         void *stream);
 
+/*
+ * Regrouping of the fused transport loops (vgmi_nav_propagate, the serving loop): every
+ * chunk_steps steps the tracks are counting-sorted on the device by the logical volume they
+ * are in, so the lanes of a wavefront run the same volume's code; 0 disables it (default 10).
+ * Results per track do not depend on it. No reference counterpart (a GPU scheduling knob).
+ */
+VGMI_API vgmi_status vgmi_geom_set_regroup(vgmi_geom *geom, int chunk_steps);
+
 /* ------------------------------------------------------------------------
  * Serving loop (continuous batching). No reference counterpart: the reference
  * is called synchronously by a transport engine; this is the request queue a

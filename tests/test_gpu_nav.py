@@ -66,7 +66,9 @@ def test_safety(geo):
     assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
 
 
-def test_propagate_matches_oracle(geo):
+@pytest.mark.parametrize("regroup", [0, 7, 10])
+def test_propagate_matches_oracle(geo, regroup):
+    """The fused loop, unsorted (0) and regrouped by volume every 7 / 10 steps: same results."""
     g, G, O = geo
     n, nsteps = 1 << 14, 50
     p, d = geometry.world_points(g, n, seed=24)
@@ -75,7 +77,11 @@ def test_propagate_matches_oracle(geo):
     P, D = SOA3D.from_array(p), SOA3D.from_array(d)
     nv, ls = _dev_u32(nav), _dev_u32(last)
     taken = torch.zeros(n, dtype=torch.int32, device="cuda")
-    G.Propagate(P, D, 10.0, nv, ls, nsteps, taken)
+    G.set_regroup(regroup)
+    try:
+        G.Propagate(P, D, 10.0, nv, ls, nsteps, taken)
+    finally:
+        G.set_regroup(10)
     rp, rn, rl, rtaken = O.propagate(p, d, 10.0, nav, last, nsteps, threads=8)
     assert np.array_equal(nv.cpu().numpy().astype(np.uint32), rn)
     assert np.array_equal(ls.cpu().numpy().astype(np.uint32), rl)

@@ -93,6 +93,7 @@ vgmi_status vgmi_geom_build(const vgmi_geom_desc *desc, int flags, vgmi_geom **o
     return set_error(VGMI_ERR_INVALID, "vgmi_geom_build: %s", err.c_str());
   }
   for (auto const &r : G->host.shapes) G->typeMask |= 1u << unsigned(r.type);
+  G->nlogical = int(G->host.lvs.size());
   if (!(flags & VGMI_GEOM_HOST_ONLY)) {
     if (int rc = upload(*G)) {
       delete G;

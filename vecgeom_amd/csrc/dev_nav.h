@@ -171,12 +171,16 @@ VGMI_DEV double lower_bound_in(ShapeRecord const &s, V3 const &p)
 // distance exceeds both the current step and the limit ends the step at the limit either way.
 // A daughter is skipped when its box entry distance or its distance lower bound exceeds
 // cap + 1e-6, which leaves every distance below the limit, and the visiting order, unchanged.
+// `bound` receives a lower bound of the Euclidean distance from lp to every daughter: the
+// box distance of each subtree or daughter box the walk does not descend into, and the box
+// distance (or shape lower bound, when computed) of each daughter it tests.
 template <unsigned TM>
 VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V3 const &ld, double &step, int last,
-                              int &hit, double cap, Stack stk)
+                              int &hit, double cap, double &bound, Stack stk)
 {
   const V3 inv(1.0 / NonZero(ld.x), 1.0 / NonZero(ld.y), 1.0 / NonZero(ld.z));
   const double capm = cap + 1e-6;
+  double bnd        = kInfLength;
   int sp            = 1;
   stk.at(0)         = 0;
   do {
@@ -185,26 +189,36 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
     if (nc >= 0) {
       const int off = g.offset[L.nodeBase + id];
       for (int i = 0; i < nc; ++i) {
-        const int prim = g.prim[L.primBase + off + i];
+        const int prim     = g.prim[L.primBase + off + i];
+        AABBRec const &box = g.dAABB[L.dOffset + prim];
         double tmin, tmax;
-        slab(g.dAABB[L.dOffset + prim], lp, inv, tmin, tmax);
+        slab(box, lp, inv, tmin, tmax);
+        double lb = box_safety(box, lp);
         if (tmin <= tmax && tmax >= 0.0 && tmin < step && !(tmin > capm)) {
-          const int pv        = g.daughters[L.dOffset + prim];
-          PVolRec const &P    = g.pvols[pv];
+          const int pv         = g.daughters[L.dOffset + prim];
+          PVolRec const &P     = g.pvols[pv];
           ShapeRecord const &S = shape_of_pvol(g, pv);
-          const V3 loc        = to_local(P.t, P.r, lp);
-          if (lower_bound_in(S, loc) > capm) continue;
-          const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
-          if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
+          const V3 loc         = to_local(P.t, P.r, lp);
+          const double lbs     = lower_bound_in(S, loc);
+          lb                   = vmax(lb, lbs);
+          if (!(lbs > capm)) {
+            const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
+            if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
+          }
         }
+        bnd = vmin(bnd, lb);
       }
     } else {
       const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      AABBRec const &bl = g.nodes[L.nodeBase + l];
+      AABBRec const &br = g.nodes[L.nodeBase + r];
       double tminL, tmaxL, tminR, tmaxR;
-      slab(g.nodes[L.nodeBase + l], lp, inv, tminL, tmaxL);
-      slab(g.nodes[L.nodeBase + r], lp, inv, tminR, tmaxR);
+      slab(bl, lp, inv, tminL, tmaxL);
+      slab(br, lp, inv, tminR, tmaxR);
       const bool tl = tminL <= tmaxL && tmaxL >= 0.0 && tminL < step && !(tminL > capm);
       const bool tr = tminR <= tmaxR && tmaxR >= 0.0 && tminR < step && !(tminR > capm);
+      if (!tl) bnd = vmin(bnd, box_safety(bl, lp));
+      if (!tr) bnd = vmin(bnd, box_safety(br, lp));
       if (tminR < tminL) {
         if (tr) stk.at(sp++) = r;
         if (tl) stk.at(sp++) = l;
@@ -214,6 +228,7 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
       }
     }
   } while (sp > 0);
+  bound = bnd;
 }
 
 // BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint)
@@ -343,10 +358,19 @@ struct State {
   bool boundary;
 };
 
-// VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState / ComputeStep
+// Margin of the daughter-budget test: covers the rounding of the bounds and of the point
+// updates over many steps, and the tolerance-level undershoot of DistanceToIn.
+constexpr double kBudgetMargin = 1e-6;
+
+// VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState / ComputeStep.
+// `budget` carries, across the steps of one track, a lower bound of the distance from the
+// current point to every daughter of the current volume (negative: unknown). While it exceeds
+// the step limit and the mother distance is finite, no daughter can be hit within the limit,
+// so the daughter walk cannot change the outcome of the step and is skipped; the walk itself
+// refreshes the bound. Every change of state invalidates it.
 template <unsigned TM>
 VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, double limit, State const &in,
-                             State &out, bool relocate, Stack stk)
+                             State &out, bool relocate, Stack stk, double &budget)
 {
   Mat m;
   top_matrix(g, in.nav, m);
@@ -357,18 +381,25 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   double step       = dist_out<TM>(g.shapes[L.shape], lp, ld);
   if (step < 0.) step = 0.;
   int hit = -1;
-  if (L.ndaughters > 0)
-    check_daughters<TM>(g, L, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, stk);
+  if (L.ndaughters == 0) {
+    budget = kInfLength;
+  } else if (!(step < kInfLength && budget > limit + kBudgetMargin)) {
+    check_daughters<TM>(g, L, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, budget,
+                        stk);
+  }
   out = in;
   if (step == kInfLength && limit > 0.) {
     out.boundary = true;
     out.nav      = pop(g, out.nav);
+    budget       = -1.;
     return kTolerance;
   }
   if (step > limit) {
     out.boundary = false;
+    budget -= limit;
     return limit;
   }
+  budget       = -1.;
   out.boundary = true;
   out.last     = out.nav;
   if (hit >= 0) out.nav = push(g, out.nav, hit);

@@ -35,4 +35,6 @@ struct vgmi_geom {
   size_t device_bytes = 0;
   int device = -1;
   unsigned typeMask = 0; // bit t set when a shape of ShapeType t is present
+  int nlogical = 0;
+  int regroupSteps = 10; // transport loops regroup tracks by volume every this many steps (0: never)
 };

@@ -1,5 +1,6 @@
 // Navigation kernels and their C ABI entry points (one lane per track).
 #include "dev_nav.h"
+#include <algorithm>
 #include "geom_internal.h"
 #include "vgmi_internal.h"
 
@@ -48,10 +49,10 @@ __global__ __launch_bounds__(kNavBlock) void k_step(GeomView g, StepArgs a)
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
   if (i >= a.n) return;
   State in{a.nav_in[i], a.last_in ? a.last_in[i] : 0u, false}, out = in;
-  double step = 0.;
+  double step = 0., budget = -1.;
   if (in.nav != 0) {
     step = compute_step<TM>(g, V3(a.px[i], a.py[i], a.pz[i]), V3(a.dx[i], a.dy[i], a.dz[i]), a.limit[i], in, out,
-                        a.relocate != 0, stk);
+                        a.relocate != 0, stk, budget);
   } else {
     out.nav = 0;
   }
@@ -82,6 +83,9 @@ struct PropArgs {
   int32_t *taken;
   size_t n;
   int nsteps;
+  const uint32_t *perm;  // lane j runs track perm[*first + j] (null: track j)
+  const uint32_t *first; // number of leading perm entries to skip (tracks outside the world)
+  int accumulate;        // add to taken instead of storing
 };
 
 // The transport loop of one track stays in registers for all its steps; only the
@@ -90,15 +94,22 @@ template <unsigned TM>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoopWaves<TM>))) void k_propagate(GeomView g, PropArgs a)
 {
   VGMI_NAV_STACK;
-  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
-  if (i >= a.n) return;
+  size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (a.perm) {
+    i += *a.first;
+    if (i >= a.n) return;
+    i = a.perm[i];
+  } else if (i >= a.n) {
+    return;
+  }
   V3 p(a.px[i], a.py[i], a.pz[i]);
   const V3 d(a.dx[i], a.dy[i], a.dz[i]);
   State st{a.nav[i], a.last[i], false};
-  int s = 0;
+  double budget = -1.;
+  int s         = 0;
   for (; s < a.nsteps && st.nav != 0; ++s) {
     State out;
-    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk);
+    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk, budget);
     p                 = p + step * d;
     st                = out;
   }
@@ -107,7 +118,91 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoop
   a.pz[i]  = p.z;
   a.nav[i] = st.nav;
   a.last[i] = st.last;
-  if (a.taken) a.taken[i] = s;
+  if (a.taken) a.taken[i] = a.accumulate ? a.taken[i] + s : s;
+}
+
+// ---- regrouping: a counting sort of tracks by the logical volume their state is in
+// (key 0: outside the world), so that the lanes of a wavefront run the same volume's code.
+constexpr int kSortBlock = 256;
+constexpr int kLocalBins = 8192; // bins kept in LDS (32 KiB); larger key ranges use global atomics
+
+VGMI_DEV uint32_t state_key(GeomView const &g, uint32_t nav)
+{
+  return nav ? uint32_t(g.pvols[top(g, nav)].lv) + 1u : 0u;
+}
+
+__global__ void k_fill_u32(uint32_t *a, size_t n, uint32_t v)
+{
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) a[i] = v;
+}
+
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(GeomView g, const uint32_t *nav, size_t n, uint32_t *hist,
+                                                          int nbins)
+{
+  extern __shared__ uint32_t sh[];
+  const bool local = nbins <= kLocalBins;
+  if (local)
+    for (int b = threadIdx.x; b < nbins; b += kSortBlock) sh[b] = 0;
+  __syncthreads();
+  for (size_t i = size_t(blockIdx.x) * kSortBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kSortBlock) {
+    const uint32_t k = state_key(g, nav[i]);
+    if (local)
+      atomicAdd(&sh[k], 1u);
+    else
+      atomicAdd(&hist[k], 1u);
+  }
+  __syncthreads();
+  if (local)
+    for (int b = threadIdx.x; b < nbins; b += kSortBlock)
+      if (sh[b]) atomicAdd(&hist[b], sh[b]);
+}
+
+// Exclusive prefix sum of hist[0..nbins) in place, one block of 1024 threads.
+__global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *hist, int nbins)
+{
+  __shared__ uint32_t part[1024];
+  const int t   = threadIdx.x;
+  const int per = (nbins + 1023) / 1024;
+  const int b0 = t * per, b1 = b0 + per < nbins ? b0 + per : nbins;
+  uint32_t sum = 0;
+  for (int b = b0; b < b1; ++b) sum += hist[b];
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (int b = b0; b < b1; ++b) {
+    const uint32_t c = hist[b];
+    hist[b]          = run;
+    run += c;
+  }
+}
+
+// perm[cursor[key]++] = i, with block-local ranks so that each block reserves one range per key.
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, const uint32_t *nav, size_t n,
+                                                             uint32_t *cursor, uint32_t *perm, int nbins)
+{
+  extern __shared__ uint32_t sh[];
+  const size_t i   = size_t(blockIdx.x) * kSortBlock + threadIdx.x;
+  const uint32_t k = i < n ? state_key(g, nav[i]) : 0u;
+  if (nbins > kLocalBins) {
+    if (i < n) perm[atomicAdd(&cursor[k], 1u)] = uint32_t(i);
+    return;
+  }
+  for (int b = threadIdx.x; b < nbins; b += kSortBlock) sh[b] = 0;
+  __syncthreads();
+  const uint32_t rank = i < n ? atomicAdd(&sh[k], 1u) : 0u;
+  __syncthreads();
+  for (int b = threadIdx.x; b < nbins; b += kSortBlock) {
+    const uint32_t c = sh[b];
+    if (c) sh[b] = atomicAdd(&cursor[b], c);
+  }
+  __syncthreads();
+  if (i < n) perm[sh[k] + rank] = uint32_t(i);
 }
 
 // Serving-loop kernel: locate each track from the world, then run its own number of
@@ -122,10 +217,11 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoop
   const V3 d(a.dx[i], a.dy[i], a.dz[i]);
   State st{locate_global<TM>(g, p, stk), 0u, false};
   const int nsteps = a.nsteps[i];
+  double budget    = -1.;
   int s            = 0;
   for (; s < nsteps && st.nav != 0; ++s) {
     State out;
-    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk);
+    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk, budget);
     p                 = p + step * d;
     st                = out;
   }
@@ -157,6 +253,9 @@ using namespace vgmi;
 using namespace vgmi::nav;
 
 static unsigned nav_grid(size_t n) { return unsigned((n + kNavBlock - 1) / kNavBlock); }
+
+// Below this many tracks a transport loop is not regrouped (the sort would not pay).
+constexpr size_t kRegroupMinTracks = 4096;
 
 // Launches KERNEL<kSimpleTypes> when the geometry has only simple solids, else KERNEL<kAllTypes>.
 #define VGMI_NAV_LAUNCH(KERNEL, GEOM, N, STREAM, ...)                                                                \
@@ -230,9 +329,44 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
   if (n == 0 || nsteps <= 0) return VGMI_OK;
   if (!px || !py || !pz || !dx || !dy || !dz || !nav || !last)
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_propagate: null pointer");
-  PropArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, n, nsteps};
-  VGMI_NAV_LAUNCH(k_propagate, geom, n, stream, geom->view, a);
+  if (n > 0xffffffffull) return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_propagate: more than 2^32 tracks");
+  hipStream_t st = (hipStream_t)stream;
+  const int chunk = geom->regroupSteps;
+  if (chunk <= 0 || n < kRegroupMinTracks) {
+    PropArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, n, nsteps, nullptr, nullptr, 0};
+    VGMI_NAV_LAUNCH(k_propagate, geom, n, stream, geom->view, a);
+    return launched("vgmi_nav_propagate");
+  }
+  // Chunks of `chunk` steps; before each, tracks are regrouped by the volume they are in.
+  const int nbins     = geom->nlogical + 1;
+  const size_t wbytes = (size_t(nbins) + n) * sizeof(uint32_t);
+  uint32_t *work      = nullptr;
+  if (hipMallocAsync(reinterpret_cast<void **>(&work), wbytes, st) != hipSuccess)
+    return set_error(VGMI_ERR_DEVICE, "vgmi_nav_propagate: workspace of %zu bytes", wbytes);
+  uint32_t *cursor = work, *perm = work + nbins;
+  const size_t lds = nbins <= kLocalBins ? size_t(nbins) * sizeof(uint32_t) : 0;
+  const unsigned hgrid = unsigned(std::min<size_t>((n + kSortBlock - 1) / kSortBlock, 2048));
+  const unsigned sgrid = unsigned((n + kSortBlock - 1) / kSortBlock);
+  if (steps_taken) hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint32_t *>(steps_taken), n, 0u);
+  for (int done = 0; done < nsteps; done += chunk) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(unsigned((nbins + 255) / 256)), dim3(256), 0, st, cursor, size_t(nbins), 0u);
+    hipLaunchKernelGGL(k_sort_hist, dim3(hgrid), dim3(kSortBlock), lds, st, geom->view, nav, n, cursor, nbins);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, cursor, nbins);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, nav, n, cursor, perm, nbins);
+    PropArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, n, std::min(chunk, nsteps - done), perm,
+               cursor, 1};
+    VGMI_NAV_LAUNCH(k_propagate, geom, n, stream, geom->view, a);
+  }
+  (void)hipFreeAsync(work, st);
   return launched("vgmi_nav_propagate");
+}
+
+vgmi_status vgmi_geom_set_regroup(vgmi_geom *geom, int chunk_steps)
+{
+  if (!geom) return set_error(VGMI_ERR_INVALID, "vgmi_geom_set_regroup: null geometry");
+  if (chunk_steps < 0) return set_error(VGMI_ERR_INVALID, "vgmi_geom_set_regroup: negative chunk");
+  geom->regroupSteps = chunk_steps;
+  return VGMI_OK;
 }
 
 } // extern "C"

@@ -31,6 +31,7 @@ def _declare():
         "vgmi_nav_step": [vp, ci] + [vp] * 13 + [sz, ci, vp],
         "vgmi_nav_safety": [vp, ci] + [vp] * 5 + [sz, vp],
         "vgmi_nav_propagate": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
+        "vgmi_geom_set_regroup": [vp, ci],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -100,6 +101,10 @@ class Geometry:
                 "aabbs": aabbs.reshape(-1, 6)}
 
     # ---- device navigation
+    def set_regroup(self, chunk_steps):
+        """Regroup tracks by volume every chunk_steps steps in the fused transport loops (0: never)."""
+        check(lib().vgmi_geom_set_regroup(self._h, int(chunk_steps)), "vgmi_geom_set_regroup")
+
     def LocateGlobalPoint(self, points, out=None, precision=FP64):
         n = len(points)
         if out is None:
