@@ -66,7 +66,7 @@ def test_alone_equals_in_the_mix(geo):
 def test_matches_oracle_transport(geo):
     g, G = geo
     O = OracleGeometry(g)
-    (p, d, nsteps), = _requests(g, [3000], seed=32)
+    (p, d, nsteps), = _requests(g, [6000], seed=32)  # above the regrouping threshold
     with Server(G) as S:
         r = S.wait(S.submit(p, d, nsteps))
     nav0 = O.locate(p, threads=8)

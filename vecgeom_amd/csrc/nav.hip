@@ -83,6 +83,7 @@ struct PropArgs {
   int32_t *taken;
   size_t n;
   int nsteps;
+  const int32_t *tsteps; // per-track step counts (null: nsteps for every track)
   const uint32_t *perm;  // lane j runs track perm[*first + j] (null: track j)
   const uint32_t *first; // number of leading perm entries to skip (tracks outside the world)
   int accumulate;        // add to taken instead of storing
@@ -105,9 +106,14 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoop
   V3 p(a.px[i], a.py[i], a.pz[i]);
   const V3 d(a.dx[i], a.dy[i], a.dz[i]);
   State st{a.nav[i], a.last[i], false};
+  int todo = a.nsteps;
+  if (a.tsteps) {
+    const int rem = a.tsteps[i] - (a.accumulate ? a.taken[i] : 0);
+    todo          = rem < todo ? rem : todo;
+  }
   double budget = -1.;
   int s         = 0;
-  for (; s < a.nsteps && st.nav != 0; ++s) {
+  for (; s < todo && st.nav != 0; ++s) {
     State out;
     const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk, budget);
     p                 = p + step * d;
@@ -126,17 +132,24 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoop
 constexpr int kSortBlock = 256;
 constexpr int kLocalBins = 8192; // bins kept in LDS (32 KiB); larger key ranges use global atomics
 
-VGMI_DEV uint32_t state_key(GeomView const &g, uint32_t nav)
-{
-  return nav ? uint32_t(g.pvols[top(g, nav)].lv) + 1u : 0u;
-}
+// Key 0 also takes tracks that have used up their own step count.
+struct SortKeys {
+  const uint32_t *nav;
+  const int32_t *tsteps, *taken; // optional
+  VGMI_DEV uint32_t operator()(GeomView const &g, size_t i) const
+  {
+    const uint32_t s = nav[i];
+    if (s == 0 || (tsteps && taken[i] >= tsteps[i])) return 0u;
+    return uint32_t(g.pvols[top(g, s)].lv) + 1u;
+  }
+};
 
 __global__ void k_fill_u32(uint32_t *a, size_t n, uint32_t v)
 {
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) a[i] = v;
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_sort_hist(GeomView g, const uint32_t *nav, size_t n, uint32_t *hist,
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(GeomView g, SortKeys key, size_t n, uint32_t *hist,
                                                           int nbins)
 {
   extern __shared__ uint32_t sh[];
@@ -145,7 +158,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(GeomView g, const uint
     for (int b = threadIdx.x; b < nbins; b += kSortBlock) sh[b] = 0;
   __syncthreads();
   for (size_t i = size_t(blockIdx.x) * kSortBlock + threadIdx.x; i < n; i += size_t(gridDim.x) * kSortBlock) {
-    const uint32_t k = state_key(g, nav[i]);
+    const uint32_t k = key(g, i);
     if (local)
       atomicAdd(&sh[k], 1u);
     else
@@ -183,12 +196,12 @@ __global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *hist, int nbins)
 }
 
 // perm[cursor[key]++] = i, with block-local ranks so that each block reserves one range per key.
-__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, const uint32_t *nav, size_t n,
-                                                             uint32_t *cursor, uint32_t *perm, int nbins)
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, SortKeys key, size_t n, uint32_t *cursor,
+                                                             uint32_t *perm, int nbins)
 {
   extern __shared__ uint32_t sh[];
   const size_t i   = size_t(blockIdx.x) * kSortBlock + threadIdx.x;
-  const uint32_t k = i < n ? state_key(g, nav[i]) : 0u;
+  const uint32_t k = i < n ? key(g, i) : 0u;
   if (nbins > kLocalBins) {
     if (i < n) perm[atomicAdd(&cursor[k], 1u)] = uint32_t(i);
     return;
@@ -205,47 +218,26 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, const u
   if (i < n) perm[sh[k] + rank] = uint32_t(i);
 }
 
-// Serving-loop kernel: locate each track from the world, then run its own number of
-// ComputeStepAndPropagatedState steps (per-track step counts), all in registers.
+// Start of a serving launch: locate each track from the world, copy its point to the output
+// arrays (which the transport loop then updates in place), clear last and taken.
 template <unsigned TM>
-__global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoopWaves<TM>))) void k_locate_propagate(GeomView g, ServeArgs a)
+__global__ __launch_bounds__(kNavBlock) void k_locate_init(GeomView g, const double *px, const double *py,
+                                                           const double *pz, double *opx, double *opy, double *opz,
+                                                           uint32_t *nav, uint32_t *last, int32_t *taken, size_t n)
 {
   VGMI_NAV_STACK;
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
-  if (i >= a.n) return;
-  V3 p(a.px[i], a.py[i], a.pz[i]);
-  const V3 d(a.dx[i], a.dy[i], a.dz[i]);
-  State st{locate_global<TM>(g, p, stk), 0u, false};
-  const int nsteps = a.nsteps[i];
-  double budget    = -1.;
-  int s            = 0;
-  for (; s < nsteps && st.nav != 0; ++s) {
-    State out;
-    const double step = compute_step<TM>(g, p, d, a.limit, st, out, true, stk, budget);
-    p                 = p + step * d;
-    st                = out;
-  }
-  a.opx[i]    = p.x;
-  a.opy[i]    = p.y;
-  a.opz[i]    = p.z;
-  a.onav[i]   = st.nav;
-  a.otaken[i] = s;
+  if (i >= n) return;
+  const V3 p(px[i], py[i], pz[i]);
+  opx[i]  = p.x;
+  opy[i]  = p.y;
+  opz[i]  = p.z;
+  nav[i]  = locate_global<TM>(g, p, stk);
+  last[i] = 0u;
+  taken[i] = 0;
 }
 
 } // namespace nav
-
-vgmi_status launch_locate_propagate(const vgmi_geom *geom, ServeArgs const &a, hipStream_t st)
-{
-  if (a.n == 0) return VGMI_OK;
-  const dim3 grid(unsigned((a.n + nav::kNavBlock - 1) / nav::kNavBlock));
-  if ((geom->typeMask & ~nav::kSimpleTypes) == 0u)
-    hipLaunchKernelGGL(nav::k_locate_propagate<nav::kSimpleTypes>, grid, dim3(nav::kNavBlock), 0, st, geom->view, a);
-  else
-    hipLaunchKernelGGL(nav::k_locate_propagate<nav::kAllTypes>, grid, dim3(nav::kNavBlock), 0, st, geom->view, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "serving launch: %s", hipGetErrorString(e));
-  return VGMI_OK;
-}
 
 } // namespace vgmi
 
@@ -283,6 +275,57 @@ static vgmi_status launched(const char *who)
   if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "%s launch: %s", who, hipGetErrorString(e));
   return VGMI_OK;
 }
+
+namespace vgmi {
+
+size_t transport_work_words(const vgmi_geom *geom, size_t n)
+{
+  if (geom->regroupSteps <= 0 || n < kRegroupMinTracks) return 0;
+  return size_t(geom->nlogical) + 1 + n;
+}
+
+vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint32_t *work, hipStream_t st)
+{
+  const int chunk = geom->regroupSteps;
+  const size_t n  = t.n;
+  if (n == 0 || t.nsteps <= 0) return VGMI_OK;
+  if (!work || chunk <= 0 || n < kRegroupMinTracks) {
+    PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, t.nsteps, t.track_steps,
+               nullptr, nullptr, 0};
+    VGMI_NAV_LAUNCH(k_propagate, geom, n, st, geom->view, a);
+    return launched("transport loop");
+  }
+  // Chunks of `chunk` steps; before each, the tracks still running are regrouped by the
+  // volume they are in (a counting sort into perm; key 0, outside or done, is skipped).
+  const int nbins  = geom->nlogical + 1;
+  uint32_t *cursor = work, *perm = work + nbins;
+  const size_t lds = nbins <= kLocalBins ? size_t(nbins) * sizeof(uint32_t) : 0;
+  const unsigned hgrid = unsigned(std::min<size_t>((n + kSortBlock - 1) / kSortBlock, 2048));
+  const unsigned sgrid = unsigned((n + kSortBlock - 1) / kSortBlock);
+  if (t.taken) hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint32_t *>(t.taken), n, 0u);
+  const SortKeys key{t.nav, t.track_steps, t.taken};
+  for (int done = 0; done < t.nsteps; done += chunk) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(unsigned((nbins + 255) / 256)), dim3(256), 0, st, cursor, size_t(nbins), 0u);
+    hipLaunchKernelGGL(k_sort_hist, dim3(hgrid), dim3(kSortBlock), lds, st, geom->view, key, n, cursor, nbins);
+    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, cursor, nbins);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, key, n, cursor, perm, nbins);
+    PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, std::min(chunk, t.nsteps - done),
+               t.track_steps, perm, cursor, 1};
+    VGMI_NAV_LAUNCH(k_propagate, geom, n, st, geom->view, a);
+  }
+  return launched("transport loop");
+}
+
+vgmi_status launch_locate_init(const vgmi_geom *geom, const double *px, const double *py, const double *pz,
+                               double *opx, double *opy, double *opz, uint32_t *nav, uint32_t *last,
+                               int32_t *taken, size_t n, hipStream_t st)
+{
+  if (n == 0) return VGMI_OK;
+  VGMI_NAV_LAUNCH(k_locate_init, geom, n, st, geom->view, px, py, pz, opx, opy, opz, nav, last, taken, n);
+  return launched("serving locate");
+}
+
+} // namespace vgmi
 
 extern "C" {
 
@@ -331,34 +374,15 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_propagate: null pointer");
   if (n > 0xffffffffull) return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_propagate: more than 2^32 tracks");
   hipStream_t st = (hipStream_t)stream;
-  const int chunk = geom->regroupSteps;
-  if (chunk <= 0 || n < kRegroupMinTracks) {
-    PropArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, n, nsteps, nullptr, nullptr, 0};
-    VGMI_NAV_LAUNCH(k_propagate, geom, n, stream, geom->view, a);
-    return launched("vgmi_nav_propagate");
+  TransportArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, nullptr, n, nsteps};
+  uint32_t *work = nullptr;
+  if (const size_t words = transport_work_words(geom, n)) {
+    if (hipMallocAsync(reinterpret_cast<void **>(&work), words * sizeof(uint32_t), st) != hipSuccess)
+      return set_error(VGMI_ERR_DEVICE, "vgmi_nav_propagate: workspace of %zu words", words);
   }
-  // Chunks of `chunk` steps; before each, tracks are regrouped by the volume they are in.
-  const int nbins     = geom->nlogical + 1;
-  const size_t wbytes = (size_t(nbins) + n) * sizeof(uint32_t);
-  uint32_t *work      = nullptr;
-  if (hipMallocAsync(reinterpret_cast<void **>(&work), wbytes, st) != hipSuccess)
-    return set_error(VGMI_ERR_DEVICE, "vgmi_nav_propagate: workspace of %zu bytes", wbytes);
-  uint32_t *cursor = work, *perm = work + nbins;
-  const size_t lds = nbins <= kLocalBins ? size_t(nbins) * sizeof(uint32_t) : 0;
-  const unsigned hgrid = unsigned(std::min<size_t>((n + kSortBlock - 1) / kSortBlock, 2048));
-  const unsigned sgrid = unsigned((n + kSortBlock - 1) / kSortBlock);
-  if (steps_taken) hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint32_t *>(steps_taken), n, 0u);
-  for (int done = 0; done < nsteps; done += chunk) {
-    hipLaunchKernelGGL(k_fill_u32, dim3(unsigned((nbins + 255) / 256)), dim3(256), 0, st, cursor, size_t(nbins), 0u);
-    hipLaunchKernelGGL(k_sort_hist, dim3(hgrid), dim3(kSortBlock), lds, st, geom->view, nav, n, cursor, nbins);
-    hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, cursor, nbins);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, nav, n, cursor, perm, nbins);
-    PropArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, n, std::min(chunk, nsteps - done), perm,
-               cursor, 1};
-    VGMI_NAV_LAUNCH(k_propagate, geom, n, stream, geom->view, a);
-  }
-  (void)hipFreeAsync(work, st);
-  return launched("vgmi_nav_propagate");
+  const vgmi_status rc = launch_transport(geom, a, work, st);
+  if (work) (void)hipFreeAsync(work, st);
+  return rc;
 }
 
 vgmi_status vgmi_geom_set_regroup(vgmi_geom *geom, int chunk_steps)

@@ -27,8 +27,9 @@ using Clock = std::chrono::steady_clock;
 
 namespace {
 
-// Bytes per track in a lane: 6 doubles in, step count in, 3 doubles + state + steps taken out.
-constexpr size_t kTrackBytes = 6 * 8 + 4 + 3 * 8 + 4 + 4;
+// Bytes per track in a lane: 6 doubles in, step count in, 3 doubles + state + steps taken
+// out, and the device-only "last exited" state.
+constexpr size_t kTrackBytes = 6 * 8 + 4 + 3 * 8 + 4 + 4 + 4;
 
 struct Request {
   const double *px, *py, *pz, *dx, *dy, *dz;
@@ -47,7 +48,9 @@ struct Lane {
   hipEvent_t done    = nullptr;
   char *hstage       = nullptr; // pinned
   char *dstage       = nullptr;
+  uint32_t *dwork    = nullptr; // regrouping workspace
   int64_t cap        = 0;
+  int maxSteps       = 0;       // largest step count of the current batch
   bool busy          = false; // owns a batch (from packing until its results are unpacked)
   bool issued        = false; // its copies, launch and event record are on the stream
   int status         = 0;
@@ -56,7 +59,7 @@ struct Lane {
   std::vector<int64_t> offset;
   // SoA sections of the staging buffers for the current batch (same layout on host and
   // device): inputs packed contiguously from offset 0, outputs from outBegin.
-  size_t oPx, oPy, oPz, oDx, oDy, oDz, oSteps, inEnd, oOpx, oOpy, oOpz, oNav, oTaken, outBegin, outEnd;
+  size_t oPx, oPy, oPz, oDx, oDy, oDz, oSteps, inEnd, oOpx, oOpy, oOpz, oNav, oTaken, outBegin, outEnd, oLast;
 };
 
 } // namespace
@@ -101,6 +104,7 @@ void layout(Lane &L, size_t n)
   L.oNav   = take(4 * n);
   L.oTaken = take(4 * n);
   L.outEnd = o;
+  L.oLast  = take(4 * n); // device only
 }
 
 void dispatch_loop(vgmi_server *S)
@@ -140,7 +144,8 @@ void dispatch_loop(vgmi_server *S)
     }
     layout(*L, size_t(L->ntracks));
     // pack inputs (outside the lock)
-    char *h = L->hstage;
+    char *h     = L->hstage;
+    L->maxSteps = 0;
     for (size_t k = 0; k < L->batch.size(); ++k) {
       Request const *r = L->batch[k];
       const size_t o = size_t(L->offset[k]);
@@ -151,17 +156,20 @@ void dispatch_loop(vgmi_server *S)
       std::memcpy(h + L->oDy + 8 * o, r->dy, 8 * r->n);
       std::memcpy(h + L->oDz + 8 * o, r->dz, 8 * r->n);
       std::memcpy(h + L->oSteps + 4 * o, r->nsteps, 4 * r->n);
+      for (int64_t i = 0; i < r->n; ++i) L->maxSteps = r->nsteps[i] > L->maxSteps ? r->nsteps[i] : L->maxSteps;
     }
-    char *d = L->dstage;
-    ServeArgs a{reinterpret_cast<const double *>(d + L->oPx),  reinterpret_cast<const double *>(d + L->oPy),
-                reinterpret_cast<const double *>(d + L->oPz),  reinterpret_cast<const double *>(d + L->oDx),
-                reinterpret_cast<const double *>(d + L->oDy),  reinterpret_cast<const double *>(d + L->oDz),
-                reinterpret_cast<const int32_t *>(d + L->oSteps), S->limit,
-                reinterpret_cast<double *>(d + L->oOpx),      reinterpret_cast<double *>(d + L->oOpy),
-                reinterpret_cast<double *>(d + L->oOpz),      reinterpret_cast<uint32_t *>(d + L->oNav),
-                reinterpret_cast<int32_t *>(d + L->oTaken),   size_t(L->ntracks)};
+    char *d        = L->dstage;
+    const size_t n = size_t(L->ntracks);
+    auto D         = [&](size_t off) { return reinterpret_cast<double *>(d + off); };
+    auto U         = [&](size_t off) { return reinterpret_cast<uint32_t *>(d + off); };
+    auto I         = [&](size_t off) { return reinterpret_cast<int32_t *>(d + off); };
+    TransportArgs a{D(L->oOpx), D(L->oOpy), D(L->oOpz), D(L->oDx),    D(L->oDy),       D(L->oDz), S->limit,
+                    U(L->oNav), U(L->oLast), I(L->oTaken), I(L->oSteps), n,               L->maxSteps};
     hipError_t e = hipMemcpyAsync(d, h, L->inEnd, hipMemcpyHostToDevice, L->stream);
-    int st       = e == hipSuccess ? launch_locate_propagate(S->geom, a, L->stream) : VGMI_ERR_DEVICE;
+    int st       = e == hipSuccess ? launch_locate_init(S->geom, D(L->oPx), D(L->oPy), D(L->oPz), a.px, a.py, a.pz,
+                                                        a.nav, a.last, a.taken, n, L->stream)
+                                   : VGMI_ERR_DEVICE;
+    if (st == VGMI_OK) st = launch_transport(S->geom, a, L->dwork, L->stream);
     if (st == VGMI_OK)
       e = hipMemcpyAsync(h + L->outBegin, d + L->outBegin, L->outEnd - L->outBegin, hipMemcpyDeviceToHost, L->stream);
     if (st == VGMI_OK && e == hipSuccess) e = hipEventRecord(L->done, L->stream);
@@ -265,6 +273,8 @@ vgmi_status vgmi_server_create(const vgmi_geom *geom, int precision, double step
     if (e == hipSuccess) e = hipEventCreateWithFlags(&L.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&L.hstage), bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&L.dstage), bytes);
+    const size_t words = transport_work_words(geom, size_t(lane_tracks));
+    if (e == hipSuccess && words) e = hipMalloc(reinterpret_cast<void **>(&L.dwork), words * sizeof(uint32_t));
     if (e != hipSuccess) {
       vgmi_server_destroy(S);
       return set_error(VGMI_ERR_DEVICE, "vgmi_server_create: %s", hipGetErrorString(e));
@@ -345,6 +355,7 @@ vgmi_status vgmi_server_destroy(vgmi_server *S)
   for (auto &L : S->lanes) {
     if (L.stream) (void)hipStreamSynchronize(L.stream);
     if (L.dstage) (void)hipFree(L.dstage);
+    if (L.dwork) (void)hipFree(L.dwork);
     if (L.hstage) (void)hipHostFree(L.hstage);
     if (L.done) (void)hipEventDestroy(L.done);
     if (L.stream) (void)hipStreamDestroy(L.stream);

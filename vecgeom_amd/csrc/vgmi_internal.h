@@ -12,17 +12,25 @@ struct vgmi_shape {
 };
 
 namespace vgmi {
-// Device arrays of one serving launch (inputs, per-track step counts, outputs).
-struct ServeArgs {
-  const double *px, *py, *pz, *dx, *dy, *dz;
-  const int32_t *nsteps;
+// Device arrays of one fused transport launch; tracks are updated in place.
+struct TransportArgs {
+  double *px, *py, *pz;
+  const double *dx, *dy, *dz;
   double limit;
-  double *opx, *opy, *opz;
-  uint32_t *onav;
-  int32_t *otaken;
+  uint32_t *nav, *last;
+  int32_t *taken;              // steps done per track (required with track_steps)
+  const int32_t *track_steps;  // per-track step counts, or null for a uniform count
   size_t n;
+  int nsteps;                  // the uniform step count, or the largest of track_steps
 };
-vgmi_status launch_locate_propagate(const vgmi_geom *geom, ServeArgs const &a, hipStream_t st);
+// Words of device workspace launch_transport needs to regroup n tracks.
+size_t transport_work_words(const vgmi_geom *geom, size_t n);
+// The transport loop, regrouped by volume when `work` is given and the geometry asks for it.
+vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &a, uint32_t *work, hipStream_t st);
+// Locates every point (p_in -> p_out copy, nav), clears last and taken.
+vgmi_status launch_locate_init(const vgmi_geom *geom, const double *px, const double *py, const double *pz,
+                               double *opx, double *opy, double *opz, uint32_t *nav, uint32_t *last,
+                               int32_t *taken, size_t n, hipStream_t st);
 
 // Records the message for vgmi_last_error() and returns `code`.
 int set_error(int code, const char *fmt, ...);
