@@ -5,6 +5,8 @@
 //   0 box         dx dy dz                                   (volumes/UnplacedBox.h)
 //   1 tube        rmin rmax dz sphi dphi                     (volumes/UnplacedTube.h:29)
 //   5 trapezoid   dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2 (volumes/UnplacedTrapezoid.h:56)
+//   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
+//                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
 #include "shapes.h"
 #include <chrono>
 #include <thread>
@@ -40,6 +42,17 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 5 ? new Torus2(a[0], a[1], a[2], a[3], a[4]) : nullptr;
   case 6:
     return n >= 6 ? new Sphere(a[0], a[1], a[2], a[3], a[4], a[5]) : nullptr;
+  case 9: {
+    if (n < 1) return nullptr;
+    const int ntri = int(a[0]);
+    if (ntri < 4 || n < 1 + 9 * ntri) return nullptr;
+    auto *t = new Tessellated(a + 1, ntri);
+    if (!t->ok) {
+      delete t;
+      return nullptr;
+    }
+    return t;
+  }
   case 5:
     return n >= 11 ? new Trapezoid(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]) : nullptr;
   default:

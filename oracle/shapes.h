@@ -338,3 +338,45 @@ public:
 unsigned SolveCubic(double a, double b, double c, double *x);
 int SolveQuartic(double a, double b, double c, double d, double *x);
 } // namespace vgo
+
+namespace vgo {
+// ---------------------------------------------------------------- Tessellated
+// Triangular facets (VecGeom/volumes/Tile.h:58-160, TessellatedStruct<3,double>), queried through
+// the scalar backend's one-facet clusters (VecGeom/volumes/TessellatedCluster.h:183-300,:470-540) and
+// TessellatedImplementation.h:54-300. The reference loops over the clusters in the order of their
+// bounding-box hit distance (HybridNavigator2.h:111-190) and keeps the first of equal distances;
+// the restatement visits every facet that passes the same single-precision box test and keeps,
+// among equal distances, the lowest facet index.
+struct TriFacet {
+  V3 v[3];     // vertices, anticlockwise seen from outside
+  V3 side[3];  // cluster side vectors: normal x (v[i+1] - v[i]), normalised
+  V3 n;        // outward unit normal
+  double dist; // plane offset: n.p + dist = signed distance
+  float blo[3], bhi[3]; // bounding box of the facet in single precision (ABBoxManager::ABBox_s)
+};
+class Tessellated : public Shape {
+public:
+  std::vector<TriFacet> f;
+  V3 minExt, maxExt, testDir;
+  bool ok = false;
+  // tri: 9 doubles per triangle (v0, v1, v2)
+  Tessellated(const double *tri, int ntri);
+  void DistanceToSolid(bool toIn, V3 const &p, V3 const &d, double stepMax, double &distance, int &isurf,
+                       double &distother, int &isurfother) const;
+  double SafetySq(bool toIn, V3 const &p, int &isurf) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override
+  {
+    lo = minExt;
+    hi = maxExt;
+  }
+};
+// Direction used by Contains/Inside: the first of a fixed list with |n.dir| >= 1e-8 for every
+// facet (the reference draws a random direction once, TessellatedStruct.h:380-390).
+V3 TessellatedTestDir(std::vector<TriFacet> const &f);
+} // namespace vgo

@@ -168,6 +168,25 @@ def statements(text):
 METHODS = "DistanceToIn|DistanceToOut|SafetyToIn|SafetyToOut|Inside|Contains"
 
 
+def trd_like_triangles(x1, x2, y1, y2, z):
+    """Triangles of test/unit_tests/TestTessellated.cpp:18-38 CreateTrdLikeTessellated: six
+    quadrilateral facets, each added as the triangles (v0, v1, v2) and (v0, v2, v3)
+    (TessellatedStruct::AddQuadrilateralFacet). Returns [ntri, 9 doubles per triangle]."""
+    quads = [
+        [(-x2, y2, z), (-x2, -y2, z), (x2, -y2, z), (x2, y2, z)],
+        [(-x1, y1, -z), (x1, y1, -z), (x1, -y1, -z), (-x1, -y1, -z)],
+        [(-x2, -y2, z), (-x1, -y1, -z), (x1, -y1, -z), (x2, -y2, z)],
+        [(x2, -y2, z), (x1, -y1, -z), (x1, y1, -z), (x2, y2, z)],
+        [(x2, y2, z), (x1, y1, -z), (-x1, y1, -z), (-x2, y2, z)],
+        [(-x2, y2, z), (-x1, y1, -z), (-x1, -y1, -z), (-x2, -y2, z)],
+    ]
+    tris = []
+    for q in quads:
+        tris.append([q[0], q[1], q[2]])
+        tris.append([q[0], q[2], q[3]])
+    return [float(len(tris))] + [float(c) for t in tris for v in t for c in v]
+
+
 def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
     text = open(path).read()
     vecs, scalars, shapes, pending, arrays, extents = {}, {}, {}, {}, {}, {}
@@ -200,6 +219,15 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                 break
             st = st2
         st = re.sub(r"^\s*(?:const\s+)?", "", st)
+        # tessellated solids made by the test's CreateTrdLikeTessellated helper
+        m = re.match(r"(?:vecgeom::)?SimpleTessellated\s*&\s*(\w+)\s*=\s*\*\s*CreateTrdLikeTessellated\s*<\s*\w+\s*>"
+                     r"\s*\((.*)\)$", st)
+        if m and shape_t == "SimpleTessellated":
+            try:
+                shapes[m.group(1)] = trd_like_triangles(*shape_params(m.group(2)))
+            except Unsupported:
+                shapes.pop(m.group(1), None)
+            continue
         # shape definitions
         m = re.match(r"(?:%s)\s*\*?\s*(\w+)\s*(?:=\s*new\s+(?:%s)\s*)?\((.*)\)$" % (shape_t, shape_t), st)
         if m:
@@ -350,6 +378,13 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                 except Unsupported:
                     rec = None
             else:
+                if "&&" in body and "||" not in body:
+                    # a conjunction: every scalar ApproxEqual(var, value) term on a stored result is
+                    # a requirement of its own (the other terms, e.g. on normals, are skipped)
+                    terms = [t.strip() for t in body.split("&&")]
+                    body = next((t for t in terms
+                                 if re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,.*\)", t)
+                                 and re.match(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)", t).group(1) in pending), body)
                 alts = re.findall(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,\s*((?:[^()]|\([^()]*(?:\([^()]*\))?[^()]*\))+)\)", body)
                 if alts and all(a[0] == alts[0][0] for a in alts) and alts[0][0] in pending and "&&" not in body:
                     var = alts[0][0]
@@ -415,6 +450,7 @@ SHAPES = {
     "sphere": ("TestSphere.cpp", r"Sphere_t", (6,)),
     "polyhedron": ("TestPolyhedra.cpp", r"Polyhedra_t", "polyhedron"),
     "torus2": ("TestTorus2.cpp", r"Torus_t", (5,)),
+    "tessellated": ("TestTessellated.cpp", r"SimpleTessellated", "tessellated"),
 }
 
 if __name__ == "__main__":
@@ -426,6 +462,8 @@ if __name__ == "__main__":
         recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat)
         if counts is None:  # sphi dphi nz z[nz] rmin[nz] rmax[nz]
             recs = [r for r in recs if len(r["shape"]) >= 3 and len(r["shape"]) == 3 + 3 * int(r["shape"][2])]
+        elif counts == "tessellated":  # ntri, 9 doubles per triangle
+            recs = [r for r in recs if len(r["shape"]) == 1 + 9 * int(r["shape"][0])]
         elif counts == "polyhedron":  # sphi dphi nsides nz z[nz] rmin[nz] rmax[nz]
             recs = [r for r in recs if len(r["shape"]) >= 4 and len(r["shape"]) == 4 + 3 * int(r["shape"][3])]
         else:

@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("kind,params", CASES, ids=["%s%d" % (c[0], i) for i, c in enumerate(CASES)])
 def test_mixed_decisions_exact_and_errors_bounded(kind, params):
     from vecgeom_amd import MIXED, make_shape
-    print("case", kind, params, flush=True)
+    print("case", kind, params[:12], flush=True)
     o, p, d = gpu_util.inputs_for(kind, params, 1 << 15, seed=99)
     s = make_shape(kind, params)
     P, D = gpu_util.to_dev(p), gpu_util.to_dev(d)

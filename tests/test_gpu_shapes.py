@@ -12,6 +12,8 @@ import pytest
 import torch
 
 from tests import gpu_util, kat_util
+from tests.golden.extract_kats import trd_like_triangles
+from vecgeom_amd import workloads
 
 pytestmark = pytest.mark.gpu
 
@@ -55,6 +57,10 @@ CASES = [
     ("torus2", [2.0, 10.0, 50.0, 0.3, 2.0]),
     ("torus2", [5.0, 10.0, 50.0, 1.0, 4.5]),
     ("torus2", [100.0, 150.0, 200.0, 0.0, math.pi / 3]),
+    ("tessellated", workloads.convex_hull_sphere(20.0)),
+    ("tessellated", trd_like_triangles(10, 30, 20, 40, 40)),
+    ("tessellated", trd_like_triangles(0.15, 0.15, 24.707, 24.707, 22.7)),
+    ("tessellated", workloads.convex_hull_sphere(500.0, nphi=12, nbands=7)),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing stops once a step is below
@@ -67,7 +73,7 @@ METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", 
 @pytest.mark.parametrize("kind,params", CASES, ids=["%s%d" % (c[0], i) for i, c in enumerate(CASES)])
 def test_fp64_parity(kind, params):
     from vecgeom_amd import make_shape
-    print("case", kind, params, flush=True)
+    print("case", kind, params[:12], flush=True)
     o, p, d = gpu_util.inputs_for(kind, params, 1 << 15, seed=zlib.crc32(repr((kind, params)).encode()))
     s = make_shape(kind, params)
     P, D = gpu_util.to_dev(p), gpu_util.to_dev(d)
@@ -96,7 +102,7 @@ def test_query_all_matches_single_methods():
 
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
-                                   "torus2"])
+                                   "torus2", "tessellated"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

@@ -8,7 +8,7 @@ import pytest
 from oracle.pyoracle import OracleShape
 from tests import kat_util
 
-SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2"]
+SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated"]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

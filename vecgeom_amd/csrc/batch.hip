@@ -50,6 +50,42 @@ VGMI_DEV void run_tile(Data const &s, int64_t b, int64_t e, BatchOut const &o, i
   }
 }
 
+// The tessellated solid shares one test-ray walk and one safety walk across its six methods.
+template <>
+VGMI_DEV void run_tile<TessellatedData>(TessellatedData const &s, int64_t b, int64_t e, BatchOut const &o, int nthreads)
+{
+  for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
+    const Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
+                         __builtin_nontemporal_load(o.pz + i));
+    if (!(o.contains && o.inside && o.sin && o.sout && o.din && o.dout)) {
+      // a partial output set: the methods one by one
+      using S = ShapeOps<TessellatedData>;
+      if (o.contains) __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), o.contains + i);
+      if (o.inside) __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), o.inside + i);
+      if (o.sin) __builtin_nontemporal_store(S::template SafeIn<double>(s, p), o.sin + i);
+      if (o.sout) __builtin_nontemporal_store(S::template SafeOut<double>(s, p), o.sout + i);
+      if (o.din || o.dout) {
+        const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
+        if (o.din) __builtin_nontemporal_store(S::template DistIn<double>(s, p, d), o.din + i);
+        if (o.dout) __builtin_nontemporal_store(S::template DistOut<double>(s, p, d), o.dout + i);
+      }
+      continue;
+    }
+    const Vec3<double> d(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
+                         __builtin_nontemporal_load(o.dz + i));
+    bool c;
+    int in;
+    double si, so, di, dou;
+    tess::query_all(s, p, d, c, in, si, so, di, dou);
+    __builtin_nontemporal_store(uint8_t(c), o.contains + i);
+    __builtin_nontemporal_store(int32_t(in), o.inside + i);
+    __builtin_nontemporal_store(si, o.sin + i);
+    __builtin_nontemporal_store(so, o.sout + i);
+    __builtin_nontemporal_store(di, o.din + i);
+    __builtin_nontemporal_store(dou, o.dout + i);
+  }
+}
+
 __global__ void k_batch_reset(uint32_t *counter) { *counter = 0u; }
 
 template <int BLOCK>

@@ -7,6 +7,7 @@
 #include "dev_polycone.h"
 #include "dev_polyhedron.h"
 #include "dev_sphere.h"
+#include "dev_tessellated.h"
 #include "dev_torus.h"
 #include "dev_trapezoid.h"
 #include "dev_tube.h"
@@ -21,7 +22,8 @@
   X(kPolycone, PolyconeData, pcon, pcon)                                                                             \
   X(kSphere, SphereData, sph, sph)                                                                                   \
   X(kPolyhedron, PolyhedronData, phed, phed)                                                                         \
-  X(kTorus2, TorusData, tor, tor)
+  X(kTorus2, TorusData, tor, tor)                                                                                    \
+  X(kTessellated, TessellatedData, tess, tess)
 
 namespace vgmi {
 

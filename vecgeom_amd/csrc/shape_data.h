@@ -134,6 +134,23 @@ struct TorusData {
   TubeData bounding;
 };
 
+// Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
+// and a threaded BVH over them live in the variable-length pool:
+//   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
+//     plane offset, v0, side0, v1, side1, v2, side2, facet index (as a double);
+//   node j (kNodeDoubles doubles): box lo[3] hi[3] (enlarged by a margin covering the
+//     single-precision facet boxes), then int32 first facet, facet count (0: internal node),
+//     escape node (next node in depth-first order after this subtree), unused.
+struct TessellatedData {
+  int32_t nfacets, nnodes;
+  int32_t facetOff, nodeOff;
+  const double *base;
+  double minExt[3], maxExt[3];
+  double testDir[3]; // direction of the Contains/Inside ray
+};
+constexpr int kFacetDoubles = 26;
+constexpr int kNodeDoubles  = 8;
+
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
@@ -150,6 +167,7 @@ struct ShapeRecord {
     SphereData sph;
     PolyhedronData phed;
     TorusData tor;
+    TessellatedData tess;
     double raw[64];
   } u;
 };

@@ -777,6 +777,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakePolyhedron(params, nparams, r, extra, err);
   case kTorus2:
     return MakeTorus2(params, nparams, r, err);
+  case kTessellated:
+    return MakeTessellated(params, nparams, r, extra, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -787,6 +789,7 @@ void SetShapeExtraBase(ShapeRecord &r, const double *device_base)
 {
   if (r.type == kPolycone) r.u.pcon.base = device_base;
   if (r.type == kPolyhedron) r.u.phed.base = device_base;
+  if (r.type == kTessellated) r.u.tess.base = device_base;
 }
 
 void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
@@ -831,6 +834,9 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     return;
   case kCone:
     ConeExtent(r.u.cone, lo, hi);
+    return;
+  case kTessellated:
+    for (int i = 0; i < 3; ++i) lo[i] = r.u.tess.minExt[i], hi[i] = r.u.tess.maxExt[i];
     return;
   default:
     lo[0] = lo[1] = lo[2] = 0;

@@ -65,11 +65,47 @@ def near_surface(g, n, lo, hi, contains_fn, dist_out_fn, band=1e-3):
     return np.ascontiguousarray(out[:, :n])
 
 
+def convex_hull_sphere(radius=20.0, nphi=32, nbands=17):
+    """Tessellated convex hull of a UV sphere: nphi longitude segments and nbands latitude
+    bands (two triangle caps, the other bands quadrilaterals split into two triangles), so
+    nphi * (2 * nbands - 2) triangles -- 1024 for the defaults. Vertices are ordered
+    anticlockwise seen from outside. Returns the tessellated parameter list
+    [ntri, then 9 doubles per triangle]."""
+    theta = np.pi * np.arange(nbands + 1) / nbands
+    phi = 2.0 * np.pi * np.arange(nphi) / nphi
+
+    def vert(i, j):
+        if i == 0:
+            return np.array([0.0, 0.0, radius])
+        if i == nbands:
+            return np.array([0.0, 0.0, -radius])
+        t, f = theta[i], phi[j % nphi]
+        return radius * np.array([np.sin(t) * np.cos(f), np.sin(t) * np.sin(f), np.cos(t)])
+
+    tris = []
+
+    def add(a, b, c):
+        n = np.cross(b - a, c - b)
+        if np.dot(n, a + b + c) < 0:  # make the normal point outwards
+            b, c = c, b
+        tris.append((a, b, c))
+
+    for j in range(nphi):
+        add(vert(0, 0), vert(1, j), vert(1, j + 1))
+        add(vert(nbands, 0), vert(nbands - 1, j + 1), vert(nbands - 1, j))
+        for i in range(1, nbands - 1):
+            a, b, c, d = vert(i, j), vert(i + 1, j), vert(i + 1, j + 1), vert(i, j + 1)
+            add(a, b, c)
+            add(a, c, d)
+    return [float(len(tris))] + [float(x) for t in tris for v in t for x in v]
+
+
 # Batched shape set evaluated in one launch (the shapes implemented so far; each
 # entry is (kind, constructor parameters)). Tube: rmin 2, rmax 8, dz 10, phi 30-270 deg;
 # polycone with 10 z planes; general trapezoid; cone with an inner radius and slopes;
 # 8-sided polyhedron with 6 z planes (inner radii in the middle planes); sphere shell
-# with theta (30-130 deg) and phi (20-290 deg) cuts; Torus2 with rtor 50, rmax 10.
+# with theta (30-130 deg) and phi (20-290 deg) cuts; Torus2 with rtor 50, rmax 10; orb;
+# tessellated convex hull of 1024 triangles (a UV sphere of radius 20).
 def mix_shapes():
     r = np.radians
     return [
@@ -87,6 +123,7 @@ def mix_shapes():
         ("sphere", [5.0, 10.0, float(r(20)), float(r(270)), float(r(30)), float(r(100))]),
         ("torus2", [0.0, 10.0, 50.0, 0.0, float(2 * np.pi)]),
         ("orb", [10.0]),
+        ("tessellated", convex_hull_sphere(20.0)),
     ]
 
 
