@@ -115,7 +115,7 @@ def _time_launch(fn, reps):
     return a.elapsed_time(b) * 1e3 / reps
 
 
-def autotune(batch, points, dirs, outs, tiles=(1024, 2048, 4096, 8192, 16384), blocks=(128, 256, 512),
+def autotune(batch, points, dirs, outs, tiles=(128, 256, 512, 1024, 2048, 4096), blocks=(64, 128, 256, 512),
              blocks_per_cu=(2, 4, 8, 16), reps=5, cache=DEFAULT_CACHE, use_cache=True):
     """Pick the fastest launch shape for this mix on this GPU; returns (config, timings)."""
     gpu = torch.cuda.get_device_name(0)
