@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--nav-steps", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=6.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mix", action="store_true", help="skip the batched mix (the line then has value null)")
     ap.add_argument("--no-nav", action="store_true")
     ap.add_argument("--no-box-tube", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -244,31 +245,40 @@ def run_nav(args):
         nav.copy_(nav0)
         last.zero_()
 
+    from vecgeom_amd.shapes import FP64, MIXED
     s = torch.cuda.current_stream()
-    for _ in range(max(1, args.warmup)):
-        reset()
-        G.Propagate(P, D, limit, nav, last, nsteps, taken)
-    torch.cuda.synchronize()
-    steps_per_run = int(taken.sum())
-    total_us = 0.0
-    reps = max(3, args.steps // 4)
-    for _ in range(reps):
-        reset()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(s)
-        G.Propagate(P, D, limit, nav, last, nsteps, taken)
-        b.record(s)
-        torch.cuda.synchronize()
-        total_us += a.elapsed_time(b) * 1e3
-    avg_us = total_us / reps
     bytes_launch = 88 * n  # per track: read p, d, state (56 B), write p, state (32 B)
-    gbs = bytes_launch / (avg_us * 1e-6) / 1e9
-    res = {"nav_steps_per_s": round(steps_per_run / (avg_us * 1e-6), 1), "avg_launch_us": round(avg_us, 2),
-           "steps_per_launch": steps_per_run, "tracks": n, "nsteps": nsteps, "step_limit_mm": limit,
-           "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_propagate",
-                        "bytes_per_launch": bytes_launch,
-                        "note": "latency/divergence bound: track state stays in registers for all steps"}}
+    res = {"tracks": n, "nsteps": nsteps, "step_limit_mm": limit}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        for _ in range(max(1, args.warmup)):
+            reset()
+            G.Propagate(P, D, limit, nav, last, nsteps, taken, precision=prec)
+        torch.cuda.synchronize()
+        steps_per_run = int(taken.sum())
+        total_us = 0.0
+        reps = max(3, args.steps // 4)
+        for _ in range(reps):
+            reset()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(s)
+            G.Propagate(P, D, limit, nav, last, nsteps, taken, precision=prec)
+            b.record(s)
+            torch.cuda.synchronize()
+            total_us += a.elapsed_time(b) * 1e3
+        avg_us = total_us / reps
+        gbs = bytes_launch / (avg_us * 1e-6) / 1e9
+        res[name] = {"nav_steps_per_s": round(steps_per_run / (avg_us * 1e-6), 1), "avg_launch_us": round(avg_us, 2),
+                     "steps_per_launch": steps_per_run,
+                     "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_propagate",
+                                  "bytes_per_launch": bytes_launch,
+                                  "note": "latency/divergence bound: track state stays in registers for all steps"}}
+        if name == "mixed":
+            res[name]["final_state_equal_fp64"] = bool(torch.equal(nav, nav_fp64) and torch.equal(P.x, px_fp64))
+        else:
+            nav_fp64, px_fp64 = nav.clone(), P.x.clone()
+    res["nav_steps_per_s"] = res["fp64"]["nav_steps_per_s"]
+    res["mixed_over_fp64"] = round(res["mixed"]["nav_steps_per_s"] / res["fp64"]["nav_steps_per_s"], 3)
     if not args.no_cpu:
         from oracle.pyoracle import OracleGeometry
         O = OracleGeometry(g)
@@ -347,8 +357,11 @@ def main():
     if args.copy_check:
         run_copy_check()
         return
-    print("bench: batched shape mix", file=sys.stderr, flush=True)
-    mix = run_mix(args)
+    mix = {"value": None, "ms_per_step": None, "config": {"workload": "batched shape mix skipped (--no-mix)"},
+           "roofline": None}
+    if not args.no_mix:
+        print("bench: batched shape mix", file=sys.stderr, flush=True)
+        mix = run_mix(args)
     extra = {}
     if not args.no_box_tube:
         print("bench: box/tube", file=sys.stderr, flush=True)

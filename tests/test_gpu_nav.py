@@ -1,8 +1,10 @@
-"""GPU parity of the FP64 BVH navigation kernels against the CPU oracle.
+"""GPU parity of the BVH navigation kernels (FP64 and mixed precision) against the CPU oracle.
 
 Bar (from the north star): next-volume indices / navigation states identical, FP64
 steps and safeties within kTolerance (expected bit-identical: the kernels evaluate
-the same FP64 operations as the oracle, without contraction).
+the same FP64 operations as the oracle, without contraction). The mixed-precision
+kernels use FP32 box tests only as conservative filters in front of the same FP64
+decisions, so they are held to the same bit-identical bar.
 """
 import numpy as np
 import pytest
@@ -10,6 +12,7 @@ import torch
 
 from oracle.pyoracle import OracleGeometry
 from vecgeom_amd import SOA3D, geometry
+from vecgeom_amd.shapes import FP64, MIXED
 
 pytestmark = pytest.mark.gpu
 
@@ -26,15 +29,20 @@ def _dev_u32(a):
     return torch.as_tensor(np.asarray(a).astype(np.int32), device="cuda")
 
 
-def test_locate(geo):
+PRECISIONS = pytest.mark.parametrize("prec", [FP64, MIXED], ids=["fp64", "mixed"])
+
+
+@PRECISIONS
+def test_locate(geo, prec):
     g, G, O = geo
     p, _ = geometry.world_points(g, 1 << 16, seed=21)
-    got = G.LocateGlobalPoint(SOA3D.from_array(p)).cpu().numpy().astype(np.uint32)
+    got = G.LocateGlobalPoint(SOA3D.from_array(p), precision=prec).cpu().numpy().astype(np.uint32)
     ref = O.locate(p, threads=8)
     assert np.array_equal(got, ref), int((got != ref).sum())
 
 
-def test_step_by_step(geo):
+@PRECISIONS
+def test_step_by_step(geo, prec):
     g, G, O = geo
     n = 1 << 14
     p, d = geometry.world_points(g, n, seed=22)
@@ -45,7 +53,7 @@ def test_step_by_step(geo):
         pp, nv, ls = p.copy(), nav.copy(), last.copy()
         for s in range(20):
             step, nav_o, last_o, bnd = G.ComputeStep(SOA3D.from_array(pp), D, 10.0, _dev_u32(nv), _dev_u32(ls),
-                                                     relocate=relocate)
+                                                     relocate=relocate, precision=prec)
             rs, rn, rl, rb = O.step(pp, d, 10.0, nv, ls, relocate=relocate, threads=8)
             assert np.array_equal(nav_o.cpu().numpy().astype(np.uint32), rn), (relocate, s)
             assert np.array_equal(last_o.cpu().numpy().astype(np.uint32), rl), (relocate, s)
@@ -57,17 +65,19 @@ def test_step_by_step(geo):
             nv, ls = rn, rl
 
 
-def test_safety(geo):
+@PRECISIONS
+def test_safety(geo, prec):
     g, G, O = geo
     p, _ = geometry.world_points(g, 1 << 15, seed=23)
     nav = O.locate(p, threads=8)
-    got = G.ComputeSafety(SOA3D.from_array(p), _dev_u32(nav)).cpu().numpy()
+    got = G.ComputeSafety(SOA3D.from_array(p), _dev_u32(nav), precision=prec).cpu().numpy()
     ref = O.safety(p, nav, threads=8)
     assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
 
 
+@PRECISIONS
 @pytest.mark.parametrize("regroup", [0, 7, 10])
-def test_propagate_matches_oracle(geo, regroup):
+def test_propagate_matches_oracle(geo, regroup, prec):
     """The fused loop, unsorted (0) and regrouped by volume every 7 / 10 steps: same results."""
     g, G, O = geo
     n, nsteps = 1 << 14, 50
@@ -79,7 +89,7 @@ def test_propagate_matches_oracle(geo, regroup):
     taken = torch.zeros(n, dtype=torch.int32, device="cuda")
     G.set_regroup(regroup)
     try:
-        G.Propagate(P, D, 10.0, nv, ls, nsteps, taken)
+        G.Propagate(P, D, 10.0, nv, ls, nsteps, taken, precision=prec)
     finally:
         G.set_regroup(10)
     rp, rn, rl, rtaken = O.propagate(p, d, 10.0, nav, last, nsteps, threads=8)

@@ -2,6 +2,8 @@
 #include "geom_internal.h"
 #include "vgmi_internal.h"
 #include "shape_host.h"
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -11,10 +13,66 @@ namespace {
 
 size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 
+// FP32 box widened by `pad` and rounded outwards.
+void widen32(const double lo[3], const double hi[3], double pad, float flo[3], float fhi[3])
+{
+  for (int k = 0; k < 3; ++k) {
+    float a = float(lo[k] - pad), b = float(hi[k] + pad);
+    if (double(a) > lo[k] - pad) a = std::nextafter(a, -HUGE_VALF);
+    if (double(b) < hi[k] + pad) b = std::nextafter(b, HUGE_VALF);
+    flo[k] = a, fhi[k] = b;
+  }
+}
+
+// Widening of the FP32 boxes of one logical volume: kNavBox32Pad plus 2e-6 of the largest
+// coordinate a point inside the volume or on its daughters' boxes can have, which exceeds
+// the FP32 rounding of the point, of the box and of the test arithmetic.
+double pad32(HostGeometry const &h, LVRec const &L)
+{
+  double lo[3], hi[3], R = 0.;
+  ShapeExtent(h.shapes[L.shape], lo, hi);
+  for (int k = 0; k < 3; ++k) R = std::max(R, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+  for (int i = 0; i < L.nNodes; ++i)
+    for (int k = 0; k < 3; ++k)
+      R = std::max(R, std::max(std::fabs(h.nodes[L.nodeBase + i].lo[k]), std::fabs(h.nodes[L.nodeBase + i].hi[k])));
+  return kNavBox32Pad + 2e-6 * R;
+}
+
 // Copies every host table into one device allocation and fills the view.
 int upload(vgmi_geom &G)
 {
   HostGeometry const &h = G.host;
+  // navigation records: the placed-volume, BVH-node and daughter tables re-packed
+  std::vector<NavVol> vols(h.pvols.size());
+  for (size_t i = 0; i < h.pvols.size(); ++i) {
+    PVolRec const &P = h.pvols[i];
+    LVRec const &L   = h.lvs[P.lv];
+    NavVol &V        = vols[i];
+    V                = NavVol{};
+    V.lv = P.lv, V.childId = P.childId, V.shape = L.shape, V.ndaughters = L.ndaughters;
+    V.nodeBase = L.nodeBase, V.slotBase = L.dOffset;
+    std::memcpy(V.t, P.t, sizeof(V.t));
+    std::memcpy(V.r, P.r, sizeof(V.r));
+  }
+  std::vector<NavNode> bnodes(h.nchild.size());
+  for (size_t i = 0; i < bnodes.size(); ++i) {
+    bnodes[i] = NavNode{};
+    std::memcpy(bnodes[i].lo, h.nodes[i].lo, sizeof(bnodes[i].lo));
+    std::memcpy(bnodes[i].hi, h.nodes[i].hi, sizeof(bnodes[i].hi));
+    bnodes[i].nchild = h.nchild[i];
+    bnodes[i].offset = h.offset[i];
+  }
+  std::vector<NavSlot> slots(h.daughters.size());
+  for (LVRec const &L : h.lvs)
+    for (int k = 0; k < L.ndaughters; ++k) {
+      const int prim = h.prim[L.primBase + k];
+      NavSlot &S     = slots[L.dOffset + k];
+      S              = NavSlot{};
+      std::memcpy(S.lo, h.dAABB[L.dOffset + prim].lo, sizeof(S.lo));
+      std::memcpy(S.hi, h.dAABB[L.dOffset + prim].hi, sizeof(S.hi));
+      S.pvol  = h.daughters[L.dOffset + prim];
+      S.shape = h.lvs[h.pvols[S.pvol].lv].shape;
+    }
   struct Part {
     const void *src;
     size_t bytes;
@@ -30,7 +88,30 @@ int upload(vgmi_geom &G)
                   {h.nodes.data(), h.nodes.size() * sizeof(AABBRec), 0},
                   {h.prim.data(), h.prim.size() * sizeof(int32_t), 0},
                   {h.navind.data(), h.navind.size() * sizeof(uint32_t), 0},
-                  {h.extraPool.data(), h.extraPool.size() * sizeof(double), 0}};
+                  {h.extraPool.data(), h.extraPool.size() * sizeof(double), 0},
+                  {vols.data(), vols.size() * sizeof(NavVol), 0},
+                  {bnodes.data(), bnodes.size() * sizeof(NavNode), 0},
+                  {slots.data(), slots.size() * sizeof(NavSlot), 0},
+                  {nullptr, 0, 0},
+                  {nullptr, 0, 0}};
+  std::vector<NavNode32> bnodes32(bnodes.size());
+  std::vector<NavSlot32> slots32(slots.size());
+  for (LVRec const &L : h.lvs) {
+    if (L.ndaughters == 0) continue;
+    const double pad = pad32(h, L);
+    for (int i = L.nodeBase; i < L.nodeBase + L.nNodes; ++i) {
+      widen32(bnodes[i].lo, bnodes[i].hi, pad, bnodes32[i].lo, bnodes32[i].hi);
+      bnodes32[i].nchild = bnodes[i].nchild;
+      bnodes32[i].offset = bnodes[i].offset;
+    }
+    for (int i = L.dOffset; i < L.dOffset + L.ndaughters; ++i) {
+      widen32(slots[i].lo, slots[i].hi, pad, slots32[i].lo, slots32[i].hi);
+      slots32[i].pvol  = slots[i].pvol;
+      slots32[i].shape = slots[i].shape;
+    }
+  }
+  parts[14] = Part{bnodes32.data(), bnodes32.size() * sizeof(NavNode32), 0};
+  parts[15] = Part{slots32.data(), slots32.size() * sizeof(NavSlot32), 0};
   size_t total = 0;
   for (auto &p : parts) {
     p.off = total;
@@ -67,6 +148,11 @@ int upload(vgmi_geom &G)
   v.nodes          = static_cast<const AABBRec *>(at(7));
   v.prim           = static_cast<const int32_t *>(at(8));
   v.navind         = static_cast<const uint32_t *>(at(9));
+  v.vols           = static_cast<const NavVol *>(at(11));
+  v.bnodes         = static_cast<const NavNode *>(at(12));
+  v.slots          = static_cast<const NavSlot *>(at(13));
+  v.bnodes32       = static_cast<const NavNode32 *>(at(14));
+  v.slots32        = static_cast<const NavSlot32 *>(at(15));
   v.worldPvol      = h.worldPvol;
   v.maxDepth       = h.maxDepth;
   G.device_block   = blk;

@@ -65,7 +65,7 @@ VGMI_DEV int top(GeomView const &g, uint32_t nav) { return nav > 0 ? int(g.navin
 VGMI_DEV uint32_t pop(GeomView const &g, uint32_t nav) { return nav > 0 ? g.navind[nav] : 0u; }
 VGMI_DEV uint32_t push(GeomView const &g, uint32_t nav, int pvol)
 {
-  return nav > 0 ? g.navind[nav + 3 + g.pvols[pvol].childId] : 1u;
+  return nav > 0 ? g.navind[nav + 3 + g.vols[pvol].childId] : 1u;
 }
 
 // Global matrix of the state's top volume, from the cached table entry.
@@ -100,10 +100,11 @@ VGMI_DEV double safety_in(ShapeRecord const &s, V3 const &p) { return rec_safety
 template <unsigned TM>
 VGMI_DEV double safety_out(ShapeRecord const &s, V3 const &p) { return rec_safety_out<TM>(s, p); }
 
-VGMI_DEV ShapeRecord const &shape_of_pvol(GeomView const &g, int pvol) { return g.shapes[g.lvs[g.pvols[pvol].lv].shape]; }
+VGMI_DEV ShapeRecord const &shape_of_pvol(GeomView const &g, int pvol) { return g.shapes[g.vols[pvol].shape]; }
 
-// AABB::ComputeIntersectionInvDir (VecGeom/base/AABB.h:136-154)
-VGMI_DEV void slab(AABBRec const &b, V3 const &p, V3 const &inv, double &tmin, double &tmax)
+// AABB::ComputeIntersectionInvDir (VecGeom/base/AABB.h:136-154); B is any record with lo[3], hi[3]
+template <class B>
+VGMI_DEV void slab(B const &b, V3 const &p, V3 const &inv, double &tmin, double &tmax)
 {
   double a0 = (b.lo[0] - p.x) * inv.x, a1 = (b.lo[1] - p.y) * inv.y, a2 = (b.lo[2] - p.z) * inv.z;
   double b0 = (b.hi[0] - p.x) * inv.x, b1 = (b.hi[1] - p.y) * inv.y, b2 = (b.hi[2] - p.z) * inv.z;
@@ -126,18 +127,65 @@ VGMI_DEV void slab(AABBRec const &b, V3 const &p, V3 const &inv, double &tmin, d
   tmax = vmin(b0, vmin(b1, b2)) * (1. + 2 * DBL_EPSILON);
 }
 
-VGMI_DEV bool box_contains(AABBRec const &b, V3 const &p)
+template <class B>
+VGMI_DEV bool box_contains(B const &b, V3 const &p)
 {
   return p.x >= b.lo[0] && p.x <= b.hi[0] && p.y >= b.lo[1] && p.y <= b.hi[1] && p.z >= b.lo[2] && p.z <= b.hi[2];
 }
 
-VGMI_DEV double box_safety(AABBRec const &b, V3 const &p)
+template <class B>
+VGMI_DEV double box_safety(B const &b, V3 const &p)
 {
   double c0 = 0.5 * (b.hi[0] + b.lo[0]), c1 = 0.5 * (b.hi[1] + b.lo[1]), c2 = 0.5 * (b.hi[2] + b.lo[2]);
   double s0 = vabs(p.x - c0) - 0.5 * (b.hi[0] - b.lo[0]);
   double s1 = vabs(p.y - c1) - 0.5 * (b.hi[1] - b.lo[1]);
   double s2 = vabs(p.z - c2) - 0.5 * (b.hi[2] - b.lo[2]);
   return vmax(s0, vmax(s1, s2));
+}
+
+// ---- mixed precision: FP32 box filters over the widened FP32 tables.
+// TM bit 31 selects the mixed-precision traversal; the shape-type bits are unchanged.
+constexpr unsigned kMixedFlag = 1u << 31;
+template <unsigned TM>
+constexpr bool kMixed = (TM & kMixedFlag) != 0;
+
+using F3 = Vec3<float>;
+// Relative and absolute slack of the FP32 slab comparisons (the FP32 arithmetic error is
+// ~1e-7 relative; the spatial rounding is absorbed by the box widening).
+constexpr float kSlabRel = 1e-5f, kSlabAbs = 1e-5f;
+
+VGMI_DEV F3 to_f3(V3 const &v) { return F3(float(v.x), float(v.y), float(v.z)); }
+VGMI_DEV float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// FP32 slab test against a widened box: true unless the FP64 test of the unwidened box
+// certainly fails (ray misses the box, box behind the point, or entry beyond `lim`).
+template <class B>
+VGMI_DEV bool slab32_pass(B const &b, F3 const &p, F3 const &inv, float lim, float &tmin_out)
+{
+  float a0 = (b.lo[0] - p.x) * inv.x, a1 = (b.lo[1] - p.y) * inv.y, a2 = (b.lo[2] - p.z) * inv.z;
+  float b0 = (b.hi[0] - p.x) * inv.x, b1 = (b.hi[1] - p.y) * inv.y, b2 = (b.hi[2] - p.z) * inv.z;
+  const float tmin = vmax(vmin(a0, b0), vmax(vmin(a1, b1), vmin(a2, b2)));
+  const float tmax = vmin(vmax(a0, b0), vmin(vmax(a1, b1), vmax(a2, b2)));
+  tmin_out = tmin;
+  const float sl = kSlabRel * (vabs(tmin) + vabs(tmax)) + kSlabAbs;
+  return tmin <= tmax + sl && tmax >= -sl && tmin < lim + (kSlabRel * vabs(lim) + kSlabAbs);
+}
+
+// Lower bound (in FP64 units) of the FP64 box_safety of the unwidened box.
+template <class B>
+VGMI_DEV double box_safety32(B const &b, F3 const &p)
+{
+  const float c0 = 0.5f * (b.hi[0] + b.lo[0]), c1 = 0.5f * (b.hi[1] + b.lo[1]), c2 = 0.5f * (b.hi[2] + b.lo[2]);
+  const float s0 = vabs(p.x - c0) - 0.5f * (b.hi[0] - b.lo[0]);
+  const float s1 = vabs(p.y - c1) - 0.5f * (b.hi[1] - b.lo[1]);
+  const float s2 = vabs(p.z - c2) - 0.5f * (b.hi[2] - b.lo[2]);
+  return double(vmax(s0, vmax(s1, s2)));
+}
+
+template <class B>
+VGMI_DEV bool box_contains32(B const &b, F3 const &p)
+{
+  return p.x >= b.lo[0] && p.x <= b.hi[0] && p.y >= b.lo[1] && p.y <= b.hi[1] && p.z >= b.lo[2] && p.z <= b.hi[2];
 }
 
 // Lower bound of the distance from a local point to a shape (zero or negative inside): the
@@ -175,29 +223,38 @@ VGMI_DEV double lower_bound_in(ShapeRecord const &s, V3 const &p)
 // box distance of each subtree or daughter box the walk does not descend into, and the box
 // distance (or shape lower bound, when computed) of each daughter it tests.
 template <unsigned TM>
-VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V3 const &ld, double &step, int last,
+VGMI_DEV void check_daughters_mixed(GeomView const &g, NavVol const &M, V3 const &lp, V3 const &ld, double &step,
+                                    int last, int &hit, double cap, double &bound, Stack stk);
+
+template <unsigned TM>
+VGMI_DEV void check_daughters(GeomView const &g, NavVol const &M, V3 const &lp, V3 const &ld, double &step, int last,
                               int &hit, double cap, double &bound, Stack stk)
 {
+  if constexpr (kMixed<TM>) {
+    check_daughters_mixed<TM>(g, M, lp, ld, step, last, hit, cap, bound, stk);
+    return;
+  }
   const V3 inv(1.0 / NonZero(ld.x), 1.0 / NonZero(ld.y), 1.0 / NonZero(ld.z));
   const double capm = cap + 1e-6;
+  const NavNode *nodes = g.bnodes + M.nodeBase;
+  const NavSlot *slots = g.slots + M.slotBase;
   double bnd        = kInfLength;
   int sp            = 1;
   stk.at(0)         = 0;
   do {
     const uint32_t id = stk.at(--sp);
-    const int nc      = g.nchild[L.nodeBase + id];
+    NavNode const &N  = nodes[id];
+    const int nc      = N.nchild;
     if (nc >= 0) {
-      const int off = g.offset[L.nodeBase + id];
       for (int i = 0; i < nc; ++i) {
-        const int prim     = g.prim[L.primBase + off + i];
-        AABBRec const &box = g.dAABB[L.dOffset + prim];
+        NavSlot const &box = slots[N.offset + i];
         double tmin, tmax;
         slab(box, lp, inv, tmin, tmax);
         double lb = box_safety(box, lp);
         if (tmin <= tmax && tmax >= 0.0 && tmin < step && !(tmin > capm)) {
-          const int pv         = g.daughters[L.dOffset + prim];
-          PVolRec const &P     = g.pvols[pv];
-          ShapeRecord const &S = shape_of_pvol(g, pv);
+          const int pv         = box.pvol;
+          NavVol const &P      = g.vols[pv];
+          ShapeRecord const &S = g.shapes[box.shape];
           const V3 loc         = to_local(P.t, P.r, lp);
           const double lbs     = lower_bound_in(S, loc);
           lb                   = vmax(lb, lbs);
@@ -210,8 +267,8 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
       }
     } else {
       const uint32_t l = 2 * id + 1, r = 2 * id + 2;
-      AABBRec const &bl = g.nodes[L.nodeBase + l];
-      AABBRec const &br = g.nodes[L.nodeBase + r];
+      NavNode const &bl = nodes[l];
+      NavNode const &br = nodes[r];
       double tminL, tmaxL, tminR, tmaxR;
       slab(bl, lp, inv, tminL, tmaxL);
       slab(br, lp, inv, tminR, tmaxR);
@@ -231,25 +288,103 @@ VGMI_DEV void check_daughters(GeomView const &g, LVRec const &L, V3 const &lp, V
   bound = bnd;
 }
 
-// BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint)
+// Mixed-precision CheckDaughterIntersections: the same walk with FP32 box tests as filters.
+// A box the FP64 walk enters always passes its FP32 test (widened box, slack), so the walk
+// visits a superset of the FP64 walk's candidates, in the FP64 order (sibling order is taken
+// from FP64 entry distances whenever both siblings pass), and evaluates every candidate in
+// FP64. An extra candidate cannot change the step: its FP64 box test fails, so its distance
+// is not below the current step. Results are those of the FP64 walk, bit for bit.
 template <unsigned TM>
-VGMI_DEV bool level_locate(GeomView const &g, LVRec const &L, int excl, V3 const &lp, int &pvol, V3 &dlp, Stack stk)
+VGMI_DEV void check_daughters_mixed(GeomView const &g, NavVol const &M, V3 const &lp, V3 const &ld, double &step,
+                                    int last, int &hit, double cap, double &bound, Stack stk)
 {
+  const F3 pf = to_f3(lp);
+  const F3 df = to_f3(ld);
+  const F3 invf(rcp32(NonZero(df.x)), rcp32(NonZero(df.y)), rcp32(NonZero(df.z)));
+  const double capm = cap + 1e-6;
+  const NavNode32 *nodes = g.bnodes32 + M.nodeBase;
+  const NavSlot32 *slots = g.slots32 + M.slotBase;
+  double bnd = kInfLength;
+  int sp     = 1;
+  stk.at(0)  = 0;
+  do {
+    const uint32_t id  = stk.at(--sp);
+    NavNode32 const &N = nodes[id];
+    const int nc       = N.nchild;
+    const float lim    = float(vmin(step, capm));
+    if (nc >= 0) {
+      for (int i = 0; i < nc; ++i) {
+        NavSlot32 const &box = slots[N.offset + i];
+        float t;
+        const bool pass = slab32_pass(box, pf, invf, float(vmin(step, capm)), t);
+        double lb       = box_safety32(box, pf);
+        if (pass) {
+          const int pv         = box.pvol;
+          NavVol const &P      = g.vols[pv];
+          ShapeRecord const &S = g.shapes[box.shape];
+          const V3 loc         = to_local(P.t, P.r, lp);
+          const double lbs     = lower_bound_in(S, loc);
+          lb                   = vmax(lb, lbs);
+          if (!(lbs > capm)) {
+            const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
+            if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
+          }
+        }
+        bnd = vmin(bnd, lb);
+      }
+    } else {
+      const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      float tl32, tr32;
+      const bool tl = slab32_pass(nodes[l], pf, invf, lim, tl32);
+      const bool tr = slab32_pass(nodes[r], pf, invf, lim, tr32);
+      if (!tl) bnd = vmin(bnd, box_safety32(nodes[l], pf));
+      if (!tr) bnd = vmin(bnd, box_safety32(nodes[r], pf));
+      bool rfirst = false;
+      if (tl && tr) {
+        // sibling order from the FP64 entry distances, as in the FP64 walk
+        const V3 inv(1.0 / NonZero(ld.x), 1.0 / NonZero(ld.y), 1.0 / NonZero(ld.z));
+        double tminL, tmaxL, tminR, tmaxR;
+        slab(g.bnodes[M.nodeBase + l], lp, inv, tminL, tmaxL);
+        slab(g.bnodes[M.nodeBase + r], lp, inv, tminR, tmaxR);
+        rfirst = tminR < tminL;
+      }
+      if (rfirst) {
+        stk.at(sp++) = r;
+        stk.at(sp++) = l;
+      } else {
+        if (tl) stk.at(sp++) = l;
+        if (tr) stk.at(sp++) = r;
+      }
+    }
+  } while (sp > 0);
+  bound = bnd;
+}
+
+// BVH::LevelLocate(exclvol, localpoint, pvol, daughterlocalpoint)
+// Mixed precision: FP32 containment filters over the widened boxes (a superset of the FP64
+// box hits, visited in the same fixed order), FP64 shape containment.
+template <unsigned TM>
+VGMI_DEV bool level_locate_mixed(GeomView const &g, NavVol const &M, int excl, V3 const &lp, int &pvol, V3 &dlp,
+                                 Stack stk)
+{
+  const F3 pf              = to_f3(lp);
+  const NavNode32 *nodes   = g.bnodes32 + M.nodeBase;
+  const NavSlot32 *slots   = g.slots32 + M.slotBase;
   int sp    = 1;
   stk.at(0) = 0;
   do {
-    const uint32_t id = stk.at(--sp);
-    const int nc      = g.nchild[L.nodeBase + id];
+    const uint32_t id  = stk.at(--sp);
+    NavNode32 const &N = nodes[id];
+    const int nc       = N.nchild;
     if (nc >= 0) {
-      const int off = g.offset[L.nodeBase + id];
       for (int i = 0; i < nc; ++i) {
-        const int prim = g.prim[L.primBase + off + i];
-        if (box_contains(g.dAABB[L.dOffset + prim], lp)) {
-          const int pv = g.daughters[L.dOffset + prim];
+        NavSlot32 const &box = slots[N.offset + i];
+        if (box_contains32(box, pf)) {
+          const int pv = box.pvol;
           if (pv == excl) continue;
-          PVolRec const &P = g.pvols[pv];
-          const V3 loc     = to_local(P.t, P.r, lp);
-          if (contains<TM>(shape_of_pvol(g, pv), loc)) {
+          NavVol const &P = g.vols[pv];
+          const V3 loc    = to_local(P.t, P.r, lp);
+          if (contains<TM>(g.shapes[box.shape], loc)) {
             pvol = pv;
             dlp  = loc;
             return true;
@@ -258,36 +393,118 @@ VGMI_DEV bool level_locate(GeomView const &g, LVRec const &L, int excl, V3 const
       }
     } else {
       const uint32_t l = 2 * id + 1, r = 2 * id + 2;
-      if (box_contains(g.nodes[L.nodeBase + l], lp)) stk.at(sp++) = l;
-      if (box_contains(g.nodes[L.nodeBase + r], lp)) stk.at(sp++) = r;
+      if (box_contains32(nodes[l], pf)) stk.at(sp++) = l;
+      if (box_contains32(nodes[r], pf)) stk.at(sp++) = r;
+    }
+  } while (sp > 0);
+  return false;
+}
+
+template <unsigned TM>
+VGMI_DEV bool level_locate(GeomView const &g, NavVol const &M, int excl, V3 const &lp, int &pvol, V3 &dlp, Stack stk)
+{
+  if constexpr (kMixed<TM>) return level_locate_mixed<TM>(g, M, excl, lp, pvol, dlp, stk);
+  const NavNode *nodes = g.bnodes + M.nodeBase;
+  const NavSlot *slots = g.slots + M.slotBase;
+  int sp    = 1;
+  stk.at(0) = 0;
+  do {
+    const uint32_t id = stk.at(--sp);
+    NavNode const &N  = nodes[id];
+    const int nc      = N.nchild;
+    if (nc >= 0) {
+      for (int i = 0; i < nc; ++i) {
+        NavSlot const &box = slots[N.offset + i];
+        if (box_contains(box, lp)) {
+          const int pv = box.pvol;
+          if (pv == excl) continue;
+          NavVol const &P = g.vols[pv];
+          const V3 loc    = to_local(P.t, P.r, lp);
+          if (contains<TM>(g.shapes[box.shape], loc)) {
+            pvol = pv;
+            dlp  = loc;
+            return true;
+          }
+        }
+      }
+    } else {
+      const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      if (box_contains(nodes[l], lp)) stk.at(sp++) = l;
+      if (box_contains(nodes[r], lp)) stk.at(sp++) = r;
     }
   } while (sp > 0);
   return false;
 }
 
 // BVH::ComputeSafety
+// Mixed precision: a box whose FP32 safety lower bound is not below the current safety is
+// skipped (the FP64 walk skips it too); every other box test, the sibling order and the
+// shape safeties are FP64, so the result is the FP64 walk's.
 template <unsigned TM>
-VGMI_DEV double safety_to_daughters(GeomView const &g, LVRec const &L, V3 const &lp, double safety, Stack stk)
+VGMI_DEV double safety_to_daughters_mixed(GeomView const &g, NavVol const &M, V3 const &lp, double safety, Stack stk)
 {
+  const F3 pf              = to_f3(lp);
+  const NavNode *nodes     = g.bnodes + M.nodeBase;
+  const NavSlot *slots     = g.slots + M.slotBase;
+  const NavNode32 *nodes32 = g.bnodes32 + M.nodeBase;
+  const NavSlot32 *slots32 = g.slots32 + M.slotBase;
   int sp    = 1;
   stk.at(0) = 0;
   do {
     const uint32_t id = stk.at(--sp);
-    const int nc      = g.nchild[L.nodeBase + id];
+    const int nc      = nodes32[id].nchild;
     if (nc >= 0) {
-      const int off = g.offset[L.nodeBase + id];
+      const int off = nodes32[id].offset;
       for (int i = 0; i < nc; ++i) {
-        const int prim = g.prim[L.primBase + off + i];
-        if (box_safety(g.dAABB[L.dOffset + prim], lp) < safety) {
-          const int pv     = g.daughters[L.dOffset + prim];
-          PVolRec const &P = g.pvols[pv];
-          const double s   = safety_in<TM>(shape_of_pvol(g, pv), to_local(P.t, P.r, lp));
+        if (!(box_safety32(slots32[off + i], pf) < safety)) continue;
+        NavSlot const &box = slots[off + i];
+        if (box_safety(box, lp) < safety) {
+          NavVol const &P = g.vols[box.pvol];
+          const double s  = safety_in<TM>(g.shapes[box.shape], to_local(P.t, P.r, lp));
           if (s < safety) safety = s;
         }
       }
     } else {
       const uint32_t l = 2 * id + 1, r = 2 * id + 2;
-      const double sl = box_safety(g.nodes[L.nodeBase + l], lp), sr = box_safety(g.nodes[L.nodeBase + r], lp);
+      const bool cl = box_safety32(nodes32[l], pf) < safety, cr = box_safety32(nodes32[r], pf) < safety;
+      const double sl = cl ? box_safety(nodes[l], lp) : kInfLength, sr = cr ? box_safety(nodes[r], lp) : kInfLength;
+      const bool tl = sl < safety, tr = sr < safety;
+      if (sr < sl) {
+        if (tr) stk.at(sp++) = r;
+        if (tl) stk.at(sp++) = l;
+      } else {
+        if (tl) stk.at(sp++) = l;
+        if (tr) stk.at(sp++) = r;
+      }
+    }
+  } while (sp > 0);
+  return safety;
+}
+
+template <unsigned TM>
+VGMI_DEV double safety_to_daughters(GeomView const &g, NavVol const &M, V3 const &lp, double safety, Stack stk)
+{
+  if constexpr (kMixed<TM>) return safety_to_daughters_mixed<TM>(g, M, lp, safety, stk);
+  const NavNode *nodes = g.bnodes + M.nodeBase;
+  const NavSlot *slots = g.slots + M.slotBase;
+  int sp    = 1;
+  stk.at(0) = 0;
+  do {
+    const uint32_t id = stk.at(--sp);
+    NavNode const &N  = nodes[id];
+    const int nc      = N.nchild;
+    if (nc >= 0) {
+      for (int i = 0; i < nc; ++i) {
+        NavSlot const &box = slots[N.offset + i];
+        if (box_safety(box, lp) < safety) {
+          NavVol const &P = g.vols[box.pvol];
+          const double s  = safety_in<TM>(g.shapes[box.shape], to_local(P.t, P.r, lp));
+          if (s < safety) safety = s;
+        }
+      }
+    } else {
+      const uint32_t l = 2 * id + 1, r = 2 * id + 2;
+      const double sl = box_safety(nodes[l], lp), sr = box_safety(nodes[r], lp);
       const bool tl = sl < safety, tr = sr < safety;
       if (sr < sl) {
         if (tr) stk.at(sp++) = r;
@@ -306,15 +523,15 @@ template <unsigned TM>
 VGMI_DEV uint32_t locate_down(GeomView const &g, int pvol, int excl, V3 p, uint32_t path, Stack stk)
 {
   path              = push(g, path, pvol);
-  LVRec const *L    = &g.lvs[g.pvols[pvol].lv];
+  const NavVol *M   = &g.vols[pvol];
   bool deeper       = true;
-  while (L->ndaughters > 0 && deeper) {
+  while (M->ndaughters > 0 && deeper) {
     int next = -1;
     V3 dlp;
-    deeper = level_locate<TM>(g, *L, excl, p, next, dlp, stk);
+    deeper = level_locate<TM>(g, *M, excl, p, next, dlp, stk);
     if (deeper) {
       path = push(g, path, next);
-      L    = &g.lvs[g.pvols[next].lv];
+      M    = &g.vols[next];
       p    = dlp;
     }
   }
@@ -337,11 +554,11 @@ VGMI_DEV uint32_t relocate_force_different(GeomView const &g, V3 const &lp, uint
   if (mother < 0) return path;
   V3 tmp = lp;
   while (mother >= 0) {
-    if (mother == entry || !contains<TM>(shape_of_pvol(g, mother), tmp)) {
-      path             = pop(g, path);
-      PVolRec const &P = g.pvols[mother];
-      tmp              = to_master(P.t, P.r, tmp);
-      mother           = top(g, path);
+    NavVol const &P = g.vols[mother];
+    if (mother == entry || !contains<TM>(g.shapes[P.shape], tmp)) {
+      path   = pop(g, path);
+      tmp    = to_master(P.t, P.r, tmp);
+      mother = top(g, path);
     } else {
       break;
     }
@@ -377,14 +594,14 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   const V3 lp       = to_local(m.t, m.r, gp);
   const V3 ld       = rotate(m.r, gd);
   const int pvol    = top(g, in.nav);
-  LVRec const &L    = g.lvs[g.pvols[pvol].lv];
-  double step       = dist_out<TM>(g.shapes[L.shape], lp, ld);
+  NavVol const &M   = g.vols[pvol];
+  double step       = dist_out<TM>(g.shapes[M.shape], lp, ld);
   if (step < 0.) step = 0.;
   int hit = -1;
-  if (L.ndaughters == 0) {
+  if (M.ndaughters == 0) {
     budget = kInfLength;
   } else if (!(step < kInfLength && budget > limit + kBudgetMargin)) {
-    check_daughters<TM>(g, L, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, budget,
+    check_daughters<TM>(g, M, lp, ld, step, top(g, in.last), hit, step < kInfLength ? limit : kInfLength, budget,
                         stk);
   }
   out = in;
@@ -409,9 +626,9 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   if (top(g, out.nav) == pvol) {
     out.nav = relocate_force_different<TM>(g, after, out.nav, stk);
   } else {
-    PVolRec const &N = g.pvols[hit];
-    out.nav          = pop(g, out.nav);
-    out.nav          = locate_down<TM>(g, hit, -1, to_local(N.t, N.r, after), out.nav, stk);
+    NavVol const &N = g.vols[hit];
+    out.nav         = pop(g, out.nav);
+    out.nav         = locate_down<TM>(g, hit, -1, to_local(N.t, N.r, after), out.nav, stk);
   }
   return step;
 }
@@ -422,10 +639,10 @@ VGMI_DEV double compute_safety(GeomView const &g, V3 const &gp, uint32_t nav, St
 {
   Mat m;
   top_matrix(g, nav, m);
-  const V3 lp    = to_local(m.t, m.r, gp);
-  LVRec const &L = g.lvs[g.pvols[top(g, nav)].lv];
-  double s       = safety_out<TM>(g.shapes[L.shape], lp);
-  if (s > 0.0 && L.ndaughters > 0) s = safety_to_daughters<TM>(g, L, lp, s, stk);
+  const V3 lp     = to_local(m.t, m.r, gp);
+  NavVol const &M = g.vols[top(g, nav)];
+  double s        = safety_out<TM>(g.shapes[M.shape], lp);
+  if (s > 0.0 && M.ndaughters > 0) s = safety_to_daughters<TM>(g, M, lp, s, stk);
   return s;
 }
 

@@ -38,6 +38,44 @@ struct AABBRec {
   double lo[3], hi[3];
 };
 
+// ---- Navigation records (device layout derived from the tables above at upload).
+// Each record holds everything one traversal step needs from it, so a step issues its
+// loads as a few independent 16-byte reads instead of a chain of dependent ones.
+
+// Placed volume with its logical volume's fields folded in (128 B).
+struct alignas(16) NavVol {
+  int32_t lv, childId, shape, ndaughters; // shape: the logical volume's shape record
+  int32_t nodeBase, slotBase, pad0, pad1; // BVH nodes; daughter slots (== LVRec::dOffset)
+  double t[3], r[9];                      // placement (as PVolRec)
+};
+
+// BVH node (64 B): AABB, child count (-1: internal) and primitive offset (LVRec::primBase-relative).
+struct alignas(16) NavNode {
+  double lo[3], hi[3];
+  int32_t nchild, offset, pad0, pad1;
+};
+
+// Daughter slot in BVH primitive order (64 B): slot slotBase + k holds the daughter of
+// primitive prim[primBase + k], i.e. its ABBox, placed volume and shape record.
+struct alignas(16) NavSlot {
+  double lo[3], hi[3];
+  int32_t pvol, shape, pad0, pad1;
+};
+
+// FP32 copies for the mixed-precision traversal (32 B each): the boxes are widened by
+// kNavBox32Pad plus 2e-6 of the volume's coordinate range and rounded outwards, so every box
+// test the FP64 traversal passes is passed in FP32 as well (the FP32 tests only filter;
+// FP64 decides).
+constexpr double kNavBox32Pad = 1e-3; // mm
+struct alignas(16) NavNode32 {
+  float lo[3], hi[3];
+  int32_t nchild, offset;
+};
+struct alignas(16) NavSlot32 {
+  float lo[3], hi[3];
+  int32_t pvol, shape;
+};
+
 // Kernel-argument view of the device tables.
 struct GeomView {
   const ShapeRecord *shapes;
@@ -50,6 +88,11 @@ struct GeomView {
   const AABBRec *nodes;      // per BVH node
   const int32_t *prim;       // per BVH primitive slot
   const uint32_t *navind;    // NavIndex table (source/NavIndexTable.cpp:9-99 layout)
+  const NavVol *vols;        // per placed volume
+  const NavNode *bnodes;     // per BVH node (same index as nchild/offset/nodes)
+  const NavSlot *slots;      // per daughter entry, in primitive order
+  const NavNode32 *bnodes32; // FP32 copies of bnodes / slots
+  const NavSlot32 *slots32;
   int32_t worldPvol;
   int32_t maxDepth;
 };

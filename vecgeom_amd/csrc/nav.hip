@@ -11,11 +11,11 @@ namespace nav {
 // solids gets kernels without the polycone/polyhedron/sphere/torus solvers, whose register
 // demand would otherwise set the occupancy of every navigation kernel.
 constexpr unsigned kSimpleTypes = (1u << kBox) | (1u << kTube) | (1u << kCone) | (1u << kTrapezoid) | (1u << kOrb);
-constexpr unsigned kAllTypes    = ~0u;
+constexpr unsigned kAllTypes    = ~kMixedFlag;
 // Waves per SIMD the transport-loop kernels are compiled for: the simple-solid variants are
 // held to 128 VGPRs (4 waves, a few bytes of spill); the all-solid variants keep every register.
 template <unsigned TM>
-constexpr unsigned kLoopWaves = TM == kAllTypes ? 1u : 4u;
+constexpr unsigned kLoopWaves = (TM & ~kMixedFlag) == kAllTypes ? 1u : 4u;
 
 #define VGMI_NAV_STACK                                                                                               \
   __shared__ uint32_t lds_stack[kStackDepth * kNavBlock];                                                           \
@@ -140,7 +140,7 @@ struct SortKeys {
   {
     const uint32_t s = nav[i];
     if (s == 0 || (tsteps && taken[i] >= tsteps[i])) return 0u;
-    return uint32_t(g.pvols[top(g, s)].lv) + 1u;
+    return uint32_t(g.vols[top(g, s)].lv) + 1u;
   }
 };
 
@@ -249,15 +249,24 @@ static unsigned nav_grid(size_t n) { return unsigned((n + kNavBlock - 1) / kNavB
 // Below this many tracks a transport loop is not regrouped (the sort would not pay).
 constexpr size_t kRegroupMinTracks = 4096;
 
-// Launches KERNEL<kSimpleTypes> when the geometry has only simple solids, else KERNEL<kAllTypes>.
-#define VGMI_NAV_LAUNCH(KERNEL, GEOM, N, STREAM, ...)                                                                \
+// Launches KERNEL<types | precision>: kSimpleTypes when the geometry has only simple solids,
+// else kAllTypes; kMixedFlag for VGMI_MIXED.
+#define VGMI_NAV_LAUNCH_TM(KERNEL, TMV, N, STREAM, ...)                                                              \
+  hipLaunchKernelGGL(KERNEL<TMV>, dim3(nav_grid(N)), dim3(kNavBlock), 0, (hipStream_t)(STREAM), __VA_ARGS__)
+#define VGMI_NAV_LAUNCH(KERNEL, GEOM, PREC, N, STREAM, ...)                                                          \
   do {                                                                                                               \
-    if (((GEOM)->typeMask & ~kSimpleTypes) == 0u)                                                                    \
-      hipLaunchKernelGGL(KERNEL<kSimpleTypes>, dim3(nav_grid(N)), dim3(kNavBlock), 0, (hipStream_t)(STREAM),         \
-                         __VA_ARGS__);                                                                               \
-    else                                                                                                             \
-      hipLaunchKernelGGL(KERNEL<kAllTypes>, dim3(nav_grid(N)), dim3(kNavBlock), 0, (hipStream_t)(STREAM),            \
-                         __VA_ARGS__);                                                                               \
+    const bool simple_ = ((GEOM)->typeMask & ~kSimpleTypes) == 0u;                                                   \
+    if ((PREC) == VGMI_MIXED) {                                                                                      \
+      if (simple_)                                                                                                   \
+        VGMI_NAV_LAUNCH_TM(KERNEL, kSimpleTypes | kMixedFlag, N, STREAM, __VA_ARGS__);                              \
+      else                                                                                                           \
+        VGMI_NAV_LAUNCH_TM(KERNEL, kAllTypes | kMixedFlag, N, STREAM, __VA_ARGS__);                                 \
+    } else {                                                                                                         \
+      if (simple_)                                                                                                   \
+        VGMI_NAV_LAUNCH_TM(KERNEL, kSimpleTypes, N, STREAM, __VA_ARGS__);                                            \
+      else                                                                                                           \
+        VGMI_NAV_LAUNCH_TM(KERNEL, kAllTypes, N, STREAM, __VA_ARGS__);                                               \
+    }                                                                                                                \
   } while (0)
 
 static vgmi_status check_geom(const vgmi_geom *g, int precision, const char *who)
@@ -292,7 +301,7 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
   if (!work || chunk <= 0 || n < kRegroupMinTracks) {
     PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, t.nsteps, t.track_steps,
                nullptr, nullptr, 0};
-    VGMI_NAV_LAUNCH(k_propagate, geom, n, st, geom->view, a);
+    VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, n, st, geom->view, a);
     return launched("transport loop");
   }
   // Chunks of `chunk` steps; before each, the tracks still running are regrouped by the
@@ -311,17 +320,17 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
     hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, key, n, cursor, perm, nbins);
     PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, std::min(chunk, t.nsteps - done),
                t.track_steps, perm, cursor, 1};
-    VGMI_NAV_LAUNCH(k_propagate, geom, n, st, geom->view, a);
+    VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, n, st, geom->view, a);
   }
   return launched("transport loop");
 }
 
-vgmi_status launch_locate_init(const vgmi_geom *geom, const double *px, const double *py, const double *pz,
-                               double *opx, double *opy, double *opz, uint32_t *nav, uint32_t *last,
-                               int32_t *taken, size_t n, hipStream_t st)
+vgmi_status launch_locate_init(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                               const double *pz, double *opx, double *opy, double *opz, uint32_t *nav,
+                               uint32_t *last, int32_t *taken, size_t n, hipStream_t st)
 {
   if (n == 0) return VGMI_OK;
-  VGMI_NAV_LAUNCH(k_locate_init, geom, n, st, geom->view, px, py, pz, opx, opy, opz, nav, last, taken, n);
+  VGMI_NAV_LAUNCH(k_locate_init, geom, precision, n, st, geom->view, px, py, pz, opx, opy, opz, nav, last, taken, n);
   return launched("serving locate");
 }
 
@@ -335,7 +344,7 @@ vgmi_status vgmi_nav_locate(const vgmi_geom *geom, int precision, const double *
   if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_locate")) return s;
   if (n == 0) return VGMI_OK;
   if (!px || !py || !pz || !nav_out) return set_error(VGMI_ERR_INVALID, "vgmi_nav_locate: null pointer");
-  VGMI_NAV_LAUNCH(k_locate, geom, n, stream, geom->view, px, py, pz, nav_out, n);
+  VGMI_NAV_LAUNCH(k_locate, geom, precision, n, stream, geom->view, px, py, pz, nav_out, n);
   return launched("vgmi_nav_locate");
 }
 
@@ -350,7 +359,7 @@ vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const double *px
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_step: null pointer");
   StepArgs a{px, py, pz, dx, dy, dz, step_limit, nav_in, last_in, nav_out, last_out, boundary_out, step_out, n,
              relocate};
-  VGMI_NAV_LAUNCH(k_step, geom, n, stream, geom->view, a);
+  VGMI_NAV_LAUNCH(k_step, geom, precision, n, stream, geom->view, a);
   return launched("vgmi_nav_step");
 }
 
@@ -360,7 +369,7 @@ vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *
   if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_safety")) return s;
   if (n == 0) return VGMI_OK;
   if (!px || !py || !pz || !nav_in || !safety_out) return set_error(VGMI_ERR_INVALID, "vgmi_nav_safety: null pointer");
-  VGMI_NAV_LAUNCH(k_safety, geom, n, stream, geom->view, px, py, pz, nav_in, safety_out, n);
+  VGMI_NAV_LAUNCH(k_safety, geom, precision, n, stream, geom->view, px, py, pz, nav_in, safety_out, n);
   return launched("vgmi_nav_safety");
 }
 
@@ -374,7 +383,7 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_propagate: null pointer");
   if (n > 0xffffffffull) return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_propagate: more than 2^32 tracks");
   hipStream_t st = (hipStream_t)stream;
-  TransportArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, nullptr, n, nsteps};
+  TransportArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, nullptr, n, nsteps, precision};
   uint32_t *work = nullptr;
   if (const size_t words = transport_work_words(geom, n)) {
     if (hipMallocAsync(reinterpret_cast<void **>(&work), words * sizeof(uint32_t), st) != hipSuccess)

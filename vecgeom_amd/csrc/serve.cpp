@@ -66,6 +66,7 @@ struct Lane {
 
 struct vgmi_server {
   const vgmi_geom *geom = nullptr;
+  int precision         = VGMI_FP64;
   double limit          = 0;
   int64_t queueCap      = 0;
   std::vector<Lane> lanes;
@@ -164,9 +165,10 @@ void dispatch_loop(vgmi_server *S)
     auto U         = [&](size_t off) { return reinterpret_cast<uint32_t *>(d + off); };
     auto I         = [&](size_t off) { return reinterpret_cast<int32_t *>(d + off); };
     TransportArgs a{D(L->oOpx), D(L->oOpy), D(L->oOpz), D(L->oDx),    D(L->oDy),       D(L->oDz), S->limit,
-                    U(L->oNav), U(L->oLast), I(L->oTaken), I(L->oSteps), n,               L->maxSteps};
+                    U(L->oNav), U(L->oLast), I(L->oTaken), I(L->oSteps), n,               L->maxSteps,
+                    S->precision};
     hipError_t e = hipMemcpyAsync(d, h, L->inEnd, hipMemcpyHostToDevice, L->stream);
-    int st       = e == hipSuccess ? launch_locate_init(S->geom, D(L->oPx), D(L->oPy), D(L->oPz), a.px, a.py, a.pz,
+    int st       = e == hipSuccess ? launch_locate_init(S->geom, S->precision, D(L->oPx), D(L->oPy), D(L->oPz), a.px, a.py, a.pz,
                                                         a.nav, a.last, a.taken, n, L->stream)
                                    : VGMI_ERR_DEVICE;
     if (st == VGMI_OK) st = launch_transport(S->geom, a, L->dwork, L->stream);
@@ -251,7 +253,8 @@ vgmi_status vgmi_server_create(const vgmi_geom *geom, int precision, double step
   if (!out) return set_error(VGMI_ERR_INVALID, "vgmi_server_create: null output handle");
   *out = nullptr;
   if (!geom || !geom->device_block) return set_error(VGMI_ERR_INVALID, "vgmi_server_create: geometry not on a device");
-  if (precision != VGMI_FP64) return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_server_create: only VGMI_FP64 is implemented");
+  if (precision != VGMI_FP64 && precision != VGMI_MIXED)
+    return set_error(VGMI_ERR_INVALID, "vgmi_server_create: unknown precision %d", precision);
   if (lane_tracks < 1 || lanes < 1 || lanes > 16 || queue_tracks < lane_tracks)
     return set_error(VGMI_ERR_INVALID, "vgmi_server_create: need lane_tracks >= 1, 1..16 lanes, queue >= lane");
   size_t freeB = 0, totalB = 0;
@@ -262,6 +265,7 @@ vgmi_status vgmi_server_create(const vgmi_geom *geom, int precision, double step
                      freeB);
   auto *S      = new vgmi_server();
   S->geom      = geom;
+  S->precision = precision;
   S->limit     = step_limit;
   S->queueCap  = queue_tracks;
   S->device    = geom->device;

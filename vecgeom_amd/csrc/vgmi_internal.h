@@ -22,15 +22,16 @@ struct TransportArgs {
   const int32_t *track_steps;  // per-track step counts, or null for a uniform count
   size_t n;
   int nsteps;                  // the uniform step count, or the largest of track_steps
+  int precision;               // VGMI_FP64 or VGMI_MIXED
 };
 // Words of device workspace launch_transport needs to regroup n tracks.
 size_t transport_work_words(const vgmi_geom *geom, size_t n);
 // The transport loop, regrouped by volume when `work` is given and the geometry asks for it.
 vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &a, uint32_t *work, hipStream_t st);
 // Locates every point (p_in -> p_out copy, nav), clears last and taken.
-vgmi_status launch_locate_init(const vgmi_geom *geom, const double *px, const double *py, const double *pz,
-                               double *opx, double *opy, double *opz, uint32_t *nav, uint32_t *last,
-                               int32_t *taken, size_t n, hipStream_t st);
+vgmi_status launch_locate_init(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                               const double *pz, double *opx, double *opy, double *opz, uint32_t *nav,
+                               uint32_t *last, int32_t *taken, size_t n, hipStream_t st);
 
 // Records the message for vgmi_last_error() and returns `code`.
 int set_error(int code, const char *fmt, ...);
