@@ -225,6 +225,13 @@ VGMI_API vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, do
  * Results per track do not depend on it. No reference counterpart (a GPU scheduling knob).
  */
 VGMI_API vgmi_status vgmi_geom_set_regroup(vgmi_geom *geom, int chunk_steps);
+/*
+ * Tracks per wavefront in the same loops (1, 2, 4, ..., 64; 0 = automatic, the default:
+ * as few as keep every wavefront slot of the GPU busy). A wavefront executes the union of
+ * its tracks' code paths, so fewer tracks per wavefront cut the latency of small batches.
+ * Results per track do not depend on it. No reference counterpart (a GPU scheduling knob).
+ */
+VGMI_API vgmi_status vgmi_geom_set_lanes_per_wave(vgmi_geom *geom, int lanes);
 
 /* ------------------------------------------------------------------------
  * Serving loop (continuous batching). No reference counterpart: the reference

@@ -76,9 +76,11 @@ def test_safety(geo, prec):
 
 
 @PRECISIONS
+@pytest.mark.parametrize("lanes", [0, 1, 8, 64])
 @pytest.mark.parametrize("regroup", [0, 7, 10])
-def test_propagate_matches_oracle(geo, regroup, prec):
-    """The fused loop, unsorted (0) and regrouped by volume every 7 / 10 steps: same results."""
+def test_propagate_matches_oracle(geo, regroup, lanes, prec):
+    """The fused loop, unsorted (0) and regrouped by volume every 7 / 10 steps, with automatic,
+    1, 8 and 64 tracks per wavefront: same results."""
     g, G, O = geo
     n, nsteps = 1 << 14, 50
     p, d = geometry.world_points(g, n, seed=24)
@@ -88,10 +90,12 @@ def test_propagate_matches_oracle(geo, regroup, prec):
     nv, ls = _dev_u32(nav), _dev_u32(last)
     taken = torch.zeros(n, dtype=torch.int32, device="cuda")
     G.set_regroup(regroup)
+    G.set_lanes_per_wave(lanes)
     try:
         G.Propagate(P, D, 10.0, nv, ls, nsteps, taken, precision=prec)
     finally:
         G.set_regroup(10)
+        G.set_lanes_per_wave(0)
     rp, rn, rl, rtaken = O.propagate(p, d, 10.0, nav, last, nsteps, threads=8)
     assert np.array_equal(nv.cpu().numpy().astype(np.uint32), rn)
     assert np.array_equal(ls.cpu().numpy().astype(np.uint32), rl)

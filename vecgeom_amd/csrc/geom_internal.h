@@ -37,4 +37,5 @@ struct vgmi_geom {
   unsigned typeMask = 0; // bit t set when a shape of ShapeType t is present
   int nlogical = 0;
   int regroupSteps = 10; // transport loops regroup tracks by volume every this many steps (0: never)
+  int lanesPerWave = 0;  // tracks per wavefront of the transport loops (0: chosen from the track count)
 };

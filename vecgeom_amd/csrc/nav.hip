@@ -87,6 +87,7 @@ struct PropArgs {
   const uint32_t *perm;  // lane j runs track perm[*first + j] (null: track j)
   const uint32_t *first; // number of leading perm entries to skip (tracks outside the world)
   int accumulate;        // add to taken instead of storing
+  int lanes;             // tracks per wavefront (1..64); lanes >= this one idle
 };
 
 // The transport loop of one track stays in registers for all its steps; only the
@@ -95,7 +96,9 @@ template <unsigned TM>
 __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoopWaves<TM>))) void k_propagate(GeomView g, PropArgs a)
 {
   VGMI_NAV_STACK;
-  size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  const unsigned lane = threadIdx.x & 63u;
+  if (lane >= unsigned(a.lanes)) return;
+  size_t i = ((size_t(blockIdx.x) * kNavBlock + threadIdx.x) >> 6) * unsigned(a.lanes) + lane;
   if (a.perm) {
     i += *a.first;
     if (i >= a.n) return;
@@ -287,6 +290,21 @@ static vgmi_status launched(const char *who)
 
 namespace vgmi {
 
+// Tracks per wavefront of a transport loop over n tracks. A wavefront runs the union of its
+// lanes' code paths, so a lone track steps several times faster than one of 64 divergent
+// tracks; while the GPU has idle wavefront slots, tracks are spread over more wavefronts
+// (fewer per wavefront) at no cost in throughput. kWaveSlots = 256 CUs x 4 SIMDs x 4 waves.
+constexpr size_t kWaveSlots = 4096;
+int lanes_per_wave(const vgmi_geom *geom, size_t n)
+{
+  if (geom->lanesPerWave > 0) return geom->lanesPerWave;
+  int l = 1;
+  while (l < 64 && size_t(l) * kWaveSlots < n) l <<= 1;
+  return l;
+}
+// Threads to launch so that ceil(n / lanes) wavefronts of 64 lanes exist.
+static size_t lane_threads(size_t n, int lanes) { return (n + size_t(lanes) - 1) / size_t(lanes) * 64; }
+
 size_t transport_work_words(const vgmi_geom *geom, size_t n)
 {
   if (geom->regroupSteps <= 0 || n < kRegroupMinTracks) return 0;
@@ -299,9 +317,10 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
   const size_t n  = t.n;
   if (n == 0 || t.nsteps <= 0) return VGMI_OK;
   if (!work || chunk <= 0 || n < kRegroupMinTracks) {
+    const int lanes = lanes_per_wave(geom, n);
     PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, t.nsteps, t.track_steps,
-               nullptr, nullptr, 0};
-    VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, n, st, geom->view, a);
+               nullptr, nullptr, 0, lanes};
+    VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, lane_threads(n, lanes), st, geom->view, a);
     return launched("transport loop");
   }
   // Chunks of `chunk` steps; before each, the tracks still running are regrouped by the
@@ -311,6 +330,7 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
   const size_t lds = nbins <= kLocalBins ? size_t(nbins) * sizeof(uint32_t) : 0;
   const unsigned hgrid = unsigned(std::min<size_t>((n + kSortBlock - 1) / kSortBlock, 2048));
   const unsigned sgrid = unsigned((n + kSortBlock - 1) / kSortBlock);
+  const int lanes      = lanes_per_wave(geom, n);
   if (t.taken) hipLaunchKernelGGL(k_fill_u32, dim3(1024), dim3(256), 0, st, reinterpret_cast<uint32_t *>(t.taken), n, 0u);
   const SortKeys key{t.nav, t.track_steps, t.taken};
   for (int done = 0; done < t.nsteps; done += chunk) {
@@ -319,8 +339,8 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, cursor, nbins);
     hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, key, n, cursor, perm, nbins);
     PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, std::min(chunk, t.nsteps - done),
-               t.track_steps, perm, cursor, 1};
-    VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, n, st, geom->view, a);
+               t.track_steps, perm, cursor, 1, lanes};
+    VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, lane_threads(n, lanes), st, geom->view, a);
   }
   return launched("transport loop");
 }
@@ -392,6 +412,15 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
   const vgmi_status rc = launch_transport(geom, a, work, st);
   if (work) (void)hipFreeAsync(work, st);
   return rc;
+}
+
+vgmi_status vgmi_geom_set_lanes_per_wave(vgmi_geom *geom, int lanes)
+{
+  if (!geom) return set_error(VGMI_ERR_INVALID, "vgmi_geom_set_lanes_per_wave: null geometry");
+  if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+    return set_error(VGMI_ERR_INVALID, "vgmi_geom_set_lanes_per_wave: 0 (automatic) or a power of two up to 64");
+  geom->lanesPerWave = lanes;
+  return VGMI_OK;
 }
 
 vgmi_status vgmi_geom_set_regroup(vgmi_geom *geom, int chunk_steps)

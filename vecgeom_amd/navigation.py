@@ -32,6 +32,7 @@ def _declare():
         "vgmi_nav_safety": [vp, ci] + [vp] * 5 + [sz, vp],
         "vgmi_nav_propagate": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
         "vgmi_geom_set_regroup": [vp, ci],
+        "vgmi_geom_set_lanes_per_wave": [vp, ci],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -104,6 +105,10 @@ class Geometry:
     def set_regroup(self, chunk_steps):
         """Regroup tracks by volume every chunk_steps steps in the fused transport loops (0: never)."""
         check(lib().vgmi_geom_set_regroup(self._h, int(chunk_steps)), "vgmi_geom_set_regroup")
+
+    def set_lanes_per_wave(self, lanes):
+        """Tracks per wavefront of the fused loops (power of two up to 64; 0 = automatic)."""
+        check(lib().vgmi_geom_set_lanes_per_wave(self._h, int(lanes)), "vgmi_geom_set_lanes_per_wave")
 
     def LocateGlobalPoint(self, points, out=None, precision=FP64):
         n = len(points)
