@@ -5,6 +5,7 @@
 //   0 box         dx dy dz                                   (volumes/UnplacedBox.h)
 //   1 tube        rmin rmax dz sphi dphi                     (volumes/UnplacedTube.h:29)
 //   5 trapezoid   dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2 (volumes/UnplacedTrapezoid.h:56)
+//  10 hype        rmin rmax stIn stOut dz                    (volumes/UnplacedHype.h:56)
 //   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
 //                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
 #include "shapes.h"
@@ -26,6 +27,8 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 7 ? new Cone(a[0], a[1], a[2], a[3], a[4], a[5], a[6]) : nullptr;
   case 8:
     return n >= 1 ? new Orb(a[0]) : nullptr;
+  case 10:
+    return n >= 5 ? new Hype(a[0], a[1], a[2], a[3], a[4]) : nullptr;
   case 3: {
     if (n < 3) return nullptr;
     int nz = int(a[2]);

@@ -154,6 +154,38 @@ public:
   void Extent(V3 &lo, V3 &hi) const override { lo = V3(-r, -r, -r), hi = V3(r, r, r); }
 };
 
+// ---------------------------------------------------------------- Hype (UniversalHype)
+// VecGeom/volumes/HypeStruct.h:16-146 (cached members), VecGeom/volumes/kernel/HypeImplementation.h:40-330,
+// VecGeom/volumes/HypeUtilities.h:25-438 (scalar HypeHelpers<Precision> specialisation :374-436)
+class Hype : public Shape {
+public:
+  double rmin, rmax, stIn, stOut, dz;
+  double tIn, tOut, tIn2, tOut2, rmin2, rmax2, dz2;
+  double endInnerRadius2, endOuterRadius2, endInnerRadius, endOuterRadius;
+  double zTol, innerRadTol, outerRadTol;
+  Hype(double rmin, double rmax, double stIn, double stOut, double dz);
+  bool inner() const { return rmin > 0. || stIn != 0.; } // InnerSurfaceExists (HypeStruct.h:87)
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override
+  {
+    lo = V3(-endOuterRadius, -endOuterRadius, -dz), hi = V3(endOuterRadius, endOuterRadius, dz);
+  }
+  void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  bool CompletelyInside(V3 const &p) const;
+  bool CompletelyOutside(V3 const &p) const;
+  bool OnSurfaceMovingIn(V3 const &p, V3 const &d) const;
+  bool OnSurfaceMovingOut(V3 const &p, V3 const &d) const;
+  bool OnOuterMovingOut(V3 const &p, V3 const &d) const;
+  bool OnInnerMovingOut(V3 const &p, V3 const &d) const;
+  bool ZPlane(V3 const &p, V3 const &d, bool toIn, double &zDist) const;
+  bool Hyperbolic(V3 const &p, V3 const &d, bool toIn, bool innerSurf, double &dist) const;
+};
+
 // ---------------------------------------------------------------- Cone (UniversalCone)
 // VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
 // ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820

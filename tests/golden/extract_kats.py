@@ -451,6 +451,7 @@ SHAPES = {
     "polyhedron": ("TestPolyhedra.cpp", r"Polyhedra_t", "polyhedron"),
     "torus2": ("TestTorus2.cpp", r"Torus_t", (5,)),
     "tessellated": ("TestTessellated.cpp", r"SimpleTessellated", "tessellated"),
+    "hype": ("TestHype.cpp", r"Hype_t", (5,)),
 }
 
 if __name__ == "__main__":

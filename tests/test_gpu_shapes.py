@@ -61,6 +61,10 @@ CASES = [
     ("tessellated", trd_like_triangles(10, 30, 20, 40, 40)),
     ("tessellated", trd_like_triangles(0.15, 0.15, 24.707, 24.707, 22.7)),
     ("tessellated", workloads.convex_hull_sphere(500.0, nphi=12, nbands=7)),
+    ("hype", [10.0, 20.0, math.pi / 4, math.pi / 3, 50.0]),
+    ("hype", [10.0, 20.0, math.pi / 4, math.pi / 4, 50.0]),
+    ("hype", [0.0, 20.0, 0.0, math.pi / 3, 50.0]),
+    ("hype", [10.0, 20.0, 0.0, math.pi / 4, 50.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing stops once a step is below
@@ -102,7 +106,7 @@ def test_query_all_matches_single_methods():
 
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
-                                   "torus2", "tessellated"])
+                                   "torus2", "tessellated", "hype"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

@@ -8,7 +8,8 @@ import pytest
 from oracle.pyoracle import OracleShape
 from tests import kat_util
 
-SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated"]
+SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated",
+          "hype"]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

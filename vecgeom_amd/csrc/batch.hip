@@ -126,6 +126,7 @@ static int cost_class(int type)
   case kTube:
     return 2;
   case kCone:
+  case kHype:
     return 4;
   case kPolycone:
   case kPolyhedron:

@@ -3,6 +3,7 @@
 #pragma once
 #include "dev_box.h"
 #include "dev_cone.h"
+#include "dev_hype.h"
 #include "dev_orb.h"
 #include "dev_polycone.h"
 #include "dev_polyhedron.h"
@@ -23,7 +24,8 @@
   X(kSphere, SphereData, sph, sph)                                                                                   \
   X(kPolyhedron, PolyhedronData, phed, phed)                                                                         \
   X(kTorus2, TorusData, tor, tor)                                                                                    \
-  X(kTessellated, TessellatedData, tess, tess)
+  X(kTessellated, TessellatedData, tess, tess)                                                                       \
+  X(kHype, HypeData, hype, hype)
 
 namespace vgmi {
 

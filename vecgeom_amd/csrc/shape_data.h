@@ -134,6 +134,17 @@ struct TorusData {
   TubeData bounding;
 };
 
+// Hype (VecGeom/volumes/HypeStruct.h:16-83): defining parameters and the cached tangents,
+// squares, end radii and tolerance levels of CalculateCached.
+struct HypeData {
+  double rmin, rmax, stIn, stOut, dz;
+  double tIn, tOut, tIn2, tOut2, rmin2, rmax2, dz2;
+  double endInnerRadius2, endOuterRadius2, endInnerRadius, endOuterRadius;
+  double zTol, innerRadTol, outerRadTol;
+  int32_t inner; // InnerSurfaceExists: rmin > 0 or stIn != 0
+  int32_t pad_;
+};
+
 // Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
 // and a threaded BVH over them live in the variable-length pool:
 //   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
@@ -168,6 +179,7 @@ struct ShapeRecord {
     PolyhedronData phed;
     TorusData tor;
     TessellatedData tess;
+    HypeData hype;
     double raw[64];
   } u;
 };

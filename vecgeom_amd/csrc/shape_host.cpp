@@ -206,6 +206,34 @@ int MakeOrb(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
+// Hype: VecGeom/volumes/HypeStruct.h:54-83 (CalculateCached); UniversalHype (the inner surface
+// is treated when rmin > 0 or stIn != 0, HypeStruct.h:87 and kernel/shapetypes/HypeTypes.h:63-70)
+int MakeHype(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 5) return err = "hype needs 5 parameters (rmin rmax stIn stOut dz)", -1;
+  if (!(a[1] > 0 && a[4] > 0 && a[0] >= 0 && a[0] < a[1])) return err = "hype needs 0 <= rmin < rmax and dz > 0", -1;
+  r           = ShapeRecord();
+  r.type      = kHype;
+  HypeData &h = r.u.hype;
+  h.rmin = a[0], h.rmax = a[1], h.stIn = a[2], h.stOut = a[3], h.dz = a[4];
+  h.tIn             = std::tan(h.stIn);
+  h.tOut            = std::tan(h.stOut);
+  h.tIn2            = h.tIn * h.tIn;
+  h.tOut2           = h.tOut * h.tOut;
+  h.rmin2           = h.rmin * h.rmin;
+  h.rmax2           = h.rmax * h.rmax;
+  h.dz2             = h.dz * h.dz;
+  h.endInnerRadius2 = h.tIn2 * h.dz2 + h.rmin2;
+  h.endOuterRadius2 = h.tOut2 * h.dz2 + h.rmax2;
+  h.endInnerRadius  = std::sqrt(h.endInnerRadius2);
+  h.endOuterRadius  = std::sqrt(h.endOuterRadius2);
+  h.zTol            = 1e-9 * h.dz;
+  h.innerRadTol     = 1e-9 * h.endInnerRadius;
+  h.outerRadTol     = 1e-9 * h.endOuterRadius;
+  h.inner           = (h.rmin > 0. || h.stIn != 0.) ? 1 : 0;
+  return 0;
+}
+
 // Cone: VecGeom/volumes/ConeStruct.h:415-426 (constructor), :309-346 (angle normalisation: the
 // wedge keeps the raw angles, the phi-plane vectors use the normalised ones) and :76-140 (cache).
 int MakeCone(double const *a, int n, ShapeRecord &r, std::string &err)
@@ -779,6 +807,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeTorus2(params, nparams, r, err);
   case kTessellated:
     return MakeTessellated(params, nparams, r, extra, err);
+  case kHype:
+    return MakeHype(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -838,6 +868,13 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
   case kTessellated:
     for (int i = 0; i < 3; ++i) lo[i] = r.u.tess.minExt[i], hi[i] = r.u.tess.maxExt[i];
     return;
+  case kHype: {
+    // source/UnplacedHype.cpp:94-100
+    const double e = r.u.hype.endOuterRadius;
+    lo[0] = lo[1] = -e, hi[0] = hi[1] = e;
+    lo[2] = -r.u.hype.dz, hi[2] = r.u.hype.dz;
+    return;
+  }
   default:
     lo[0] = lo[1] = lo[2] = 0;
     hi[0] = hi[1] = hi[2] = 0;
