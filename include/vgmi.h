@@ -199,6 +199,10 @@ VGMI_API vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const d
                                    uint32_t *nav_out, uint32_t *last_out, uint8_t *boundary_out, double *step_out,
                                    size_t n, int relocate, void *stream);
 
+/* precision (all vgmi_nav_* calls): VGMI_FP64, or VGMI_MIXED, which tests BVH boxes in FP32 against
+ * widened FP32 copies as conservative filters in front of the same FP64 decisions (results are the
+ * FP64 results bit for bit). */
+
 /* Isotropic safety: VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
  * (navigation/VSafetyEstimator.h:98-107, navigation/BVHSafetyEstimator.h:38-46). */
 VGMI_API vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,
@@ -251,7 +255,8 @@ VGMI_API vgmi_status vgmi_geom_set_lanes_per_wave(vgmi_geom *geom, int lanes);
  * lane_tracks tracks, or one that would put more than queue_tracks tracks in
  * the queue, is refused with VGMI_ERR_CAPACITY; creation fails with
  * VGMI_ERR_CAPACITY when the lanes would take more than half of the free HBM.
- * Only VGMI_FP64 is implemented (VGMI_ERR_UNSUPPORTED otherwise).
+ * precision is VGMI_FP64 or VGMI_MIXED (the navigation kernels' mixed mode: FP32 box filters,
+ * results identical to FP64).
  * ------------------------------------------------------------------------ */
 typedef struct vgmi_server vgmi_server;
 
