@@ -199,6 +199,21 @@ VGMI_API vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const d
                                    uint32_t *nav_out, uint32_t *last_out, uint8_t *boundary_out, double *step_out,
                                    size_t n, int relocate, void *stream);
 
+/*
+ * Step and isotropic safety per track: VNavigatorHelper<BVHNavigator>::
+ * ComputeStepAndSafetyAndPropagatedState (navigation/VNavigator.h:1014-1064), the batched form
+ * ComputeStepsAndSafetiesAndPropagatedStates (:136-139, :947-...). safety_out[i] is the safety of the
+ * start point (BVHSafetyEstimator::ComputeSafetyForLocalPoint) when calc_safety[i] is non-zero,
+ * else 0; calc_safety may be NULL (every track). The step and out-state are those of vgmi_nav_step
+ * with relocate = 1.
+ */
+VGMI_API vgmi_status vgmi_nav_step_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                                          const double *pz, const double *dx, const double *dy, const double *dz,
+                                          const double *step_limit, const uint32_t *nav_in, const uint32_t *last_in,
+                                          const uint8_t *calc_safety, uint32_t *nav_out, uint32_t *last_out,
+                                          uint8_t *boundary_out, double *step_out, double *safety_out, size_t n,
+                                          void *stream);
+
 /* precision (all vgmi_nav_* calls): VGMI_FP64, or VGMI_MIXED, which tests BVH boxes in FP32 against
  * widened FP32 copies as conservative filters in front of the same FP64 decisions (results are the
  * FP64 results bit for bit). */

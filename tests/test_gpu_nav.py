@@ -66,6 +66,33 @@ def test_step_by_step(geo, prec):
 
 
 @PRECISIONS
+def test_step_and_safety(geo, prec):
+    """ComputeStepAndSafetyAndPropagatedState: the step and out-state of ComputeStepAndPropagatedState
+    and the safety of the start point (VNavigator.h:1014-1064), where requested."""
+    g, G, O = geo
+    n = 1 << 14
+    p, d = geometry.world_points(g, n, seed=25)
+    nav = O.locate(p, threads=8)
+    last = np.zeros_like(nav)
+    calc = np.random.default_rng(5).random(n) < 0.7
+    D = SOA3D.from_array(d)
+    pp, nv, ls = p.copy(), nav.copy(), last.copy()
+    for s in range(10):
+        step, nav_o, last_o, bnd, saf = G.ComputeStepAndSafety(
+            SOA3D.from_array(pp), D, 10.0, _dev_u32(nv), _dev_u32(ls),
+            calc_safety=torch.as_tensor(calc, device="cuda"), precision=prec)
+        rs, rn, rl, rb = O.step(pp, d, 10.0, nv, ls, relocate=True, threads=8)
+        rsaf = np.where(calc & (nv != 0), O.safety(pp, nv, threads=8), 0.0)
+        assert np.array_equal(nav_o.cpu().numpy().astype(np.uint32), rn), s
+        assert np.array_equal(last_o.cpu().numpy().astype(np.uint32), rl), s
+        assert np.array_equal(bnd.cpu().numpy(), rb), s
+        assert np.array_equal(step.cpu().numpy(), rs), s
+        assert np.array_equal(saf.cpu().numpy(), rsaf), (s, float(np.max(np.abs(saf.cpu().numpy() - rsaf))))
+        pp = pp + rs * d
+        nv, ls = rn, rl
+
+
+@PRECISIONS
 def test_safety(geo, prec):
     g, G, O = geo
     p, _ = geometry.world_points(g, 1 << 15, seed=23)

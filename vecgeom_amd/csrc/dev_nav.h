@@ -80,11 +80,15 @@ VGMI_DEV void top_matrix(GeomView const &g, uint32_t nav, Mat &m)
   const uint32_t nd = w >> 16;
   uint32_t ti       = nav + 3 + nd + ((nd + 1) & 1);
   ti += ti & 1u; // 8-byte alignment of the matrix block
-  const double *a = reinterpret_cast<const double *>(g.navind + ti);
-  if (hasm & 0x02u)
-    for (int i = 0; i < 3; ++i) m.t[i] = a[i];
-  if (hasm & 0x01u)
-    for (int i = 0; i < 9; ++i) m.r[i] = a[3 + i];
+  // six 16-byte loads of the (8-byte aligned) 12 doubles, which every node stores
+  typedef double D2 __attribute__((ext_vector_type(2), aligned(8)));
+  const D2 *a  = reinterpret_cast<const D2 *>(g.navind + ti);
+  const D2 v0 = a[0], v1 = a[1], v2 = a[2], v3 = a[3], v4 = a[4], v5 = a[5];
+  if (hasm & 0x02u) m.t[0] = v0.x, m.t[1] = v0.y, m.t[2] = v1.x;
+  if (hasm & 0x01u) {
+    m.r[0] = v1.y, m.r[1] = v2.x, m.r[2] = v2.y, m.r[3] = v3.x, m.r[4] = v3.y;
+    m.r[5] = v4.x, m.r[6] = v4.y, m.r[7] = v5.x, m.r[8] = v5.y;
+  }
 }
 
 // ---- shape dispatch on a device shape record. Every function below is templated on TM,
@@ -313,12 +317,28 @@ VGMI_DEV void check_daughters_mixed(GeomView const &g, NavVol const &M, V3 const
     const int nc       = N.nchild;
     const float lim    = float(vmin(step, capm));
     if (nc >= 0) {
-      for (int i = 0; i < nc; ++i) {
-        NavSlot32 const &box = slots[N.offset + i];
-        float t;
-        const bool pass = slab32_pass(box, pf, invf, float(vmin(step, capm)), t);
-        double lb       = box_safety32(box, pf);
-        if (pass) {
+      // A leaf in two passes: the FP32 box tests of all its slots (each slot's load issued
+      // while the previous one is tested), then the FP64 candidates in slot order. The first
+      // pass uses the step at the leaf's entry, a superset of the FP64 walk's tests.
+      for (int base = 0; base < nc; base += 32) {
+        const int cnt = nc - base < 32 ? nc - base : 32;
+        const NavSlot32 *ls = slots + N.offset + base;
+        uint32_t pass = 0;
+        NavSlot32 cur = ls[0];
+        for (int i = 0; i < cnt; ++i) {
+          const NavSlot32 nxt = ls[i + 1 < cnt ? i + 1 : i];
+          float t;
+          if (slab32_pass(cur, pf, invf, lim, t))
+            pass |= 1u << i;
+          else
+            bnd = vmin(bnd, box_safety32(cur, pf));
+          cur = nxt;
+        }
+        while (pass) {
+          const int i = __builtin_ctz(pass);
+          pass &= pass - 1u;
+          NavSlot32 const &box = ls[i];
+          double lb            = box_safety32(box, pf);
           const int pv         = box.pvol;
           NavVol const &P      = g.vols[pv];
           ShapeRecord const &S = g.shapes[box.shape];
@@ -329,8 +349,8 @@ VGMI_DEV void check_daughters_mixed(GeomView const &g, NavVol const &M, V3 const
             const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
             if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
           }
+          bnd = vmin(bnd, lb);
         }
-        bnd = vmin(bnd, lb);
       }
     } else {
       const uint32_t l = 2 * id + 1, r = 2 * id + 2;
@@ -377,14 +397,25 @@ VGMI_DEV bool level_locate_mixed(GeomView const &g, NavVol const &M, int excl, V
     NavNode32 const &N = nodes[id];
     const int nc       = N.nchild;
     if (nc >= 0) {
-      for (int i = 0; i < nc; ++i) {
-        NavSlot32 const &box = slots[N.offset + i];
-        if (box_contains32(box, pf)) {
-          const int pv = box.pvol;
+      // FP32 containment of all slots first (loads issued ahead), then FP64 in slot order
+      for (int base = 0; base < nc; base += 32) {
+        const int cnt = nc - base < 32 ? nc - base : 32;
+        const NavSlot32 *ls = slots + N.offset + base;
+        uint32_t in = 0;
+        NavSlot32 cur = ls[0];
+        for (int i = 0; i < cnt; ++i) {
+          const NavSlot32 nxt = ls[i + 1 < cnt ? i + 1 : i];
+          if (box_contains32(cur, pf)) in |= 1u << i;
+          cur = nxt;
+        }
+        while (in) {
+          const int i = __builtin_ctz(in);
+          in &= in - 1u;
+          const int pv = ls[i].pvol;
           if (pv == excl) continue;
           NavVol const &P = g.vols[pv];
           const V3 loc    = to_local(P.t, P.r, lp);
-          if (contains<TM>(g.shapes[box.shape], loc)) {
+          if (contains<TM>(g.shapes[ls[i].shape], loc)) {
             pvol = pv;
             dlp  = loc;
             return true;
@@ -570,6 +601,25 @@ VGMI_DEV uint32_t relocate_force_different(GeomView const &g, V3 const &lp, uint
   return path;
 }
 
+// BVHSafetyEstimator::ComputeSafetyForLocalPoint (VecGeom/navigation/BVHSafetyEstimator.h:38-45)
+template <unsigned TM>
+VGMI_DEV double compute_safety_local(GeomView const &g, V3 const &lp, NavVol const &M, Stack stk)
+{
+  double s = safety_out<TM>(g.shapes[M.shape], lp);
+  if (s > 0.0 && M.ndaughters > 0) s = safety_to_daughters<TM>(g, M, lp, s, stk);
+  return s;
+}
+
+// VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
+template <unsigned TM>
+VGMI_DEV double compute_safety(GeomView const &g, V3 const &gp, uint32_t nav, Stack stk)
+{
+  Mat m;
+  top_matrix(g, nav, m);
+  const V3 lp = to_local(m.t, m.r, gp);
+  return compute_safety_local<TM>(g, lp, g.vols[top(g, nav)], stk);
+}
+
 struct State {
   uint32_t nav, last;
   bool boundary;
@@ -585,9 +635,13 @@ constexpr double kBudgetMargin = 1e-6;
 // the step limit and the mother distance is finite, no daughter can be hit within the limit,
 // so the daughter walk cannot change the outcome of the step and is skipped; the walk itself
 // refreshes the bound. Every change of state invalidates it.
+// With `safety` non-null: VNavigatorHelper::ComputeStepAndSafetyAndPropagatedState
+// (VecGeom/navigation/VNavigator.h:1014-1064), the isotropic safety of the local start point
+// first (0 when !calc_safety), then the same step.
 template <unsigned TM>
 VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, double limit, State const &in,
-                             State &out, bool relocate, Stack stk, double &budget)
+                             State &out, bool relocate, Stack stk, double &budget, double *safety = nullptr,
+                             bool calc_safety = false)
 {
   Mat m;
   top_matrix(g, in.nav, m);
@@ -595,6 +649,7 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   const V3 ld       = rotate(m.r, gd);
   const int pvol    = top(g, in.nav);
   NavVol const &M   = g.vols[pvol];
+  if (safety) *safety = calc_safety ? compute_safety_local<TM>(g, lp, M, stk) : 0.;
   double step       = dist_out<TM>(g.shapes[M.shape], lp, ld);
   if (step < 0.) step = 0.;
   int hit = -1;
@@ -631,19 +686,6 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
     out.nav         = locate_down<TM>(g, hit, -1, to_local(N.t, N.r, after), out.nav, stk);
   }
   return step;
-}
-
-// VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
-template <unsigned TM>
-VGMI_DEV double compute_safety(GeomView const &g, V3 const &gp, uint32_t nav, Stack stk)
-{
-  Mat m;
-  top_matrix(g, nav, m);
-  const V3 lp     = to_local(m.t, m.r, gp);
-  NavVol const &M = g.vols[top(g, nav)];
-  double s        = safety_out<TM>(g.shapes[M.shape], lp);
-  if (s > 0.0 && M.ndaughters > 0) s = safety_to_daughters<TM>(g, M, lp, s, stk);
-  return s;
 }
 
 } // namespace nav

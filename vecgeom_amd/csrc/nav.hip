@@ -14,8 +14,11 @@ constexpr unsigned kSimpleTypes = (1u << kBox) | (1u << kTube) | (1u << kCone) |
 constexpr unsigned kAllTypes    = ~kMixedFlag;
 // Waves per SIMD the transport-loop kernels are compiled for: the simple-solid variants are
 // held to 128 VGPRs (4 waves, a few bytes of spill); the all-solid variants keep every register.
+#ifndef VGMI_LOOP_WAVES
+#define VGMI_LOOP_WAVES 4
+#endif
 template <unsigned TM>
-constexpr unsigned kLoopWaves = (TM & ~kMixedFlag) == kAllTypes ? 1u : 4u;
+constexpr unsigned kLoopWaves = (TM & ~kMixedFlag) == kAllTypes ? 1u : unsigned(VGMI_LOOP_WAVES);
 
 #define VGMI_NAV_STACK                                                                                               \
   __shared__ uint32_t lds_stack[kStackDepth * kNavBlock];                                                           \
@@ -40,6 +43,8 @@ struct StepArgs {
   double *step_out;
   size_t n;
   int relocate;
+  const uint8_t *calc_safety; // with safety_out: per-track flag, null = every track
+  double *safety_out;         // null: no safety
 };
 
 template <unsigned TM>
@@ -51,10 +56,14 @@ __global__ __launch_bounds__(kNavBlock) void k_step(GeomView g, StepArgs a)
   State in{a.nav_in[i], a.last_in ? a.last_in[i] : 0u, false}, out = in;
   double step = 0., budget = -1.;
   if (in.nav != 0) {
+    double safety = 0.;
     step = compute_step<TM>(g, V3(a.px[i], a.py[i], a.pz[i]), V3(a.dx[i], a.dy[i], a.dz[i]), a.limit[i], in, out,
-                        a.relocate != 0, stk, budget);
+                            a.relocate != 0, stk, budget, a.safety_out ? &safety : nullptr,
+                            a.calc_safety ? a.calc_safety[i] != 0 : true);
+    if (a.safety_out) a.safety_out[i] = safety;
   } else {
     out.nav = 0;
+    if (a.safety_out) a.safety_out[i] = 0.;
   }
   a.step_out[i] = step;
   a.nav_out[i]  = out.nav;
@@ -378,9 +387,25 @@ vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const double *px
   if (!px || !py || !pz || !dx || !dy || !dz || !step_limit || !nav_in || !nav_out || !step_out)
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_step: null pointer");
   StepArgs a{px, py, pz, dx, dy, dz, step_limit, nav_in, last_in, nav_out, last_out, boundary_out, step_out, n,
-             relocate};
+             relocate, nullptr, nullptr};
   VGMI_NAV_LAUNCH(k_step, geom, precision, n, stream, geom->view, a);
   return launched("vgmi_nav_step");
+}
+
+vgmi_status vgmi_nav_step_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,
+                                 const double *pz, const double *dx, const double *dy, const double *dz,
+                                 const double *step_limit, const uint32_t *nav_in, const uint32_t *last_in,
+                                 const uint8_t *calc_safety, uint32_t *nav_out, uint32_t *last_out,
+                                 uint8_t *boundary_out, double *step_out, double *safety_out, size_t n, void *stream)
+{
+  if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_step_safety")) return s;
+  if (n == 0) return VGMI_OK;
+  if (!px || !py || !pz || !dx || !dy || !dz || !step_limit || !nav_in || !nav_out || !step_out || !safety_out)
+    return set_error(VGMI_ERR_INVALID, "vgmi_nav_step_safety: null pointer");
+  StepArgs a{px, py, pz, dx, dy, dz, step_limit, nav_in, last_in, nav_out, last_out, boundary_out, step_out, n,
+             1, calc_safety, safety_out};
+  VGMI_NAV_LAUNCH(k_step, geom, precision, n, stream, geom->view, a);
+  return launched("vgmi_nav_step_safety");
 }
 
 vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,

@@ -165,10 +165,11 @@ constexpr int kNodeDoubles  = 8;
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a
 // separate double pool addressed by an offset inside the record.
-struct ShapeRecord {
+// 16-byte aligned, with the parameters from offset 16, so adjacent double pairs load as dwordx4.
+struct alignas(16) ShapeRecord {
   int32_t type;
-  int32_t pad_;
-  union {
+  int32_t pad_[3];
+  union alignas(16) {
     BoxData box;
     TubeData tube;
     TrapezoidData trap;

@@ -30,6 +30,7 @@ def _declare():
         "vgmi_nav_locate": [vp, ci] + [vp] * 4 + [sz, vp],
         "vgmi_nav_step": [vp, ci] + [vp] * 13 + [sz, ci, vp],
         "vgmi_nav_safety": [vp, ci] + [vp] * 5 + [sz, vp],
+        "vgmi_nav_step_safety": [vp, ci] + [vp] * 15 + [sz, vp],
         "vgmi_nav_propagate": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
         "vgmi_geom_set_regroup": [vp, ci],
         "vgmi_geom_set_lanes_per_wave": [vp, ci],
@@ -133,6 +134,27 @@ class Geometry:
                                   nav_out.data_ptr(), last_out.data_ptr(), bnd.data_ptr(), step.data_ptr(), n,
                                   1 if relocate else 0, _stream()), "vgmi_nav_step")
         return step, nav_out, last_out, bnd
+
+    def ComputeStepAndSafety(self, points, dirs, step_limit, nav_in, last_in=None, calc_safety=None,
+                             precision=FP64):
+        """ComputeStepAndSafetyAndPropagatedState for every track: returns (step, nav_out, last_out,
+        boundary, safety); safety is that of the start point where calc_safety (default: all)."""
+        n = len(points)
+        dev = points.x.device
+        if not torch.is_tensor(step_limit):
+            step_limit = torch.full((n,), float(step_limit), dtype=torch.float64, device=dev)
+        step = torch.empty(n, dtype=torch.float64, device=dev)
+        safety = torch.empty(n, dtype=torch.float64, device=dev)
+        nav_out = torch.empty(n, dtype=torch.int32, device=dev)
+        last_out = torch.empty(n, dtype=torch.int32, device=dev)
+        bnd = torch.empty(n, dtype=torch.uint8, device=dev)
+        calc = calc_safety.to(torch.uint8) if calc_safety is not None else None
+        check(lib().vgmi_nav_step_safety(self._h, precision, *points.ptrs(), *dirs.ptrs(), step_limit.data_ptr(),
+                                         nav_in.data_ptr(), last_in.data_ptr() if last_in is not None else None,
+                                         calc.data_ptr() if calc is not None else None, nav_out.data_ptr(),
+                                         last_out.data_ptr(), bnd.data_ptr(), step.data_ptr(), safety.data_ptr(), n,
+                                         _stream()), "vgmi_nav_step_safety")
+        return step, nav_out, last_out, bnd, safety
 
     def ComputeSafety(self, points, nav, precision=FP64):
         n = len(points)
