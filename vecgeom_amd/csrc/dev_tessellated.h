@@ -211,9 +211,40 @@ VGMI_DEV double seg_dist2(V3d const &c0, V3d const &c1, V3d const &p)
   return ((c0 + (dot0 / dot1) * line) - p).Mag2();
 }
 
-// SafetySq<ToIn>: the smallest per-facet squared safety. `bound` is an upper bound of that
-// minimum (used only to skip subtrees farther away; the result is the minimum itself).
-VGMI_DEV double safety_sq(TessellatedData const &s, V3d const &p, bool toIn, double bound = kInfLength)
+// Traversal stack of the facet BVH: a tree of at most kTessMaxFacets facets (median splits, four
+// per leaf; the host builder enforces the count) has at most 22 internal levels.
+constexpr int kTessStack = 24;
+
+// Squared safety of one facet (TessellatedCluster::SafetySq, TessellatedCluster.h:498-540 with one
+// facet per cluster): the squared plane distance when the projection lies within the triangle on
+// the right side, else the smallest squared distance to its edges. Facets whose squared plane
+// distance exceeds `lim` are skipped (their value is at least that, since the projection and every
+// edge point lie in the plane; the 1e-9 margin covers the rounding of both quantities).
+VGMI_DEV void facet_safety_sq(const double *F, V3d const &p, bool toIn, double lim, double &best)
+{
+  const V3d n     = at3(F + 3);
+  const double sp = p.Dot(n) + F[6];
+  if (sp * sp > lim * (1. + 1e-9)) return;
+  const V3d q  = p - sp * n;
+  const V3d v0 = at3(F + 7), v1 = at3(F + 13), v2 = at3(F + 19);
+  bool within = (q - v0).Dot(at3(F + 10)) > -kTolerance && (q - v1).Dot(at3(F + 16)) > -kTolerance &&
+                (q - v2).Dot(at3(F + 22)) > -kTolerance;
+  within &= toIn ? sp > -kTolerance : sp < kTolerance;
+  double sq = sp * sp;
+  if (!within) {
+    const double e0 = seg_dist2(v0, v1, p), e1 = seg_dist2(v1, v2, p), e2 = seg_dist2(v2, v0, p);
+    sq              = vmin(e0, vmin(e1, e2));
+  }
+  if (sq < best) best = sq;
+}
+
+// SafetySq<ToIn>: the smallest per-facet squared safety over all facets (the minimum does not
+// depend on the visiting order). `bound` is an upper bound of that minimum; subtrees whose box is
+// farther than the current minimum (or the bound) are skipped. From outside (ToIn) the BVH is
+// walked nearer child first with a per-lane stack, which finds the closest facets early; from
+// inside most boxes hold the point (distance 0, no order to exploit), so the threaded
+// depth-first walk, which reads each box once, is used.
+VGMI_DEV double safety_sq_threaded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
 {
   double best = kInfLength;
   int j       = 0;
@@ -229,25 +260,52 @@ VGMI_DEV double safety_sq(TessellatedData const &s, V3d const &p, bool toIn, dou
       continue;
     }
     const int first = node_first(N);
-    for (int k = first; k < first + cnt; ++k) {
-      const double *F = facet(s, k);
-      const V3d n     = at3(F + 3);
-      const double sp = p.Dot(n) + F[6];
-      const V3d q     = p - sp * n;
-      const V3d v0 = at3(F + 7), v1 = at3(F + 13), v2 = at3(F + 19);
-      bool within = (q - v0).Dot(at3(F + 10)) > -kTolerance && (q - v1).Dot(at3(F + 16)) > -kTolerance &&
-                    (q - v2).Dot(at3(F + 22)) > -kTolerance;
-      within &= toIn ? sp > -kTolerance : sp < kTolerance;
-      double sq = sp * sp;
-      if (!within) {
-        const double e0 = seg_dist2(v0, v1, p), e1 = seg_dist2(v1, v2, p), e2 = seg_dist2(v2, v0, p);
-        sq              = vmin(e0, vmin(e1, e2));
-      }
-      if (sq < best) best = sq;
-    }
+    for (int k = first; k < first + cnt; ++k) facet_safety_sq(facet(s, k), p, toIn, vmin(best, bound), best);
     j = node_escape(N);
   }
   return best;
+}
+
+VGMI_DEV double safety_sq(TessellatedData const &s, V3d const &p, bool toIn, double bound = kInfLength)
+{
+  if (!toIn) return safety_sq_threaded(s, p, toIn, bound);
+  double best = kInfLength;
+  int stack[kTessStack];
+  int sp = 0, j = 0;
+  if (node_dist2(node(s, 0), p) > bound) return best;
+  for (;;) {
+    const double *N = node(s, j);
+    const int cnt   = node_count(N);
+    bool descend    = false;
+    if (cnt > 0) {
+      const int first = node_first(N);
+      for (int k = first; k < first + cnt; ++k) facet_safety_sq(facet(s, k), p, toIn, vmin(best, bound), best);
+    } else {
+      const int l = j + 1, r = node_escape(node(s, l));
+      const double lim = vmin(best, bound);
+      const double dl = node_dist2(node(s, l), p), dr = node_dist2(node(s, r), p);
+      const bool vl = !(dl > lim), vr = !(dr > lim);
+      if (vl && vr) {
+        stack[sp++] = dl <= dr ? r : l;
+        j           = dl <= dr ? l : r;
+        descend     = true;
+      } else if (vl || vr) {
+        j       = vl ? l : r;
+        descend = true;
+      }
+    }
+    if (descend) continue;
+    int next = -1;
+    while (sp > 0) {
+      const int c = stack[--sp];
+      if (!(node_dist2(node(s, c), p) > vmin(best, bound))) {
+        next = c;
+        break;
+      }
+    }
+    if (next < 0) return best;
+    j = next;
+  }
 }
 
 // An upper bound of the smallest per-facet squared safety. That minimum is at most the squared
