@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", default="50,100,200",
                     help="Poisson request rates (requests/s, all producers together) of the serving sweep")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-check", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (calibrates HBM byte counters) and exit")
     ap.add_argument("--no-tune", action="store_true")
@@ -73,6 +73,18 @@ def cpu_info():
     except OSError:
         pass
     return model, os.cpu_count()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (profiles/traffic.json, scaled by a plain-copy calibration), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            k = json.load(f)["kernels"].get(kernel)
+        return None if k is None else {"bytes": k["traffic_bytes"], "workload": k["workload"],
+                                       "source": "profiles/pmc_hbm_traffic.md"}
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def event_time_us(fn, reps):
@@ -137,7 +149,8 @@ def run_mix(args):
                                          if k not in {s for s, _ in shapes}],
                       "tuning": tuning or batch.info()},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_batch",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("vgmi::k_batch<256>"),
+                        "kernel": "k_batch",
                         "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)}}
     if not args.no_cpu:
         from oracle.pyoracle import OracleShape
@@ -198,7 +211,9 @@ def run_box_tube(args):
     res = {"queries_per_s": round(queries / (total_us * 1e-6), 1), "points_per_shape": n,
            "shapes": [[k, p] for k, p, *_ in shapes],
            "roofline": {"bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4), "traffic": None,
+                        "frac": round(dom["GBps"] / HBM_PEAK_GBS, 4),
+                        "traffic": measured_traffic("vgmi::k_query_fp64<vgmi::%sData, %d>" % (
+                            {"box": "Box", "tube": "Tube"}[kind], METHODS.index(m))),
                         "kernel": "k_query_fp64 (%s)" % dom["kernel"], "bytes_per_launch": BYTES[m] * n},
            "kernels": per_kernel}
     if not args.no_cpu:
@@ -270,7 +285,10 @@ def run_nav(args):
         res[name] = {"nav_steps_per_s": round(steps_per_run / (avg_us * 1e-6), 1), "avg_launch_us": round(avg_us, 2),
                      "steps_per_launch": steps_per_run,
                      "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None, "kernel": "k_propagate",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 5),
+                                  "traffic": measured_traffic("vgmi::nav::k_propagate<%s>" % (
+                                      "295u" if name == "fp64" else "2147483943u")),
+                                  "kernel": "k_propagate",
                                   "bytes_per_launch": bytes_launch,
                                   "note": "latency/divergence bound: track state stays in registers for all steps"}}
         if name == "mixed":
@@ -309,27 +327,31 @@ def run_serve(args):
     g = geometry.calorimeter()
     G = Geometry(g)
     pool = geometry.world_points(g, 1 << 20, seed=0x5EED0005)
+    from vecgeom_amd.shapes import FP64, MIXED
     out = {"workload": "calorimeter; 16 producer threads, Poisson arrivals; batch 2^U(10,17.6) tracks; "
                        "U{1..100} steps per track; 10 mm step limit; latency = submit to results in host memory",
            "lanes": 4, "lane_tracks": 1 << 18}
-    with Server(G, lane_tracks=1 << 18, lanes=4, queue_tracks=1 << 23) as S:
-        rng = np.random.default_rng(0)
-        alone = {}
-        for n in (1024, 65536):
-            lat = []
-            for _ in range(5):
-                r = S.wait(S.submit(pool[0][:, :n], pool[1][:, :n], rng.integers(1, 101, size=n, dtype=np.int32)))
-                lat.append(r.latency_us / 1e3)
-            alone[str(n)] = round(float(np.median(lat)), 3)
-        out["alone_latency_ms"] = alone
-        sweep = []
-        for rate in [float(x) for x in args.serve_rates.split(",") if x]:
-            rec, refused, wall = poisson_load(S, pool, rate_hz=rate, duration_s=args.serve_seconds, producers=16,
-                                              seed=int(rate))
-            s = summarize(rec, refused, wall)
-            s["rate_hz"] = rate
-            sweep.append(s)
-        out["sweep"] = sweep
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        res = {}
+        with Server(G, precision=prec, lane_tracks=1 << 18, lanes=4, queue_tracks=1 << 23) as S:
+            rng = np.random.default_rng(0)
+            alone = {}
+            for n in (1024, 16384, 65536):
+                lat = []
+                for _ in range(5):
+                    r = S.wait(S.submit(pool[0][:, :n], pool[1][:, :n], rng.integers(1, 101, size=n, dtype=np.int32)))
+                    lat.append(r.latency_us / 1e3)
+                alone[str(n)] = round(float(np.median(lat)), 3)
+            res["alone_latency_ms"] = alone
+            sweep = []
+            for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+                rec, refused, wall = poisson_load(S, pool, rate_hz=rate, duration_s=args.serve_seconds, producers=16,
+                                                  seed=int(rate))
+                sm = summarize(rec, refused, wall)
+                sm["rate_hz"] = rate
+                sweep.append(sm)
+            res["sweep"] = sweep
+        out[name] = res
     return out
 
 

@@ -3,7 +3,8 @@
 Bar: every request's result is bit-identical whether it runs alone or packed with
 others into shared launches (tracks are independent and the FP64 kernels are
 deterministic), and matches the oracle's transport loop (points within kTolerance,
-expected bit-identical; NavStateIndex values and step counts exact).
+expected bit-identical; NavStateIndex values and step counts exact); the mixed-precision
+server returns the FP64 server's results.
 """
 import threading
 
@@ -63,6 +64,21 @@ def test_alone_equals_in_the_mix(geo):
         assert m.latency_us > 0
 
 
+def test_mixed_server_equals_fp64(geo):
+    """The mixed-precision server runs the mixed navigation kernels (FP32 box filters in front
+    of the FP64 decisions): its results are the FP64 server's, bit for bit."""
+    g, G = geo
+    reqs = _requests(g, [700, 5000, 30000], seed=35)
+    with Server(G, precision=FP64) as S:
+        ref = [S.wait(S.submit(*r)) for r in reqs]
+    with Server(G, precision=MIXED) as S:
+        got = [S.wait(S.submit(*r)) for r in reqs]
+    for a, b in zip(ref, got):
+        assert np.array_equal(a.p_out, b.p_out)
+        assert np.array_equal(a.nav, b.nav)
+        assert np.array_equal(a.taken, b.taken)
+
+
 def test_matches_oracle_transport(geo):
     g, G = geo
     O = OracleGeometry(g)
@@ -83,8 +99,8 @@ def test_matches_oracle_transport(geo):
 def test_admission(geo):
     g, G = geo
     (p, d, nsteps), = _requests(g, [5000], seed=33)
-    with pytest.raises(VgmiError, match="UNSUPPORTED|only VGMI_FP64|failed"):
-        Server(G, precision=MIXED)
+    with pytest.raises(VgmiError, match="unknown precision"):
+        Server(G, precision=7)
     with Server(G, precision=FP64, lane_tracks=4096, lanes=1, queue_tracks=4096) as S:
         with pytest.raises(VgmiError, match="not admitted"):
             S.submit(p, d, nsteps)  # larger than a lane

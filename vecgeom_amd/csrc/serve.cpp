@@ -1,7 +1,7 @@
 // Serving loop: continuous batching of track-transport requests onto the GPU.
 //
 // Producers submit requests (host arrays of track positions, directions and per-track step
-// counts) from any thread. A dispatcher thread packs every pending request that fits into
+// counts) from any thread. A dispatcher thread picks the next requests (pick_batch) into
 // the next free lane's pinned staging buffer and issues, on that lane's HIP stream, one
 // host-to-device copy, one locate-and-propagate launch and one device-to-host copy; a
 // completion thread unpacks a lane's results when its event has fired and marks the
@@ -108,6 +108,45 @@ void layout(Lane &L, size_t n)
   L.oLast  = take(4 * n); // device only
 }
 
+// A launch's latency is that of its slowest track, and small launches run with fewer tracks per
+// wavefront (much faster steps), so requests are not packed across sizes: the next launch starts
+// with the smallest queued request (or with the oldest one once it has waited kMaxWait, so large
+// requests are not starved); a small first request takes further small ones, smallest first, up to
+// kPackTracks tracks in all; a large one runs alone. Called with the server lock held and a
+// non-empty queue.
+constexpr int64_t kPackTracks = 4096;
+constexpr auto kMaxWait       = std::chrono::milliseconds(20);
+
+void take(vgmi_server *S, Lane &L, std::deque<Request *>::iterator it)
+{
+  Request *r = *it;
+  S->queue.erase(it);
+  S->queuedTracks -= r->n;
+  r->state = 1;
+  L.offset.push_back(L.ntracks);
+  L.batch.push_back(r);
+  L.ntracks += r->n;
+}
+
+void pick_batch(vgmi_server *S, Lane &L)
+{
+  auto oldest = S->queue.begin();
+  auto first  = oldest;
+  if (Clock::now() - (*oldest)->tSubmit < kMaxWait)
+    for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+      if ((*it)->n < (*first)->n) first = it;
+  const int64_t pack = kPackTracks < L.cap ? kPackTracks : L.cap;
+  const bool small   = (*first)->n <= pack;
+  take(S, L, first);
+  while (small) {
+    auto best = S->queue.end();
+    for (auto it = S->queue.begin(); it != S->queue.end(); ++it)
+      if (L.ntracks + (*it)->n <= pack && (best == S->queue.end() || (*it)->n < (*best)->n)) best = it;
+    if (best == S->queue.end()) break;
+    take(S, L, best);
+  }
+}
+
 void dispatch_loop(vgmi_server *S)
 {
   (void)hipSetDevice(S->device);
@@ -115,31 +154,19 @@ void dispatch_loop(vgmi_server *S)
     Lane *L = nullptr;
     {
       std::unique_lock<std::mutex> lk(S->m);
-      S->cvWork.wait(lk, [&] {
-        if (S->stop) return true;
-        if (S->queue.empty()) return false;
+      auto ready_lane = [&]() -> Lane * {
+        if (S->queue.empty()) return nullptr;
         for (auto &l : S->lanes)
-          if (!l.busy) return true;
-        return false;
-      });
+          if (!l.busy) return &l;
+        return nullptr;
+      };
+      S->cvWork.wait(lk, [&] { return S->stop || ready_lane() != nullptr; });
       if (S->stop) return;
-      for (auto &l : S->lanes)
-        if (!l.busy) {
-          L = &l;
-          break;
-        }
+      L = ready_lane();
       L->batch.clear();
       L->offset.clear();
       L->ntracks = 0;
-      while (!S->queue.empty() && L->ntracks + S->queue.front()->n <= L->cap) {
-        Request *r = S->queue.front();
-        S->queue.pop_front();
-        S->queuedTracks -= r->n;
-        r->state = 1;
-        L->offset.push_back(L->ntracks);
-        L->batch.push_back(r);
-        L->ntracks += r->n;
-      }
+      pick_batch(S, *L);
       L->busy   = true;
       L->issued = false;
     }

@@ -192,4 +192,7 @@ def summarize(records, refused, wall, small_max=1 << 16):
     if small.any():
         out["p50_ms_le64k"] = round(float(np.percentile(lat_ms[small], 50)), 3)
         out["p99_ms_le64k"] = round(float(np.percentile(lat_ms[small], 99)), 3)
+    tiny = a[:, 0] <= (1 << 14)
+    if tiny.any():
+        out["p99_ms_le16k"] = round(float(np.percentile(lat_ms[tiny], 99)), 3)
     return out
