@@ -243,7 +243,7 @@ VGMI_DEV void facet_safety_sq(const double *F, V3d const &p, bool toIn, double l
 // farther than the current minimum (or the bound) are skipped. From outside (ToIn) the BVH is
 // walked nearer child first with a per-lane stack, which finds the closest facets early; from
 // inside most boxes hold the point (distance 0, no order to exploit), so the threaded
-// depth-first walk, which reads each box once, is used.
+// depth-first walk, which reads each box once, is used after a greedy descent seeds the minimum.
 VGMI_DEV double safety_sq_threaded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
 {
   double best = kInfLength;
@@ -266,9 +266,29 @@ VGMI_DEV double safety_sq_threaded(TessellatedData const &s, V3d const &p, bool 
   return best;
 }
 
+// From inside: the facets of the leaf reached by always descending into the nearer child give a
+// first minimum (an upper bound of the result), then the threaded walk prunes against it.
+VGMI_DEV double safety_sq_seeded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
+{
+  double seed = kInfLength;
+  int j       = 0;
+  for (;;) {
+    const double *N = node(s, j);
+    const int cnt   = node_count(N);
+    if (cnt > 0) {
+      const int first = node_first(N);
+      for (int k = first; k < first + cnt; ++k) facet_safety_sq(facet(s, k), p, toIn, vmin(seed, bound), seed);
+      break;
+    }
+    const int l = j + 1, r = node_escape(node(s, l));
+    j = node_dist2(node(s, l), p) <= node_dist2(node(s, r), p) ? l : r;
+  }
+  return vmin(seed, safety_sq_threaded(s, p, toIn, vmin(seed, bound)));
+}
+
 VGMI_DEV double safety_sq(TessellatedData const &s, V3d const &p, bool toIn, double bound = kInfLength)
 {
-  if (!toIn) return safety_sq_threaded(s, p, toIn, bound);
+  if (!toIn) return safety_sq_seeded(s, p, toIn, bound);
   double best = kInfLength;
   int stack[kTessStack];
   int sp = 0, j = 0;
