@@ -211,10 +211,6 @@ VGMI_DEV double seg_dist2(V3d const &c0, V3d const &c1, V3d const &p)
   return ((c0 + (dot0 / dot1) * line) - p).Mag2();
 }
 
-// Traversal stack of the facet BVH: a tree of at most kTessMaxFacets facets (median splits, four
-// per leaf; the host builder enforces the count) has at most 22 internal levels.
-constexpr int kTessStack = 24;
-
 // Squared safety of one facet (TessellatedCluster::SafetySq, TessellatedCluster.h:498-540 with one
 // facet per cluster): the squared plane distance when the projection lies within the triangle on
 // the right side, else the smallest squared distance to its edges. Facets whose squared plane
@@ -240,10 +236,8 @@ VGMI_DEV void facet_safety_sq(const double *F, V3d const &p, bool toIn, double l
 
 // SafetySq<ToIn>: the smallest per-facet squared safety over all facets (the minimum does not
 // depend on the visiting order). `bound` is an upper bound of that minimum; subtrees whose box is
-// farther than the current minimum (or the bound) are skipped. From outside (ToIn) the BVH is
-// walked nearer child first with a per-lane stack, which finds the closest facets early; from
-// inside most boxes hold the point (distance 0, no order to exploit), so the threaded
-// depth-first walk, which reads each box once, is used after a greedy descent seeds the minimum.
+// farther than the current minimum (or the bound) are skipped. A greedy descent into the nearer
+// child seeds the minimum, then the threaded depth-first walk (each box read once) prunes against it.
 VGMI_DEV double safety_sq_threaded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
 {
   double best = kInfLength;
@@ -266,8 +260,8 @@ VGMI_DEV double safety_sq_threaded(TessellatedData const &s, V3d const &p, bool 
   return best;
 }
 
-// From inside: the facets of the leaf reached by always descending into the nearer child give a
-// first minimum (an upper bound of the result), then the threaded walk prunes against it.
+// The facets of the leaf reached by always descending into the nearer child give a first minimum
+// (an upper bound of the result), then the threaded walk prunes against it.
 VGMI_DEV double safety_sq_seeded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
 {
   double seed = kInfLength;
@@ -288,44 +282,7 @@ VGMI_DEV double safety_sq_seeded(TessellatedData const &s, V3d const &p, bool to
 
 VGMI_DEV double safety_sq(TessellatedData const &s, V3d const &p, bool toIn, double bound = kInfLength)
 {
-  if (!toIn) return safety_sq_seeded(s, p, toIn, bound);
-  double best = kInfLength;
-  int stack[kTessStack];
-  int sp = 0, j = 0;
-  if (node_dist2(node(s, 0), p) > bound) return best;
-  for (;;) {
-    const double *N = node(s, j);
-    const int cnt   = node_count(N);
-    bool descend    = false;
-    if (cnt > 0) {
-      const int first = node_first(N);
-      for (int k = first; k < first + cnt; ++k) facet_safety_sq(facet(s, k), p, toIn, vmin(best, bound), best);
-    } else {
-      const int l = j + 1, r = node_escape(node(s, l));
-      const double lim = vmin(best, bound);
-      const double dl = node_dist2(node(s, l), p), dr = node_dist2(node(s, r), p);
-      const bool vl = !(dl > lim), vr = !(dr > lim);
-      if (vl && vr) {
-        stack[sp++] = dl <= dr ? r : l;
-        j           = dl <= dr ? l : r;
-        descend     = true;
-      } else if (vl || vr) {
-        j       = vl ? l : r;
-        descend = true;
-      }
-    }
-    if (descend) continue;
-    int next = -1;
-    while (sp > 0) {
-      const int c = stack[--sp];
-      if (!(node_dist2(node(s, c), p) > vmin(best, bound))) {
-        next = c;
-        break;
-      }
-    }
-    if (next < 0) return best;
-    j = next;
-  }
+  return safety_sq_seeded(s, p, toIn, bound);
 }
 
 // An upper bound of the smallest per-facet squared safety. That minimum is at most the squared

@@ -134,10 +134,6 @@ struct TorusData {
   TubeData bounding;
 };
 
-// Largest facet count of a tessellated solid: keeps its facet BVH (median splits, four facets per
-// leaf) at most 22 levels deep, within the device traversal stack.
-constexpr int kTessMaxFacets = 1 << 23;
-
 // Hype (VecGeom/volumes/HypeStruct.h:16-83): defining parameters and the cached tangents,
 // squares, end radii and tolerance levels of CalculateCached.
 struct HypeData {

@@ -142,7 +142,6 @@ int MakeTessellated(double const *a, int n, ShapeRecord &r, std::vector<double> 
   if (n < 1) return err = "tessellated needs ntri followed by 9 doubles per triangle", -1;
   const int ntri = int(a[0]);
   if (ntri < 4 || n < 1 + 9 * ntri) return err = "tessellated needs at least 4 triangles of 9 doubles each", -1;
-  if (ntri > kTessMaxFacets) return err = "tessellated has more than 2^23 triangles", -1;
   std::vector<HostFacet> f;
   for (int i = 0; i < ntri; ++i) {
     const double *q = a + 1 + 9 * i;
