@@ -238,9 +238,10 @@ VGMI_API vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, do
                                         void *stream);
 
 /*
- * Regrouping of the fused transport loops (vgmi_nav_propagate, the serving loop): every
- * chunk_steps steps the tracks are counting-sorted on the device by the logical volume they
- * are in, so the lanes of a wavefront run the same volume's code; 0 disables it (default 10).
+ * Regrouping of the fused transport loops (vgmi_nav_propagate, the serving loop) of at least
+ * 65536 tracks: every chunk_steps steps the tracks are counting-sorted on the device by the
+ * logical volume they are in, so the lanes of a wavefront run the same volume's code; 0
+ * disables it (default 10).
  * Results per track do not depend on it. No reference counterpart (a GPU scheduling knob).
  */
 VGMI_API vgmi_status vgmi_geom_set_regroup(vgmi_geom *geom, int chunk_steps);
@@ -287,6 +288,10 @@ VGMI_API vgmi_status vgmi_server_submit(vgmi_server *server, const double *px, c
 VGMI_API vgmi_status vgmi_server_wait(vgmi_server *server, uint64_t ticket, double *latency_us);
 /* stats[0..3] = launches, tracks done, steps done, requests refused */
 VGMI_API vgmi_status vgmi_server_stats(vgmi_server *server, int64_t stats[4]);
+/* Launches whose tracks need at most 100 steps replay HIP graphs captured at creation (one per
+ * power-of-two track-count bucket from 4096 up to lane_tracks; the kernels read the actual count
+ * from device memory). stats[0] = graph launches so far, stats[1] = graphs per lane. */
+VGMI_API vgmi_status vgmi_server_graph_stats(vgmi_server *server, int64_t stats[2]);
 /* Stops the threads after the launches in flight complete; queued requests are dropped. */
 VGMI_API vgmi_status vgmi_server_destroy(vgmi_server *server);
 

@@ -109,7 +109,7 @@ def test_propagate_matches_oracle(geo, regroup, lanes, prec):
     """The fused loop, unsorted (0) and regrouped by volume every 7 / 10 steps, with automatic,
     1, 8 and 64 tracks per wavefront: same results."""
     g, G, O = geo
-    n, nsteps = 1 << 14, 50
+    n, nsteps = (1 << 16 if regroup else 1 << 14), 50  # regrouping starts at 65536 tracks
     p, d = geometry.world_points(g, n, seed=24)
     nav = O.locate(p, threads=8)
     last = np.zeros_like(nav)

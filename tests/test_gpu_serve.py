@@ -56,6 +56,8 @@ def test_alone_equals_in_the_mix(geo):
         st = S.stats()
     assert st["launches"] < len(reqs) + 2 * len(reqs), st
     assert st["tracks"] == 3 * sum(len(r[2]) for r in reqs)
+    # step counts are at most 100, so every launch replays a captured graph
+    assert st["graphs_per_lane"] >= 5 and st["graph_launches"] == st["launches"], st
     for i, m in enumerate(mixed):
         a = alone[i % len(reqs)]
         assert np.array_equal(a.p_out, m.p_out), i
@@ -79,10 +81,28 @@ def test_mixed_server_equals_fp64(geo):
         assert np.array_equal(a.taken, b.taken)
 
 
+def test_graph_and_direct_launches_agree(geo):
+    """A request whose tracks ask for more steps than the captured graphs cover is launched
+    directly; the tracks it shares with a graph-launched request give the same results."""
+    g, G = geo
+    (p, d, nsteps), = _requests(g, [3000], seed=36)
+    longer = nsteps.copy()
+    longer[0] = 150  # beyond the graphs' step count: direct launch
+    with Server(G, lane_tracks=1 << 14, lanes=2) as S:
+        a = S.wait(S.submit(p, d, nsteps))
+        st0 = S.stats()
+        b = S.wait(S.submit(p, d, longer))
+        st1 = S.stats()
+    assert st0["graph_launches"] == 1 and st1["graph_launches"] == 1 and st1["launches"] == 2
+    assert np.array_equal(a.p_out[:, 1:], b.p_out[:, 1:])
+    assert np.array_equal(a.nav[1:], b.nav[1:])
+    assert np.array_equal(a.taken[1:], b.taken[1:])
+
+
 def test_matches_oracle_transport(geo):
     g, G = geo
     O = OracleGeometry(g)
-    (p, d, nsteps), = _requests(g, [6000], seed=32)  # above the regrouping threshold
+    (p, d, nsteps), = _requests(g, [70000], seed=32)  # above the regrouping threshold (65536)
     with Server(G) as S:
         r = S.wait(S.submit(p, d, nsteps))
     nav0 = O.locate(p, threads=8)

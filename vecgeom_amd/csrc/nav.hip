@@ -97,6 +97,7 @@ struct PropArgs {
   const uint32_t *first; // number of leading perm entries to skip (tracks outside the world)
   int accumulate;        // add to taken instead of storing
   int lanes;             // tracks per wavefront (1..64); lanes >= this one idle
+  const uint32_t *dn;    // device-side track count (<= n, the capacity), or null: n tracks
 };
 
 // The transport loop of one track stays in registers for all its steps; only the
@@ -108,11 +109,12 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoop
   const unsigned lane = threadIdx.x & 63u;
   if (lane >= unsigned(a.lanes)) return;
   size_t i = ((size_t(blockIdx.x) * kNavBlock + threadIdx.x) >> 6) * unsigned(a.lanes) + lane;
+  const size_t n = a.dn ? size_t(*a.dn) : a.n;
   if (a.perm) {
     i += *a.first;
-    if (i >= a.n) return;
+    if (i >= n) return;
     i = a.perm[i];
-  } else if (i >= a.n) {
+  } else if (i >= n) {
     return;
   }
   V3 p(a.px[i], a.py[i], a.pz[i]);
@@ -161,9 +163,10 @@ __global__ void k_fill_u32(uint32_t *a, size_t n, uint32_t v)
   for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) a[i] = v;
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_sort_hist(GeomView g, SortKeys key, size_t n, uint32_t *hist,
-                                                          int nbins)
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(GeomView g, SortKeys key, size_t cap, const uint32_t *dn,
+                                                          uint32_t *hist, int nbins)
 {
+  const size_t n = dn ? size_t(*dn) : cap;
   extern __shared__ uint32_t sh[];
   const bool local = nbins <= kLocalBins;
   if (local)
@@ -208,9 +211,10 @@ __global__ __launch_bounds__(1024) void k_sort_scan(uint32_t *hist, int nbins)
 }
 
 // perm[cursor[key]++] = i, with block-local ranks so that each block reserves one range per key.
-__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, SortKeys key, size_t n, uint32_t *cursor,
-                                                             uint32_t *perm, int nbins)
+__global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, SortKeys key, size_t cap, const uint32_t *dn,
+                                                             uint32_t *cursor, uint32_t *perm, int nbins)
 {
+  const size_t n = dn ? size_t(*dn) : cap;
   extern __shared__ uint32_t sh[];
   const size_t i   = size_t(blockIdx.x) * kSortBlock + threadIdx.x;
   const uint32_t k = i < n ? key(g, i) : 0u;
@@ -235,11 +239,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scatter(GeomView g, SortKey
 template <unsigned TM>
 __global__ __launch_bounds__(kNavBlock) void k_locate_init(GeomView g, const double *px, const double *py,
                                                            const double *pz, double *opx, double *opy, double *opz,
-                                                           uint32_t *nav, uint32_t *last, int32_t *taken, size_t n)
+                                                           uint32_t *nav, uint32_t *last, int32_t *taken, size_t cap,
+                                                           const uint32_t *dn)
 {
   VGMI_NAV_STACK;
   const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
-  if (i >= n) return;
+  if (i >= (dn ? size_t(*dn) : cap)) return;
   const V3 p(px[i], py[i], pz[i]);
   opx[i]  = p.x;
   opy[i]  = p.y;
@@ -258,8 +263,10 @@ using namespace vgmi::nav;
 
 static unsigned nav_grid(size_t n) { return unsigned((n + kNavBlock - 1) / kNavBlock); }
 
-// Below this many tracks a transport loop is not regrouped (the sort would not pay).
-constexpr size_t kRegroupMinTracks = 4096;
+// Below this many tracks a transport loop is not regrouped: the GPU is not full, and the sort
+// plus the per-chunk synchronisation cost more than the coherence gains (4096 tracks x 100 steps:
+// 2.6 ms unsorted, 4.5 ms sorted every 10 steps; the sort pays from 65536 tracks on).
+constexpr size_t kRegroupMinTracks = 65536;
 
 // Launches KERNEL<types | precision>: kSimpleTypes when the geometry has only simple solids,
 // else kAllTypes; kMixedFlag for VGMI_MIXED.
@@ -328,7 +335,7 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
   if (!work || chunk <= 0 || n < kRegroupMinTracks) {
     const int lanes = lanes_per_wave(geom, n);
     PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, t.nsteps, t.track_steps,
-               nullptr, nullptr, 0, lanes};
+               nullptr, nullptr, 0, lanes, t.dn};
     VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, lane_threads(n, lanes), st, geom->view, a);
     return launched("transport loop");
   }
@@ -344,11 +351,12 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
   const SortKeys key{t.nav, t.track_steps, t.taken};
   for (int done = 0; done < t.nsteps; done += chunk) {
     hipLaunchKernelGGL(k_fill_u32, dim3(unsigned((nbins + 255) / 256)), dim3(256), 0, st, cursor, size_t(nbins), 0u);
-    hipLaunchKernelGGL(k_sort_hist, dim3(hgrid), dim3(kSortBlock), lds, st, geom->view, key, n, cursor, nbins);
+    hipLaunchKernelGGL(k_sort_hist, dim3(hgrid), dim3(kSortBlock), lds, st, geom->view, key, n, t.dn, cursor, nbins);
     hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, cursor, nbins);
-    hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, key, n, cursor, perm, nbins);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(sgrid), dim3(kSortBlock), lds, st, geom->view, key, n, t.dn, cursor, perm,
+                       nbins);
     PropArgs a{t.px, t.py, t.pz, t.dx, t.dy, t.dz, t.limit, t.nav, t.last, t.taken, n, std::min(chunk, t.nsteps - done),
-               t.track_steps, perm, cursor, 1, lanes};
+               t.track_steps, perm, cursor, 1, lanes, t.dn};
     VGMI_NAV_LAUNCH(k_propagate, geom, t.precision, lane_threads(n, lanes), st, geom->view, a);
   }
   return launched("transport loop");
@@ -356,10 +364,11 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &t, uint
 
 vgmi_status launch_locate_init(const vgmi_geom *geom, int precision, const double *px, const double *py,
                                const double *pz, double *opx, double *opy, double *opz, uint32_t *nav,
-                               uint32_t *last, int32_t *taken, size_t n, hipStream_t st)
+                               uint32_t *last, int32_t *taken, size_t n, hipStream_t st, const uint32_t *dn)
 {
   if (n == 0) return VGMI_OK;
-  VGMI_NAV_LAUNCH(k_locate_init, geom, precision, n, st, geom->view, px, py, pz, opx, opy, opz, nav, last, taken, n);
+  VGMI_NAV_LAUNCH(k_locate_init, geom, precision, n, st, geom->view, px, py, pz, opx, opy, opz, nav, last, taken, n,
+                  dn);
   return launched("serving locate");
 }
 
@@ -428,7 +437,7 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
     return set_error(VGMI_ERR_INVALID, "vgmi_nav_propagate: null pointer");
   if (n > 0xffffffffull) return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_propagate: more than 2^32 tracks");
   hipStream_t st = (hipStream_t)stream;
-  TransportArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, nullptr, n, nsteps, precision};
+  TransportArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, nullptr, n, nsteps, precision, nullptr};
   uint32_t *work = nullptr;
   if (const size_t words = transport_work_words(geom, n)) {
     if (hipMallocAsync(reinterpret_cast<void **>(&work), words * sizeof(uint32_t), st) != hipSuccess)

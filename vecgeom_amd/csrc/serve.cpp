@@ -57,6 +57,8 @@ struct Lane {
   int64_t ntracks    = 0;
   std::vector<Request *> batch;
   std::vector<int64_t> offset;
+  // HIP graphs of the locate + transport launches for track-count buckets (capacity, graph)
+  std::vector<std::pair<int64_t, hipGraphExec_t>> graphs;
   // SoA sections of the staging buffers for the current batch (same layout on host and
   // device): inputs packed contiguously from offset 0, outputs from outBegin.
   size_t oPx, oPy, oPz, oDx, oDy, oDz, oSteps, inEnd, oOpx, oOpy, oOpz, oNav, oTaken, outBegin, outEnd, oLast;
@@ -80,17 +82,18 @@ struct vgmi_server {
   std::thread dispatcher, completer;
   int device = 0;
   // statistics
-  int64_t launches = 0, tracksDone = 0, stepsDone = 0, rejected = 0;
+  int64_t launches = 0, tracksDone = 0, stepsDone = 0, rejected = 0, graphLaunches = 0;
 };
 
 namespace {
 
-size_t lane_bytes(int64_t cap) { return size_t(cap) * kTrackBytes + 16 * 256; }
+size_t lane_bytes(int64_t cap) { return size_t(cap) * kTrackBytes + 17 * 256; }
 
-// Section offsets for a batch of n tracks, each section 256-byte aligned.
+// Section offsets for a batch of n tracks, each section 256-byte aligned, after a 256-byte header
+// whose first word is the track count (read by the kernels of a graph launch).
 void layout(Lane &L, size_t n)
 {
-  size_t o  = 0;
+  size_t o  = 256;
   auto take = [&](size_t bytes) {
     size_t at = o;
     o += (bytes + 255) / 256 * 256;
@@ -115,6 +118,9 @@ void layout(Lane &L, size_t n)
 // kPackTracks tracks in all; a large one runs alone. Called with the server lock held and a
 // non-empty queue.
 constexpr int64_t kPackTracks = 4096;
+// Launches whose tracks need at most kGraphSteps steps replay a captured graph sized for the
+// smallest power-of-two bucket (from kPackTracks up to the lane capacity) that holds them.
+constexpr int kGraphSteps = 100;
 constexpr auto kMaxWait       = std::chrono::milliseconds(20);
 
 void take(vgmi_server *S, Lane &L, std::deque<Request *>::iterator it)
@@ -170,10 +176,22 @@ void dispatch_loop(vgmi_server *S)
       L->busy   = true;
       L->issued = false;
     }
-    layout(*L, size_t(L->ntracks));
-    // pack inputs (outside the lock)
-    char *h     = L->hstage;
+    // a graph launch when a bucket holds the batch and its step counts are within kGraphSteps
     L->maxSteps = 0;
+    for (Request const *r : L->batch)
+      for (int64_t i = 0; i < r->n; ++i) L->maxSteps = r->nsteps[i] > L->maxSteps ? r->nsteps[i] : L->maxSteps;
+    hipGraphExec_t graph = nullptr;
+    int64_t cap          = L->ntracks;
+    if (L->maxSteps <= kGraphSteps)
+      for (auto const &g : L->graphs)
+        if (g.first >= L->ntracks) {
+          graph = g.second, cap = g.first;
+          break;
+        }
+    layout(*L, size_t(cap));
+    // pack inputs (outside the lock)
+    char *h = L->hstage;
+    *reinterpret_cast<uint32_t *>(h) = uint32_t(L->ntracks);
     for (size_t k = 0; k < L->batch.size(); ++k) {
       Request const *r = L->batch[k];
       const size_t o = size_t(L->offset[k]);
@@ -184,7 +202,6 @@ void dispatch_loop(vgmi_server *S)
       std::memcpy(h + L->oDy + 8 * o, r->dy, 8 * r->n);
       std::memcpy(h + L->oDz + 8 * o, r->dz, 8 * r->n);
       std::memcpy(h + L->oSteps + 4 * o, r->nsteps, 4 * r->n);
-      for (int64_t i = 0; i < r->n; ++i) L->maxSteps = r->nsteps[i] > L->maxSteps ? r->nsteps[i] : L->maxSteps;
     }
     char *d        = L->dstage;
     const size_t n = size_t(L->ntracks);
@@ -193,12 +210,16 @@ void dispatch_loop(vgmi_server *S)
     auto I         = [&](size_t off) { return reinterpret_cast<int32_t *>(d + off); };
     TransportArgs a{D(L->oOpx), D(L->oOpy), D(L->oOpz), D(L->oDx),    D(L->oDy),       D(L->oDz), S->limit,
                     U(L->oNav), U(L->oLast), I(L->oTaken), I(L->oSteps), n,               L->maxSteps,
-                    S->precision};
+                    S->precision, nullptr};
     hipError_t e = hipMemcpyAsync(d, h, L->inEnd, hipMemcpyHostToDevice, L->stream);
-    int st       = e == hipSuccess ? launch_locate_init(S->geom, S->precision, D(L->oPx), D(L->oPy), D(L->oPz), a.px, a.py, a.pz,
-                                                        a.nav, a.last, a.taken, n, L->stream)
-                                   : VGMI_ERR_DEVICE;
-    if (st == VGMI_OK) st = launch_transport(S->geom, a, L->dwork, L->stream);
+    int st       = e == hipSuccess ? VGMI_OK : VGMI_ERR_DEVICE;
+    if (st == VGMI_OK && graph) {
+      if (hipGraphLaunch(graph, L->stream) != hipSuccess) st = VGMI_ERR_DEVICE;
+    } else if (st == VGMI_OK) {
+      st = launch_locate_init(S->geom, S->precision, D(L->oPx), D(L->oPy), D(L->oPz), a.px, a.py, a.pz, a.nav, a.last,
+                              a.taken, n, L->stream);
+      if (st == VGMI_OK) st = launch_transport(S->geom, a, L->dwork, L->stream);
+    }
     if (st == VGMI_OK)
       e = hipMemcpyAsync(h + L->outBegin, d + L->outBegin, L->outEnd - L->outBegin, hipMemcpyDeviceToHost, L->stream);
     if (st == VGMI_OK && e == hipSuccess) e = hipEventRecord(L->done, L->stream);
@@ -209,6 +230,7 @@ void dispatch_loop(vgmi_server *S)
       L->status = st;
       L->issued = true;
       S->launches++;
+      if (graph) S->graphLaunches++;
     }
     S->cvWork.notify_all(); // wakes the completion thread
   }
@@ -270,6 +292,37 @@ void complete_loop(vgmi_server *S)
   }
 }
 
+// Captures, on the lane's stream, the locate + transport launches of a batch of `cap` tracks laid
+// out by layout(cap), reading the actual track count from the staging header, for each bucket.
+hipError_t capture_graphs(vgmi_server *S, Lane &L)
+{
+  for (int64_t cap = kPackTracks; ; cap *= 2) {
+    if (cap > L.cap) cap = L.cap;
+    layout(L, size_t(cap));
+    char *d        = L.dstage;
+    auto D         = [&](size_t off) { return reinterpret_cast<double *>(d + off); };
+    auto U         = [&](size_t off) { return reinterpret_cast<uint32_t *>(d + off); };
+    auto I         = [&](size_t off) { return reinterpret_cast<int32_t *>(d + off); };
+    const uint32_t *dn = reinterpret_cast<const uint32_t *>(d);
+    TransportArgs a{D(L.oOpx), D(L.oOpy), D(L.oOpz), D(L.oDx), D(L.oDy), D(L.oDz), S->limit, U(L.oNav), U(L.oLast),
+                    I(L.oTaken), I(L.oSteps), size_t(cap), kGraphSteps, S->precision, dn};
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    int st = launch_locate_init(S->geom, S->precision, D(L.oPx), D(L.oPy), D(L.oPz), a.px, a.py, a.pz, a.nav, a.last,
+                                a.taken, size_t(cap), L.stream, dn);
+    if (st == VGMI_OK) st = launch_transport(S->geom, a, L.dwork, L.stream);
+    e = hipStreamEndCapture(L.stream, &g);
+    if (e == hipSuccess && st != VGMI_OK) e = hipErrorLaunchFailure;
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return e;
+    L.graphs.emplace_back(cap, x);
+    if (cap == L.cap) return hipSuccess;
+  }
+}
+
 } // namespace
 
 extern "C" {
@@ -306,6 +359,7 @@ vgmi_status vgmi_server_create(const vgmi_geom *geom, int precision, double step
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&L.dstage), bytes);
     const size_t words = transport_work_words(geom, size_t(lane_tracks));
     if (e == hipSuccess && words) e = hipMalloc(reinterpret_cast<void **>(&L.dwork), words * sizeof(uint32_t));
+    if (e == hipSuccess) e = capture_graphs(S, L);
     if (e != hipSuccess) {
       vgmi_server_destroy(S);
       return set_error(VGMI_ERR_DEVICE, "vgmi_server_create: %s", hipGetErrorString(e));
@@ -373,6 +427,15 @@ vgmi_status vgmi_server_stats(vgmi_server *S, int64_t stats[4])
   return VGMI_OK;
 }
 
+vgmi_status vgmi_server_graph_stats(vgmi_server *S, int64_t stats[2])
+{
+  if (!S || !stats) return set_error(VGMI_ERR_INVALID, "vgmi_server_graph_stats: null argument");
+  std::lock_guard<std::mutex> lk(S->m);
+  stats[0] = S->graphLaunches;
+  stats[1] = S->lanes.empty() ? 0 : int64_t(S->lanes[0].graphs.size());
+  return VGMI_OK;
+}
+
 vgmi_status vgmi_server_destroy(vgmi_server *S)
 {
   if (!S) return VGMI_OK;
@@ -385,6 +448,7 @@ vgmi_status vgmi_server_destroy(vgmi_server *S)
   if (S->completer.joinable()) S->completer.join();
   for (auto &L : S->lanes) {
     if (L.stream) (void)hipStreamSynchronize(L.stream);
+    for (auto &g : L.graphs) (void)hipGraphExecDestroy(g.second);
     if (L.dstage) (void)hipFree(L.dstage);
     if (L.dwork) (void)hipFree(L.dwork);
     if (L.hstage) (void)hipHostFree(L.hstage);

@@ -23,6 +23,7 @@ struct TransportArgs {
   size_t n;
   int nsteps;                  // the uniform step count, or the largest of track_steps
   int precision;               // VGMI_FP64 or VGMI_MIXED
+  const uint32_t *dn;          // device-side track count (<= n, then a capacity), or null: n tracks
 };
 // Words of device workspace launch_transport needs to regroup n tracks.
 size_t transport_work_words(const vgmi_geom *geom, size_t n);
@@ -31,7 +32,8 @@ vgmi_status launch_transport(const vgmi_geom *geom, TransportArgs const &a, uint
 // Locates every point (p_in -> p_out copy, nav), clears last and taken.
 vgmi_status launch_locate_init(const vgmi_geom *geom, int precision, const double *px, const double *py,
                                const double *pz, double *opx, double *opy, double *opz, uint32_t *nav,
-                               uint32_t *last, int32_t *taken, size_t n, hipStream_t st);
+                               uint32_t *last, int32_t *taken, size_t n, hipStream_t st,
+                               const uint32_t *dn = nullptr);
 
 // Records the message for vgmi_last_error() and returns `code`.
 int set_error(int code, const char *fmt, ...);

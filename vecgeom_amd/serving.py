@@ -31,6 +31,7 @@ def _declare():
         "vgmi_server_submit": [vp] + [vp] * 7 + [i64] + [vp] * 5 + [ctypes.POINTER(ctypes.c_uint64)],
         "vgmi_server_wait": [vp, ctypes.c_uint64, ctypes.POINTER(d)],
         "vgmi_server_stats": [vp, vp],
+        "vgmi_server_graph_stats": [vp, vp],
         "vgmi_server_destroy": [vp],
     }
     for name, args in sigs.items():
@@ -97,7 +98,10 @@ class Server:
     def stats(self):
         s = np.zeros(4, dtype=np.int64)
         check(lib().vgmi_server_stats(self._h, _ptr(s)), "vgmi_server_stats")
-        return {"launches": int(s[0]), "tracks": int(s[1]), "steps": int(s[2]), "refused": int(s[3])}
+        g = np.zeros(2, dtype=np.int64)
+        check(lib().vgmi_server_graph_stats(self._h, _ptr(g)), "vgmi_server_graph_stats")
+        return {"launches": int(s[0]), "tracks": int(s[1]), "steps": int(s[2]), "refused": int(s[3]),
+                "graph_launches": int(g[0]), "graphs_per_lane": int(g[1])}
 
     def close(self):
         if getattr(self, "_h", None) is not None:
