@@ -93,6 +93,35 @@ def test_step_and_safety(geo, prec):
 
 
 @PRECISIONS
+def test_axis_aligned_directions_and_outside_points(geo, prec):
+    """Directions with exactly zero components (the NonZero reciprocal paths of the slab tests and
+    the shapes) and points outside the world (state 0), step by step against the oracle."""
+    g, G, O = geo
+    n = 6 * 2048
+    p, _ = geometry.world_points(g, n, seed=26)
+    axes = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64)
+    d = np.ascontiguousarray(np.repeat(axes, n // 6, axis=0).T)
+    p[:, :64] *= 1.5  # some points outside the world box
+    p = np.ascontiguousarray(p)
+    nav = O.locate(p, threads=8)
+    got = G.LocateGlobalPoint(SOA3D.from_array(p), precision=prec).cpu().numpy().astype(np.uint32)
+    assert np.array_equal(got, nav)
+    assert (nav == 0).sum() > 0
+    last = np.zeros_like(nav)
+    D = SOA3D.from_array(d)
+    pp, nv, ls = p.copy(), nav.copy(), last.copy()
+    for s in range(15):
+        step, nav_o, last_o, bnd = G.ComputeStep(SOA3D.from_array(pp), D, 10.0, _dev_u32(nv), _dev_u32(ls),
+                                                 relocate=True, precision=prec)
+        rs, rn, rl, rb = O.step(pp, d, 10.0, nv, ls, relocate=True, threads=8)
+        assert np.array_equal(nav_o.cpu().numpy().astype(np.uint32), rn), s
+        assert np.array_equal(last_o.cpu().numpy().astype(np.uint32), rl), s
+        assert np.array_equal(step.cpu().numpy(), rs), s
+        pp = pp + rs * d
+        nv, ls = rn, rl
+
+
+@PRECISIONS
 def test_safety(geo, prec):
     g, G, O = geo
     p, _ = geometry.world_points(g, 1 << 15, seed=23)

@@ -159,6 +159,30 @@ using F3 = Vec3<float>;
 constexpr float kSlabRel = 1e-5f, kSlabAbs = 1e-5f;
 
 VGMI_DEV F3 to_f3(V3 const &v) { return F3(float(v.x), float(v.y), float(v.z)); }
+
+// Wave-uniform access: when every active lane reads the same records, they are read through
+// the scalar cache (constant address space, uniform address), which takes no vector-memory
+// address cycles; the loop is then the same for all lanes.
+VGMI_DEV bool wave_uniform(uint32_t x) { return __ballot(x != __builtin_amdgcn_readfirstlane(x)) == 0; }
+template <class T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+// A slot read through a uniform constant-space pointer (field by field: the implicit copy
+// constructor does not take the constant address space).
+VGMI_DEV NavSlot32 load_slot(ConstPtr<NavSlot32> p, int i)
+{
+  NavSlot32 r;
+  for (int k = 0; k < 3; ++k) r.lo[k] = p[i].lo[k], r.hi[k] = p[i].hi[k];
+  r.pvol  = p[i].pvol;
+  r.shape = p[i].shape;
+  return r;
+}
+template <class T>
+VGMI_DEV ConstPtr<T> uniform_ptr(const T *p)
+{
+  const uint64_t v  = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v)), hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+  return reinterpret_cast<ConstPtr<T>>((uint64_t(hi) << 32) | lo);
+}
 VGMI_DEV float rcp32(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // FP32 slab test against a widened box: true unless the FP64 test of the unwidened box
@@ -324,15 +348,28 @@ VGMI_DEV void check_daughters_mixed(GeomView const &g, NavVol const &M, V3 const
         const int cnt = nc - base < 32 ? nc - base : 32;
         const NavSlot32 *ls = slots + N.offset + base;
         uint32_t pass = 0;
-        NavSlot32 cur = ls[0];
-        for (int i = 0; i < cnt; ++i) {
-          const NavSlot32 nxt = ls[i + 1 < cnt ? i + 1 : i];
-          float t;
-          if (slab32_pass(cur, pf, invf, lim, t))
-            pass |= 1u << i;
-          else
-            bnd = vmin(bnd, box_safety32(cur, pf));
-          cur = nxt;
+        if (wave_uniform(uint32_t(M.slotBase + N.offset + base)) && wave_uniform(uint32_t(cnt))) {
+          ConstPtr<NavSlot32> us = uniform_ptr(ls);
+          const int ucnt         = __builtin_amdgcn_readfirstlane(cnt);
+          for (int i = 0; i < ucnt; ++i) {
+            const NavSlot32 box = load_slot(us, i);
+            float t;
+            if (slab32_pass(box, pf, invf, lim, t))
+              pass |= 1u << i;
+            else
+              bnd = vmin(bnd, box_safety32(box, pf));
+          }
+        } else {
+          NavSlot32 cur = ls[0];
+          for (int i = 0; i < cnt; ++i) {
+            const NavSlot32 nxt = ls[i + 1 < cnt ? i + 1 : i];
+            float t;
+            if (slab32_pass(cur, pf, invf, lim, t))
+              pass |= 1u << i;
+            else
+              bnd = vmin(bnd, box_safety32(cur, pf));
+            cur = nxt;
+          }
         }
         while (pass) {
           const int i = __builtin_ctz(pass);
@@ -402,11 +439,20 @@ VGMI_DEV bool level_locate_mixed(GeomView const &g, NavVol const &M, int excl, V
         const int cnt = nc - base < 32 ? nc - base : 32;
         const NavSlot32 *ls = slots + N.offset + base;
         uint32_t in = 0;
-        NavSlot32 cur = ls[0];
-        for (int i = 0; i < cnt; ++i) {
-          const NavSlot32 nxt = ls[i + 1 < cnt ? i + 1 : i];
-          if (box_contains32(cur, pf)) in |= 1u << i;
-          cur = nxt;
+        if (wave_uniform(uint32_t(M.slotBase + N.offset + base)) && wave_uniform(uint32_t(cnt))) {
+          ConstPtr<NavSlot32> us = uniform_ptr(ls);
+          const int ucnt         = __builtin_amdgcn_readfirstlane(cnt);
+          for (int i = 0; i < ucnt; ++i) {
+            const NavSlot32 box = load_slot(us, i);
+            if (box_contains32(box, pf)) in |= 1u << i;
+          }
+        } else {
+          NavSlot32 cur = ls[0];
+          for (int i = 0; i < cnt; ++i) {
+            const NavSlot32 nxt = ls[i + 1 < cnt ? i + 1 : i];
+            if (box_contains32(cur, pf)) in |= 1u << i;
+            cur = nxt;
+          }
         }
         while (in) {
           const int i = __builtin_ctz(in);
