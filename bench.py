@@ -216,6 +216,32 @@ def run_box_tube(args):
                             {"box": "Box", "tube": "Tube"}[kind], METHODS.index(m))),
                         "kernel": "k_query_fp64 (%s)" % dom["kernel"], "bytes_per_launch": BYTES[m] * n},
            "kernels": per_kernel}
+    # mixed precision on the same shapes and points: throughput, and the error of every method
+    # against the FP64 kernels (which reproduce the oracle bit for bit, tests/test_gpu_shapes.py)
+    from vecgeom_amd.shapes import MIXED
+    mixed_us, errors = 0.0, {}
+    for kind, params, s, P, D, outs, _, _ in shapes:
+        errors[kind] = {}
+        for m in METHODS:
+            om = torch.empty_like(outs[m])
+            f = (lambda s=s, m=m, P=P, D=D, o=om: getattr(s, m)(P, D, out=o, precision=MIXED)) \
+                if m.startswith("Distance") else (lambda s=s, m=m, P=P, o=om: getattr(s, m)(P, out=o, precision=MIXED))
+            for _ in range(args.warmup):
+                f()
+            mixed_us += event_time_us(f, max(10, args.steps))
+            ref = outs[m]
+            if m in ("Contains", "Inside"):
+                errors[kind][m] = {"agreement": float((om == ref).double().mean())}
+                continue
+            r, g = ref.double(), om.double()
+            fin = (r < 1e300) & (g < 1e300) & (r >= 0) & (g >= 0)
+            err = (g[fin] - r[fin]).abs()
+            both = ((r >= 1e300) == (g >= 1e300)) & ((r < 0) == (g < 0))
+            errors[kind][m] = {"max_abs_mm": float(err.max()) if err.numel() else 0.0,
+                               "p99_abs_mm": float(torch.quantile(err[:1 << 20], 0.99)) if err.numel() else 0.0,
+                               "inf_and_sign_agreement": float(both.double().mean())}
+    res["mixed"] = {"queries_per_s": round(queries / (mixed_us * 1e-6), 1),
+                    "over_fp64": round(total_us / mixed_us, 3), "errors_vs_fp64": errors}
     if not args.no_cpu:
         from oracle.pyoracle import OracleShape
         sample = 1 << 17
