@@ -68,3 +68,44 @@ def test_batch_graph_replay():
         torch.cuda.synchronize()
         for m in METHODS:
             assert torch.equal(outs[m], eager[m]), (rep, m)
+
+
+def test_batch_tessellated_lds_and_pool_walks():
+    """Tessellated solids whose nodes and planes fit the launch's LDS (the 1024- and 12-facet
+    solids) next to one that does not (2304 facets, walked in the pool), over tile sizes that
+    make one workgroup take several consecutive tiles of one solid; every method equals the
+    oracle bit for bit, and a partial output set equals the full one."""
+    from tests.golden.extract_kats import trd_like_triangles
+    shapes = [("tessellated", workloads.convex_hull_sphere(20.0)),
+              ("box", [10.0, 20.0, 30.0]),
+              ("tessellated", workloads.convex_hull_sphere(20.0, nphi=48, nbands=25)),
+              ("tessellated", trd_like_triangles(10, 30, 20, 40, 40))]
+    counts = [1500, 200, 700, 900]
+    gpu = [make_shape(k, p) for k, p in shapes]
+    orc = [OracleShape(k, p) for k, p in shapes]
+    exts = [o.extent() for o in orc]
+    p, d = workloads.mix_inputs(exts, counts, [lambda x, o=o: o.query("Contains", x) for o in orc],
+                                [lambda x, y, o=o: o.query("DistanceToOut", x, y) for o in orc])
+    P, D = SOA3D.from_array(p), SOA3D.from_array(d)
+    refs, off = [], 0
+    for o, n in zip(orc, counts):
+        ps, dsl = np.ascontiguousarray(p[:, off:off + n]), np.ascontiguousarray(d[:, off:off + n])
+        refs.append({m: o.query(m, ps, dsl if m.startswith("Distance") else None) for m in METHODS})
+        off += n
+    b = ShapeBatch(gpu, counts)
+    for tuning in [(64, 64, 1), (128, 128, 2), (256, 256, 4)]:
+        print("tuning", tuning, flush=True)
+        b.set_tuning(*tuning)
+        out = b.query_all(P, D)
+        off = 0
+        for (kind, _), ref, n in zip(shapes, refs, counts):
+            for m in METHODS:
+                assert np.array_equal(ref[m], out[m][off:off + n].cpu().numpy()), (kind, m, tuning)
+            off += n
+        part = b.alloc_outputs()
+        for m in ("Contains", "SafetyToOut", "DistanceToIn"):
+            part[m] = None
+        b.query_all(P, D, part)
+        for m in METHODS:
+            if part[m] is not None:
+                assert torch.equal(part[m], out[m]), (m, tuning)

@@ -50,24 +50,25 @@ VGMI_DEV void run_tile(Data const &s, int64_t b, int64_t e, BatchOut const &o, i
   }
 }
 
-// The tessellated solid shares one test-ray walk and one safety walk across its six methods.
-template <>
-VGMI_DEV void run_tile<TessellatedData>(TessellatedData const &s, int64_t b, int64_t e, BatchOut const &o, int nthreads)
+// The tessellated solid shares one test-ray walk and one safety walk across its six methods;
+// the walks read nodes, planes and candidate boxes through `w` (the pool or the LDS image).
+template <class W>
+VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int64_t e, BatchOut const &o,
+                            int nthreads)
 {
   for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
     const Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
                          __builtin_nontemporal_load(o.pz + i));
     if (!(o.contains && o.inside && o.sin && o.sout && o.din && o.dout)) {
       // a partial output set: the methods one by one
-      using S = ShapeOps<TessellatedData>;
-      if (o.contains) __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), o.contains + i);
-      if (o.inside) __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), o.inside + i);
-      if (o.sin) __builtin_nontemporal_store(S::template SafeIn<double>(s, p), o.sin + i);
-      if (o.sout) __builtin_nontemporal_store(S::template SafeOut<double>(s, p), o.sout + i);
+      if (o.contains) __builtin_nontemporal_store(uint8_t(tess::contains64(s, w, p)), o.contains + i);
+      if (o.inside) __builtin_nontemporal_store(int32_t(tess::inside64(s, w, p)), o.inside + i);
+      if (o.sin) __builtin_nontemporal_store(tess::safety_in64(s, w, p), o.sin + i);
+      if (o.sout) __builtin_nontemporal_store(tess::safety_out64(s, w, p), o.sout + i);
       if (o.din || o.dout) {
         const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
-        if (o.din) __builtin_nontemporal_store(S::template DistIn<double>(s, p, d), o.din + i);
-        if (o.dout) __builtin_nontemporal_store(S::template DistOut<double>(s, p, d), o.dout + i);
+        if (o.din) __builtin_nontemporal_store(tess::dist_in64(s, w, p, d), o.din + i);
+        if (o.dout) __builtin_nontemporal_store(tess::dist_out64(s, w, p, d), o.dout + i);
       }
       continue;
     }
@@ -76,7 +77,7 @@ VGMI_DEV void run_tile<TessellatedData>(TessellatedData const &s, int64_t b, int
     bool c;
     int in;
     double si, so, di, dou;
-    tess::query_all(s, p, d, c, in, si, so, di, dou);
+    tess::query_all(s, w, p, d, c, in, si, so, di, dou);
     __builtin_nontemporal_store(uint8_t(c), o.contains + i);
     __builtin_nontemporal_store(int32_t(in), o.inside + i);
     __builtin_nontemporal_store(si, o.sin + i);
@@ -86,13 +87,40 @@ VGMI_DEV void run_tile<TessellatedData>(TessellatedData const &s, int64_t b, int
   }
 }
 
+// A tessellated tile. When the solid's image (nodes, planes, candidate boxes) fits in the
+// launch's LDS (ldsDoubles), the workgroup copies it there once per solid (consecutive tiles of
+// one solid reuse the copy; `staged` names the solid held) and the walks read it from LDS. The
+// results are the pool walk's bit for bit.
+template <int BLOCK>
+VGMI_DEV void tess_tile(TessellatedData const &s, int si, Tile const &tile, BatchOut const &o, double *lds,
+                        int ldsDoubles, int *staged)
+{
+  const int n = s.imageDoubles;
+  if (n == 0 || n > ldsDoubles) {
+    run_tile_tess(s, tess::pool_walk(s), tile.begin, tile.end, o, BLOCK);
+    return;
+  }
+  if (*staged != si) { // block-uniform: written after the barrier that ends the copy
+    const double *src = s.base + s.imageOff;
+    const double2 *s2 = reinterpret_cast<const double2 *>(src);
+    double2 *d2       = reinterpret_cast<double2 *>(lds);
+    for (int i = threadIdx.x; i < n / 2; i += BLOCK) d2[i] = s2[i];
+    if ((n & 1) && threadIdx.x == 0) lds[n - 1] = src[n - 1];
+    __syncthreads();
+    if (threadIdx.x == 0) *staged = si;
+  }
+  run_tile_tess(s, tess::lds_walk(s, lds), tile.begin, tile.end, o, BLOCK);
+}
+
 __global__ void k_batch_reset(uint32_t *counter) { *counter = 0u; }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__ recs, const Tile *__restrict__ tiles,
-                                                 int ntiles, uint32_t *counter, BatchOut o)
+                                                 int ntiles, uint32_t *counter, BatchOut o, int ldsDoubles)
 {
-  __shared__ int s_tile;
+  extern __shared__ __align__(16) double lds[];
+  __shared__ int s_tile, s_staged;
+  if (threadIdx.x == 0) s_staged = -1;
   while (true) {
     if (threadIdx.x == 0) s_tile = int(atomicAdd(counter, 1u));
     __syncthreads();
@@ -102,12 +130,16 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__
     const Tile tile        = tiles[t];
     const int si           = __builtin_amdgcn_readfirstlane(tile.shape);
     ShapeRecord const &rec = recs[si];
+    if (rec.type == kTessellated) {
+      tess_tile<BLOCK>(rec.u.tess, si, tile, o, lds, ldsDoubles, &s_staged);
+      continue;
+    }
     switch (rec.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
     run_tile<T>(rec.u.M, tile.begin, tile.end, o, BLOCK);                                                           \
     break;
-      VGMI_FOR_EACH_SHAPE(X)
+      VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)
 #undef X
     default:
       break;
@@ -150,9 +182,18 @@ struct vgmi_batch {
   Tile *d_tiles = nullptr;
   uint32_t *d_counter = nullptr;
   int ntiles = 0;
+  int ldsBytes = 0; // dynamic LDS per workgroup: the largest tessellated solid that fits kTessLdsMax
   hipGraph_t graph = nullptr;
   hipGraphExec_t graphExec = nullptr;
 };
+
+// Dynamic LDS a workgroup may use for a tessellated solid's image: at most what keeps two
+// workgroups on a CU (160 KiB of LDS per CU, less the kernel's 16 bytes of static LDS and the
+// allocation granule); a solid whose image is larger is walked in the pool.
+constexpr int kLdsPerCU   = 160 * 1024;
+constexpr int kTessLdsMax = kLdsPerCU / 2 - 1024;
+
+static int tess_lds_bytes(TessellatedData const &t) { return int(sizeof(double)) * t.imageDoubles; }
 
 // Default points per tile: at least four tiles per resident block so the atomic queue can
 // balance the expensive shapes, within [256, 4096] and a multiple of 64.
@@ -195,19 +236,28 @@ static vgmi_status batch_launch(vgmi_batch *b, BatchOut const &o, hipStream_t st
 {
   if (b->ntiles == 0) return VGMI_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, st, b->d_counter);
-  const int grid = std::max(1, std::min(b->ntiles, b->numCU * b->blocksPerCU));
+  // resident workgroups per CU are bounded by the dynamic LDS too
+  const int perCU = b->ldsBytes ? std::max(1, std::min(b->blocksPerCU, kLdsPerCU / (b->ldsBytes + 1024)))
+                                : b->blocksPerCU;
+  const int grid  = std::max(1, std::min(b->ntiles, b->numCU * perCU));
+  const size_t lds = size_t(b->ldsBytes);
+  const int ldsD   = b->ldsBytes / int(sizeof(double));
   switch (b->block) {
   case 64:
-    hipLaunchKernelGGL(k_batch<64>, dim3(grid), dim3(64), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    hipLaunchKernelGGL(k_batch<64>, dim3(grid), dim3(64), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o,
+                       ldsD);
     break;
   case 128:
-    hipLaunchKernelGGL(k_batch<128>, dim3(grid), dim3(128), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    hipLaunchKernelGGL(k_batch<128>, dim3(grid), dim3(128), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter,
+                       o, ldsD);
     break;
   case 512:
-    hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter,
+                       o, ldsD);
     break;
   default:
-    hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), 0, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o);
+    hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter,
+                       o, ldsD);
     break;
   }
   hipError_t e = hipGetLastError();
@@ -241,6 +291,10 @@ vgmi_status vgmi_batch_create(const vgmi_shape *const *shapes, const int64_t *co
       return set_error(VGMI_ERR_INVALID, "vgmi_batch_create: bad shape or count at %d", s);
     }
     b->recs.push_back(shapes[s]->rec);
+    if (b->recs.back().type == kTessellated && counts[s] > 0) {
+      const int need = tess_lds_bytes(b->recs.back().u.tess);
+      if (need > 0 && need <= kTessLdsMax) b->ldsBytes = std::max(b->ldsBytes, need);
+    }
     b->offsets.push_back(b->total);
     b->counts.push_back(counts[s]);
     b->total += counts[s];

@@ -15,6 +15,10 @@
 
 // (enum value, record type, union member, kernel namespace)
 #define VGMI_FOR_EACH_SHAPE(X)                                                                                       \
+  VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)                                                                             \
+  X(kTessellated, TessellatedData, tess, tess)
+
+#define VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)                                                                       \
   X(kBox, BoxData, box, box)                                                                                         \
   X(kTube, TubeData, tube, tube)                                                                                     \
   X(kCone, ConeData, cone, cone)                                                                                     \
@@ -24,7 +28,6 @@
   X(kSphere, SphereData, sph, sph)                                                                                   \
   X(kPolyhedron, PolyhedronData, phed, phed)                                                                         \
   X(kTorus2, TorusData, tor, tor)                                                                                    \
-  X(kTessellated, TessellatedData, tess, tess)                                                                       \
   X(kHype, HypeData, hype, hype)
 
 namespace vgmi {

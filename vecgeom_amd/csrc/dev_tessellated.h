@@ -21,56 +21,113 @@ namespace tess {
 using V3d = Vec3<double>;
 
 VGMI_DEV const double *facet(TessellatedData const &s, int k) { return s.base + s.facetOff + k * kFacetDoubles; }
-VGMI_DEV const double *node(TessellatedData const &s, int j) { return s.base + s.nodeOff + j * kNodeDoubles; }
 VGMI_DEV V3d at3(const double *a) { return V3d(a[0], a[1], a[2]); }
+
+// Where the walks read the node records, the facet planes (normal[3], offset) and the facets'
+// single-precision candidate boxes: the solid's pool (PoolWalk), or the solid's image of them
+// that a workgroup copied to LDS (LdsWalk). The image's node boxes are the pool's rounded
+// outwards to single precision: a larger box passes every node test the pool's box passes, so
+// the walk reaches every facet the pool walk reaches, and facets are decided by the same FP64
+// values either way. Facet vertices and sides are always read from the pool, and only for facets
+// that pass the plane and box tests.
+// The walks read each record whole, before any test on it, so that its loads are in flight
+// together instead of one after another behind the tests' early exits.
+struct NodeRec {
+  V3d lo, hi;
+  int first, count, escape;
+};
+struct FBox {
+  float lo[3], hi[3];
+};
+struct Plane {
+  V3d n;
+  double off;
+};
+
+VGMI_DEV Plane plane_at(const double *P) { return Plane{at3(P), P[3]}; }
+VGMI_DEV FBox fbox_at(const float *b) { return FBox{{b[0], b[1], b[2]}, {b[3], b[4], b[5]}}; }
+
+struct PoolWalk {
+  const double *nodes, *facets;
+  VGMI_DEV NodeRec node(int j) const
+  {
+    const double *N  = nodes + j * kNodeDoubles;
+    const int32_t *I = reinterpret_cast<const int32_t *>(N + 6);
+    return NodeRec{at3(N), at3(N + 3), I[0], I[1], I[2]};
+  }
+  VGMI_DEV Plane plane(int k) const { return plane_at(facets + k * kFacetDoubles + 3); }
+  VGMI_DEV FBox fbox(int k) const { return fbox_at(reinterpret_cast<const float *>(facets + k * kFacetDoubles)); }
+};
+
+struct LdsWalk {
+  const TessNode32 *nodes;
+  const double *planes; // kImagePlaneDoubles per facet
+  const float *fboxes;  // kImageBoxFloats per facet
+  VGMI_DEV NodeRec node(int j) const
+  {
+    const TessNode32 n = nodes[j];
+    return NodeRec{V3d(n.lo[0], n.lo[1], n.lo[2]), V3d(n.hi[0], n.hi[1], n.hi[2]), n.first, n.count, n.escape};
+  }
+  VGMI_DEV Plane plane(int k) const { return plane_at(planes + k * kImagePlaneDoubles); }
+  VGMI_DEV FBox fbox(int k) const { return fbox_at(fboxes + k * kImageBoxFloats); }
+};
+
+VGMI_DEV PoolWalk pool_walk(TessellatedData const &s) { return PoolWalk{s.base + s.nodeOff, s.base + s.facetOff}; }
+
+// The LDS walk over an image already copied to `img` (TessellatedData::imageDoubles doubles).
+VGMI_DEV LdsWalk lds_walk(TessellatedData const &s, const double *img)
+{
+  return LdsWalk{reinterpret_cast<const TessNode32 *>(img), img + s.nnodes * kImageNodeDoubles,
+                 reinterpret_cast<const float *>(img + s.nnodes * kImageNodeDoubles + s.nfacets * kImagePlaneDoubles)};
+}
 
 // BoxImplementation::IntersectCachedKernel2<float, float> on the facet's single-precision box as
 // HybridNavigator2::GetHitCandidates_v calls it (:111-150): t0 = 0, t1 = FLT_MAX.
-VGMI_DEV bool box_candidate(const double *F, V3d const &p, V3d const &inv)
+// The reference's early exits are folded into one conjunction (same value: every test is a
+// comparison of values computed the same way whether or not an earlier test failed).
+VGMI_DEV bool box_candidate(FBox const &b, V3d const &p, V3d const &inv)
 {
-  const float *b = reinterpret_cast<const float *>(F); // lo x y z, hi x y z
-  const int sx = inv.x < 0, sy = inv.y < 0, sz = inv.z < 0;
+  const bool sx = inv.x < 0, sy = inv.y < 0, sz = inv.z < 0;
   const float px = float(p.x), py = float(p.y);
-  float tmin  = float(double(b[3 * sx + 0] - px) * inv.x);
-  float tymax = float(double(b[3 * (1 - sy) + 1] - py) * inv.y);
-  if (tmin > tymax) return false;
-  float tmax  = float(double(b[3 * (1 - sx) + 0] - px) * inv.x);
-  float tymin = float(double(b[3 * sy + 1] - py) * inv.y);
-  if (tymin > tmax) return false;
-  tmin        = tmin > tymin ? tmin : tymin;
-  tmax        = tmax < tymax ? tmax : tymax;
-  float tzmin = float((double(b[3 * sz + 2]) - p.z) * inv.z);
-  float tzmax = float((double(b[3 * (1 - sz) + 2]) - p.z) * inv.z);
-  if (tmin > tzmax || tzmin > tmax) return false;
-  tmin = tmin > tzmin ? tmin : tzmin;
-  tmax = tmax < tzmax ? tmax : tzmax;
-  if (!(tmin <= FLT_MAX && tmax > float(0.0 - kTolerance))) return false;
-  return tmin < FLT_MAX;
+  float tmin        = float(double((sx ? b.hi[0] : b.lo[0]) - px) * inv.x);
+  const float tymax = float(double((sy ? b.lo[1] : b.hi[1]) - py) * inv.y);
+  float tmax        = float(double((sx ? b.lo[0] : b.hi[0]) - px) * inv.x);
+  const float tymin = float(double((sy ? b.hi[1] : b.lo[1]) - py) * inv.y);
+  const bool xy     = !(tmin > tymax) && !(tymin > tmax);
+  tmin              = tmin > tymin ? tmin : tymin;
+  tmax              = tmax < tymax ? tmax : tymax;
+  const float tzmin = float((double(sz ? b.hi[2] : b.lo[2]) - p.z) * inv.z);
+  const float tzmax = float((double(sz ? b.lo[2] : b.hi[2]) - p.z) * inv.z);
+  const bool z      = !(tmin > tzmax || tzmin > tmax);
+  tmin              = tmin > tzmin ? tmin : tzmin;
+  tmax              = tmax < tzmax ? tmax : tzmax;
+  bool ok = xy;
+  ok &= z;
+  ok &= tmin <= FLT_MAX && tmax > float(0.0 - kTolerance);
+  ok &= tmin < FLT_MAX;
+  return ok;
 }
 
 // Ray against an enlarged node box: may the subtree hold a facet hit at a distance <= bound?
-VGMI_DEV bool node_reached(const double *N, V3d const &p, V3d const &inv, double bound)
+VGMI_DEV bool node_reached(V3d const &lo, V3d const &hi, V3d const &p, V3d const &inv, double bound)
 {
-  double t0 = (N[0] - p.x) * inv.x, t1 = (N[3] - p.x) * inv.x;
-  double u0 = (N[1] - p.y) * inv.y, u1 = (N[4] - p.y) * inv.y;
-  double w0 = (N[2] - p.z) * inv.z, w1 = (N[5] - p.z) * inv.z;
+  double t0 = (lo.x - p.x) * inv.x, t1 = (hi.x - p.x) * inv.x;
+  double u0 = (lo.y - p.y) * inv.y, u1 = (hi.y - p.y) * inv.y;
+  double w0 = (lo.z - p.z) * inv.z, w1 = (hi.z - p.z) * inv.z;
   const double tmin = vmax(vmin(t0, t1), vmax(vmin(u0, u1), vmin(w0, w1)));
   const double tmax = vmin(vmax(t0, t1), vmin(vmax(u0, u1), vmax(w0, w1)));
   return tmin <= tmax * (1. + 1e-12) + 1e-12 && tmax > -1e-6 && !(tmin > bound * (1. + 1e-12) + 1e-12);
 }
 
 // Squared distance from p to a node box (0 inside).
-VGMI_DEV double node_dist2(const double *N, V3d const &p)
+VGMI_DEV double node_dist2(V3d const &lo, V3d const &hi, V3d const &p)
 {
-  const double dx = vmax(vmax(N[0] - p.x, p.x - N[3]), 0.);
-  const double dy = vmax(vmax(N[1] - p.y, p.y - N[4]), 0.);
-  const double dz = vmax(vmax(N[2] - p.z, p.z - N[5]), 0.);
+  const double dx = vmax(vmax(lo.x - p.x, p.x - hi.x), 0.);
+  const double dy = vmax(vmax(lo.y - p.y, p.y - hi.y), 0.);
+  const double dz = vmax(vmax(lo.z - p.z, p.z - hi.z), 0.);
   return dx * dx + dy * dy + dz * dz;
 }
 
-VGMI_DEV int node_first(const double *N) { return reinterpret_cast<const int32_t *>(N + 6)[0]; }
-VGMI_DEV int node_count(const double *N) { return reinterpret_cast<const int32_t *>(N + 6)[1]; }
-VGMI_DEV int node_escape(const double *N) { return reinterpret_cast<const int32_t *>(N + 7)[0]; }
 
 struct RayResult {
   double dIn, dOut;
@@ -80,47 +137,53 @@ struct RayResult {
 
 // Facet loop of DistanceToSolid: the smallest entering and exiting hits (ties: lowest index).
 // `primaryIn` selects which of the two bounds the search (the other only matters below it).
-VGMI_DEV RayResult facet_hits(TessellatedData const &s, V3d const &p, V3d const &d, bool primaryIn)
+template <class W>
+VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool primaryIn)
 {
   const V3d inv(1.0 / NonZero(d.x), 1.0 / NonZero(d.y), 1.0 / NonZero(d.z));
   RayResult r{kInfLength, kInfLength, -1, -1, 0., 0.};
   int j = 0;
   while (j < s.nnodes) {
-    const double *N    = node(s, j);
+    const NodeRec nd   = w.node(j);
     const double bound = primaryIn ? r.dIn : r.dOut;
-    if (!node_reached(N, p, inv, bound)) {
-      j = node_escape(N);
+    if (!node_reached(nd.lo, nd.hi, p, inv, bound)) {
+      j = nd.escape;
       continue;
     }
-    const int cnt = node_count(N);
-    if (cnt == 0) {
+    if (nd.count == 0) {
       ++j;
       continue;
     }
-    const int first = node_first(N);
-    for (int k = first; k < first + cnt; ++k) {
-      const double *F = facet(s, k);
-      if (!box_candidate(F, p, inv)) continue;
-      const V3d n      = at3(F + 3);
-      const double ndd = NonZero(d.Dot(n));
-      const double saf = p.Dot(n) + F[6];
+    for (int k = nd.first; k < nd.first + nd.count; ++k) {
+      // The candidate box, side and triangle tests are pure predicates, taken here plane first.
+      // A hit farther than the current one of its kind cannot replace it (equal distances
+      // are decided by the facet index below).
+      const Plane P    = w.plane(k);
+      const FBox B     = w.fbox(k);
+      const double ndd = NonZero(d.Dot(P.n));
+      const double saf = p.Dot(P.n) + P.off;
       bool vin         = ndd < 0. && saf > -kTolerance;
       bool vout        = ndd > 0. && saf < kTolerance;
       if (!vin && !vout) continue;
       const double dist = -saf / ndd;
-      const V3d q       = p + dist * d;
-      const bool hit    = (q - at3(F + 7)).Dot(at3(F + 10)) > -kTolerance &&
-                       (q - at3(F + 13)).Dot(at3(F + 16)) > -kTolerance &&
-                       (q - at3(F + 19)).Dot(at3(F + 22)) > -kTolerance;
-      if (!hit) continue;
+      if (vin ? dist > r.dIn : dist > r.dOut) continue;
+      if (!box_candidate(B, p, inv)) continue;
+      const double *F = facet(s, k);
+      const V3d v0 = at3(F + 7), s0 = at3(F + 10), v1 = at3(F + 13), s1 = at3(F + 16), v2 = at3(F + 19),
+                s2     = at3(F + 22);
       const double idx = F[25];
+      const V3d q      = p + dist * d;
+      bool hit         = (q - v0).Dot(s0) > -kTolerance;
+      hit &= (q - v1).Dot(s1) > -kTolerance;
+      hit &= (q - v2).Dot(s2) > -kTolerance;
+      if (!hit) continue;
       if (vin) {
         if (dist < r.dIn || (dist == r.dIn && idx < r.idxIn)) r.dIn = dist, r.sIn = k, r.idxIn = idx;
       } else {
         if (dist < r.dOut || (dist == r.dOut && idx < r.idxOut)) r.dOut = dist, r.sOut = k, r.idxOut = idx;
       }
     }
-    j = node_escape(N);
+    j = nd.escape;
   }
   return r;
 }
@@ -152,9 +215,10 @@ VGMI_DEV double extent_hit(TessellatedData const &s, V3d const &p, V3d const &d)
 VGMI_DEV double dot_normal(TessellatedData const &s, int slot, V3d const &d) { return d.Dot(at3(facet(s, slot) + 3)); }
 
 // DistanceToSolid<ToIn = false> along the test direction (Contains / Inside)
-VGMI_DEV RayResult test_ray(TessellatedData const &s, V3d const &p)
+template <class W>
+VGMI_DEV RayResult test_ray(TessellatedData const &s, W const &w, V3d const &p)
 {
-  return facet_hits(s, p, at3(s.testDir), false);
+  return facet_hits(s, w, p, at3(s.testDir), false);
 }
 
 VGMI_DEV bool contains_of(TessellatedData const &s, RayResult const &r)
@@ -164,11 +228,16 @@ VGMI_DEV bool contains_of(TessellatedData const &s, RayResult const &r)
   return !(r.sIn >= 0 && r.dIn < r.dOut && r.dIn * dot_normal(s, r.sIn, t) < -kTolerance);
 }
 
-VGMI_DEV bool contains64(TessellatedData const &s, V3d const &p) { return contains_of(s, test_ray(s, p)); }
-
-VGMI_DEV int inside64(TessellatedData const &s, V3d const &p)
+template <class W>
+VGMI_DEV bool contains64(TessellatedData const &s, W const &w, V3d const &p)
 {
-  const RayResult r = test_ray(s, p);
+  return contains_of(s, test_ray(s, w, p));
+}
+
+template <class W>
+VGMI_DEV int inside64(TessellatedData const &s, W const &w, V3d const &p)
+{
+  const RayResult r = test_ray(s, w, p);
   const V3d t       = at3(s.testDir);
   // DistanceToSolid's own "outside" verdict resets the exit surface
   int sOut          = r.sOut;
@@ -181,10 +250,11 @@ VGMI_DEV int inside64(TessellatedData const &s, V3d const &p)
   return kOutside;
 }
 
-VGMI_DEV double dist_in64(TessellatedData const &s, V3d const &p, V3d const &d)
+template <class W>
+VGMI_DEV double dist_in64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
 {
   if (extent_hit(s, p, d) >= kInfLength) return kInfLength;
-  const RayResult r = facet_hits(s, p, d, true);
+  const RayResult r = facet_hits(s, w, p, d, true);
   if (r.sIn < 0) {
     if (r.sOut >= 0 && r.dOut * dot_normal(s, r.sOut, d) > kTolerance) return -1.;
     return kInfLength;
@@ -193,9 +263,10 @@ VGMI_DEV double dist_in64(TessellatedData const &s, V3d const &p, V3d const &d)
   return r.dIn;
 }
 
-VGMI_DEV double dist_out64(TessellatedData const &s, V3d const &p, V3d const &d)
+template <class W>
+VGMI_DEV double dist_out64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
 {
-  const RayResult r = facet_hits(s, p, d, false);
+  const RayResult r = facet_hits(s, w, p, d, false);
   if (r.sOut < 0) return -1.;
   if (r.sIn >= 0 && r.dIn < r.dOut && r.dIn * dot_normal(s, r.sIn, d) < -kTolerance) return -1.;
   return r.dOut;
@@ -215,16 +286,19 @@ VGMI_DEV double seg_dist2(V3d const &c0, V3d const &c1, V3d const &p)
 // facet per cluster): the squared plane distance when the projection lies within the triangle on
 // the right side, else the smallest squared distance to its edges. Facets whose squared plane
 // distance exceeds `lim` are skipped (their value is at least that, since the projection and every
-// edge point lie in the plane; the 1e-9 margin covers the rounding of both quantities).
-VGMI_DEV void facet_safety_sq(const double *F, V3d const &p, bool toIn, double lim, double &best)
+// edge point lie in the plane; the 1e-9 margin covers the rounding of both quantities). P is the
+// facet's plane, F its record.
+VGMI_DEV void facet_safety_sq(Plane const &P, const double *F, V3d const &p, bool toIn, double lim, double &best)
 {
-  const V3d n     = at3(F + 3);
-  const double sp = p.Dot(n) + F[6];
+  const V3d n     = P.n;
+  const double sp = p.Dot(n) + P.off;
   if (sp * sp > lim * (1. + 1e-9)) return;
   const V3d q  = p - sp * n;
-  const V3d v0 = at3(F + 7), v1 = at3(F + 13), v2 = at3(F + 19);
-  bool within = (q - v0).Dot(at3(F + 10)) > -kTolerance && (q - v1).Dot(at3(F + 16)) > -kTolerance &&
-                (q - v2).Dot(at3(F + 22)) > -kTolerance;
+  const V3d v0 = at3(F + 7), s0 = at3(F + 10), v1 = at3(F + 13), s1 = at3(F + 16), v2 = at3(F + 19),
+            s2 = at3(F + 22);
+  bool within = (q - v0).Dot(s0) > -kTolerance;
+  within &= (q - v1).Dot(s1) > -kTolerance;
+  within &= (q - v2).Dot(s2) > -kTolerance;
   within &= toIn ? sp > -kTolerance : sp < kTolerance;
   double sq = sp * sp;
   if (!within) {
@@ -234,55 +308,65 @@ VGMI_DEV void facet_safety_sq(const double *F, V3d const &p, bool toIn, double l
   if (sq < best) best = sq;
 }
 
+// The facets of one leaf, in order.
+template <class W>
+VGMI_DEV void leaf_safety_sq(TessellatedData const &s, W const &w, NodeRec const &nd, V3d const &p, bool toIn,
+                             double bound, double &best)
+{
+  for (int k = nd.first; k < nd.first + nd.count; ++k)
+    facet_safety_sq(w.plane(k), facet(s, k), p, toIn, vmin(best, bound), best);
+}
+
 // SafetySq<ToIn>: the smallest per-facet squared safety over all facets (the minimum does not
 // depend on the visiting order). `bound` is an upper bound of that minimum; subtrees whose box is
 // farther than the current minimum (or the bound) are skipped. A greedy descent into the nearer
 // child seeds the minimum, then the threaded depth-first walk (each box read once) prunes against it.
-VGMI_DEV double safety_sq_threaded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
+template <class W>
+VGMI_DEV double safety_sq_threaded(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double bound)
 {
   double best = kInfLength;
   int j       = 0;
   while (j < s.nnodes) {
-    const double *N = node(s, j);
-    if (node_dist2(N, p) > vmin(best, bound)) {
-      j = node_escape(N);
+    const NodeRec nd = w.node(j);
+    if (node_dist2(nd.lo, nd.hi, p) > vmin(best, bound)) {
+      j = nd.escape;
       continue;
     }
-    const int cnt = node_count(N);
-    if (cnt == 0) {
+    if (nd.count == 0) {
       ++j;
       continue;
     }
-    const int first = node_first(N);
-    for (int k = first; k < first + cnt; ++k) facet_safety_sq(facet(s, k), p, toIn, vmin(best, bound), best);
-    j = node_escape(N);
+    leaf_safety_sq(s, w, nd, p, toIn, bound, best);
+    j = nd.escape;
   }
   return best;
 }
 
 // The facets of the leaf reached by always descending into the nearer child give a first minimum
 // (an upper bound of the result), then the threaded walk prunes against it.
-VGMI_DEV double safety_sq_seeded(TessellatedData const &s, V3d const &p, bool toIn, double bound)
+template <class W>
+VGMI_DEV double safety_sq_seeded(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double bound)
 {
   double seed = kInfLength;
   int j       = 0;
+  NodeRec nd = w.node(0);
   for (;;) {
-    const double *N = node(s, j);
-    const int cnt   = node_count(N);
-    if (cnt > 0) {
-      const int first = node_first(N);
-      for (int k = first; k < first + cnt; ++k) facet_safety_sq(facet(s, k), p, toIn, vmin(seed, bound), seed);
+    if (nd.count > 0) {
+      leaf_safety_sq(s, w, nd, p, toIn, bound, seed);
       break;
     }
-    const int l = j + 1, r = node_escape(node(s, l));
-    j = node_dist2(node(s, l), p) <= node_dist2(node(s, r), p) ? l : r;
+    const NodeRec l = w.node(j + 1), r = w.node(l.escape);
+    const bool left = node_dist2(l.lo, l.hi, p) <= node_dist2(r.lo, r.hi, p);
+    j               = left ? j + 1 : l.escape;
+    nd              = left ? l : r;
   }
-  return vmin(seed, safety_sq_threaded(s, p, toIn, vmin(seed, bound)));
+  return vmin(seed, safety_sq_threaded(s, w, p, toIn, vmin(seed, bound)));
 }
 
-VGMI_DEV double safety_sq(TessellatedData const &s, V3d const &p, bool toIn, double bound = kInfLength)
+template <class W>
+VGMI_DEV double safety_sq(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double bound = kInfLength)
 {
-  return safety_sq_seeded(s, p, toIn, bound);
+  return safety_sq_seeded(s, w, p, toIn, bound);
 }
 
 // An upper bound of the smallest per-facet squared safety. That minimum is at most the squared
@@ -299,10 +383,11 @@ VGMI_DEV double safety_bound(TessellatedData const &s, V3d const &p, double tHit
 
 // All six methods of one point with one test-ray walk (Contains, Inside and both safeties),
 // one safety walk (the other safety is -1) and the two directed walks.
-VGMI_DEV void query_all(TessellatedData const &s, V3d const &p, V3d const &d, bool &contains, int &inside,
-                        double &sin, double &sout, double &din, double &dout)
+template <class W>
+VGMI_DEV void query_all(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool &contains,
+                        int &inside, double &sin, double &sout, double &din, double &dout)
 {
-  const RayResult r = test_ray(s, p);
+  const RayResult r = test_ray(s, w, p);
   const V3d t       = at3(s.testDir);
   int sOut          = r.sOut;
   double dOut       = r.dOut;
@@ -320,59 +405,69 @@ VGMI_DEV void query_all(TessellatedData const &s, V3d const &p, V3d const &d, bo
     inside = kOutside;
   if (contains) {
     sin  = -1.;
-    sout = vsqrt(safety_sq(s, p, false, safety_bound(s, p, r.dOut)));
+    sout = vsqrt(safety_sq(s, w, p, false, safety_bound(s, p, r.dOut)));
   } else {
     sout = -1.;
-    sin  = vsqrt(safety_sq(s, p, true, safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength)));
+    sin  = vsqrt(safety_sq(s, w, p, true, safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength)));
   }
-  din  = dist_in64(s, p, d);
-  dout = dist_out64(s, p, d);
+  din  = dist_in64(s, w, p, d);
+  dout = dist_out64(s, w, p, d);
+}
+
+template <class W>
+VGMI_DEV double safety_in64(TessellatedData const &s, W const &w, V3d const &q)
+{
+  const RayResult r = test_ray(s, w, q);
+  if (contains_of(s, r)) return -1.;
+  return vsqrt(safety_sq(s, w, q, true, safety_bound(s, q, r.sIn >= 0 ? r.dIn : kInfLength)));
+}
+
+template <class W>
+VGMI_DEV double safety_out64(TessellatedData const &s, W const &w, V3d const &q)
+{
+  const RayResult r = test_ray(s, w, q);
+  if (!contains_of(s, r)) return -1.;
+  return vsqrt(safety_sq(s, w, q, false, safety_bound(s, q, r.dOut)));
 }
 
 template <class R>
 VGMI_DEV bool Contains(TessellatedData const &s, Vec3<R> const &p)
 {
-  return contains64(s, V3d(p.x, p.y, p.z));
+  return contains64(s, pool_walk(s), V3d(p.x, p.y, p.z));
 }
 
 template <class R>
 VGMI_DEV int Inside(TessellatedData const &s, Vec3<R> const &p)
 {
-  return inside64(s, V3d(p.x, p.y, p.z));
+  return inside64(s, pool_walk(s), V3d(p.x, p.y, p.z));
 }
 
 template <class R>
 VGMI_DEV R DistanceToIn(TessellatedData const &s, Vec3<R> const &p, Vec3<R> const &d)
 {
   if (sizeof(R) == sizeof(float)) return R(__builtin_nanf(""));
-  return R(dist_in64(s, V3d(p.x, p.y, p.z), V3d(d.x, d.y, d.z)));
+  return R(dist_in64(s, pool_walk(s), V3d(p.x, p.y, p.z), V3d(d.x, d.y, d.z)));
 }
 
 template <class R>
 VGMI_DEV R DistanceToOut(TessellatedData const &s, Vec3<R> const &p, Vec3<R> const &d)
 {
   if (sizeof(R) == sizeof(float)) return R(__builtin_nanf(""));
-  return R(dist_out64(s, V3d(p.x, p.y, p.z), V3d(d.x, d.y, d.z)));
+  return R(dist_out64(s, pool_walk(s), V3d(p.x, p.y, p.z), V3d(d.x, d.y, d.z)));
 }
 
 template <class R>
 VGMI_DEV R SafetyToIn(TessellatedData const &s, Vec3<R> const &p)
 {
   if (sizeof(R) == sizeof(float)) return R(__builtin_nanf(""));
-  const V3d q(p.x, p.y, p.z);
-  const RayResult r = test_ray(s, q);
-  if (contains_of(s, r)) return R(-1.);
-  return R(vsqrt(safety_sq(s, q, true, safety_bound(s, q, r.sIn >= 0 ? r.dIn : kInfLength))));
+  return R(safety_in64(s, pool_walk(s), V3d(p.x, p.y, p.z)));
 }
 
 template <class R>
 VGMI_DEV R SafetyToOut(TessellatedData const &s, Vec3<R> const &p)
 {
   if (sizeof(R) == sizeof(float)) return R(__builtin_nanf(""));
-  const V3d q(p.x, p.y, p.z);
-  const RayResult r = test_ray(s, q);
-  if (!contains_of(s, r)) return R(-1.);
-  return R(vsqrt(safety_sq(s, q, false, safety_bound(s, q, r.dOut))));
+  return R(safety_out64(s, pool_walk(s), V3d(p.x, p.y, p.z)));
 }
 
 } // namespace tess

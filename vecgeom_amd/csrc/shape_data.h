@@ -152,15 +152,31 @@ struct HypeData {
 //   node j (kNodeDoubles doubles): box lo[3] hi[3] (enlarged by a margin covering the
 //     single-precision facet boxes), then int32 first facet, facet count (0: internal node),
 //     escape node (next node in depth-first order after this subtree), unused.
+// The pool also holds an image of what the walks read most, laid out for a workgroup to copy to
+// LDS as one block (imageDoubles doubles at imageOff; 0 when the solid has 65536 facets or more):
+// the nodes as TessNode32 (box rounded outwards to single precision), then every facet's plane
+// (normal[3], offset), then every facet's single-precision box (lo[3], hi[3]).
 struct TessellatedData {
   int32_t nfacets, nnodes;
   int32_t facetOff, nodeOff;
   const double *base;
   double minExt[3], maxExt[3];
   double testDir[3]; // direction of the Contains/Inside ray
+  int32_t imageOff, imageDoubles;
 };
-constexpr int kFacetDoubles = 26;
-constexpr int kNodeDoubles  = 8;
+constexpr int kFacetDoubles   = 26;
+constexpr int kNodeDoubles    = 8;
+constexpr int kTessLeafFacets = 4; // facets per BVH leaf, at most
+
+struct TessNode32 {
+  float lo[3], hi[3];
+  uint16_t first, count;
+  int32_t escape;
+};
+static_assert(sizeof(TessNode32) == 32, "TessNode32 is four doubles");
+constexpr int kImageNodeDoubles  = 4;
+constexpr int kImagePlaneDoubles = 4;
+constexpr int kImageBoxFloats    = 6;
 
 // Union of all records: the device shape table stores one per unplaced shape.
 // Variable-length data (polycone sections, tessellated facets, ...) lives in a

@@ -86,12 +86,23 @@ D3 TestDir(std::vector<HostFacet> const &f)
   return Normalized(D3{cand[0][0], cand[0][1], cand[0][2]});
 }
 
+// Single-precision bounds of x from below and from above.
+float FloatDown(double x)
+{
+  float f = float(x);
+  return double(f) > x ? std::nextafter(f, -FLT_MAX) : f;
+}
+float FloatUp(double x)
+{
+  float f = float(x);
+  return double(f) < x ? std::nextafter(f, FLT_MAX) : f;
+}
+
 struct BuildNode {
   double lo[3], hi[3];
   int first, count, escape;
 };
 
-constexpr int kLeafFacets = 4;
 
 // Depth-first build: the node is appended, its left then right subtree follow it, and its
 // escape index is the first node after the subtree.
@@ -115,7 +126,7 @@ void Build(std::vector<HostFacet> &f, int b, int e, std::vector<BuildNode> &node
   }
   const int self = int(nodes.size());
   nodes.push_back(nd);
-  if (e - b <= kLeafFacets) {
+  if (e - b <= kTessLeafFacets) {
     nodes[self].first  = b;
     nodes[self].count  = e - b;
     nodes[self].escape = self + 1;
@@ -169,7 +180,11 @@ int MakeTessellated(double const *a, int n, ShapeRecord &r, std::vector<double> 
   T.facetOff = 0;
   T.nodeOff  = T.nfacets * kFacetDoubles;
   T.base     = nullptr;
-  extra.assign(size_t(T.nodeOff) + size_t(T.nnodes) * kNodeDoubles, 0.0);
+  T.imageOff = T.nodeOff + T.nnodes * kNodeDoubles;
+  T.imageDoubles = T.nfacets < 65536 ? T.nnodes * kImageNodeDoubles + T.nfacets * kImagePlaneDoubles +
+                                          (T.nfacets * kImageBoxFloats + 1) / 2
+                                    : 0;
+  extra.assign(size_t(T.imageOff) + size_t(T.imageDoubles), 0.0);
   for (int k = 0; k < T.nfacets; ++k) {
     HostFacet const &t = f[k];
     double *F          = extra.data() + size_t(k) * kFacetDoubles;
@@ -189,6 +204,25 @@ int MakeTessellated(double const *a, int n, ShapeRecord &r, std::vector<double> 
     for (int k = 0; k < 3; ++k) N[k] = nodes[j].lo[k], N[3 + k] = nodes[j].hi[k];
     int32_t ints[4] = {nodes[j].first, nodes[j].count, nodes[j].escape, 0};
     std::memcpy(N + 6, ints, sizeof(ints));
+  }
+  if (T.imageDoubles > 0) {
+    double *img = extra.data() + T.imageOff;
+    for (int j = 0; j < T.nnodes; ++j) {
+      TessNode32 n;
+      for (int k = 0; k < 3; ++k) n.lo[k] = FloatDown(nodes[j].lo[k]), n.hi[k] = FloatUp(nodes[j].hi[k]);
+      n.first  = uint16_t(nodes[j].first);
+      n.count  = uint16_t(nodes[j].count);
+      n.escape = nodes[j].escape;
+      std::memcpy(img + size_t(j) * kImageNodeDoubles, &n, sizeof(n));
+    }
+    double *pl = img + size_t(T.nnodes) * kImageNodeDoubles;
+    for (int k = 0; k < T.nfacets; ++k)
+      std::memcpy(pl + size_t(k) * kImagePlaneDoubles, extra.data() + size_t(k) * kFacetDoubles + 3,
+                  kImagePlaneDoubles * sizeof(double));
+    float *bx = reinterpret_cast<float *>(pl + size_t(T.nfacets) * kImagePlaneDoubles);
+    for (int k = 0; k < T.nfacets; ++k)
+      std::memcpy(bx + size_t(k) * kImageBoxFloats, extra.data() + size_t(k) * kFacetDoubles,
+                  kImageBoxFloats * sizeof(float));
   }
   return 0;
 }
