@@ -75,6 +75,14 @@ def cpu_info():
     return model, os.cpu_count()
 
 
+def host_cores():
+    """Threads for the all-cores CPU baseline: the host's CPUs, at most 16 (a GPU's share of the host)."""
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
     (profiles/traffic.json, scaled by a plain-copy calibration), or None."""
@@ -155,25 +163,35 @@ def run_mix(args):
     if not args.no_cpu:
         from oracle.pyoracle import OracleShape
         orc = [OracleShape(k, prm) for k, prm in shapes]
+
+        def mix_rate(sample, threads, seconds):
+            done, used = 0, 0.0
+            while used < seconds:
+                off = 0
+                for o, c in zip(orc, counts):
+                    m = min(c, sample)
+                    ps = np.ascontiguousarray(p[:, off:off + m])
+                    dsm = np.ascontiguousarray(d[:, off:off + m])
+                    t0 = time.perf_counter()
+                    for meth in METHODS:
+                        o.query(meth, ps, dsm if meth.startswith("Distance") else None, threads=threads)
+                    used += time.perf_counter() - t0
+                    done += m * len(METHODS)
+                    off += c
+            return done, used
+
         sample = 1 << 13
-        done, used = 0, 0.0
-        while used < args.cpu_seconds:
-            off = 0
-            for o, c in zip(orc, counts):
-                m = min(c, sample)
-                ps = np.ascontiguousarray(p[:, off:off + m])
-                dsm = np.ascontiguousarray(d[:, off:off + m])
-                t0 = time.perf_counter()
-                for meth in METHODS:
-                    o.query(meth, ps, dsm if meth.startswith("Distance") else None, threads=1)
-                used += time.perf_counter() - t0
-                done += m * len(METHODS)
-                off += c
+        done, used = mix_rate(sample, 1, args.cpu_seconds)
         model, ncpu = cpu_info()
         res["cpu_baseline"] = {"value": round(done / used, 1), "unit": "queries/s", "cores": 1, "kind": "port",
                                "sample": "%d queries: first %d points of every shape of the same mix x 6 methods, "
                                          "repeated for ~%.0f s; host %s, %d logical CPUs"
                                          % (done, sample, args.cpu_seconds, model, ncpu)}
+        cores = host_cores()
+        done, used = mix_rate(1 << 15, cores, args.cpu_seconds / 2)
+        res["cpu_baseline"]["all_cores"] = {"value": round(done / used, 1), "cores": cores,
+                                            "sample": "%d queries, first %d points of every shape, %d threads"
+                                                      % (done, 1 << 15, cores)}
     return res
 
 
@@ -327,18 +345,28 @@ def run_nav(args):
         from oracle.pyoracle import OracleGeometry
         O = OracleGeometry(g)
         m = min(n, 1 << 14)
-        pn = nav0[:m].cpu().numpy().astype(np.uint32)
-        t0 = time.perf_counter()
-        done, reps_c = 0, 0
-        while time.perf_counter() - t0 < args.cpu_seconds:
-            _, _, _, k = O.propagate(p[:, :m], d[:, :m], limit, pn, np.zeros_like(pn), nsteps, threads=1)
-            done += k
-            reps_c += 1
-        dt = time.perf_counter() - t0
+
+        def nav_rate(m, threads, seconds):
+            pn = nav0[:m].cpu().numpy().astype(np.uint32)
+            t0 = time.perf_counter()
+            done, reps_c = 0, 0
+            while time.perf_counter() - t0 < seconds:
+                _, _, _, k = O.propagate(p[:, :m], d[:, :m], limit, pn, np.zeros_like(pn), nsteps, threads=threads)
+                done += k
+                reps_c += 1
+            return done, reps_c, time.perf_counter() - t0
+
+        done, reps_c, dt = nav_rate(m, 1, args.cpu_seconds)
         model, ncpu = cpu_info()
         res["cpu_baseline"] = {"value": round(done / dt, 1), "unit": "nav-steps/s", "cores": 1, "kind": "port",
                                "sample": "%d x (first %d tracks x %d steps) = %d nav-steps; host %s, %d CPUs"
                                          % (reps_c, m, nsteps, done, model, ncpu)}
+        cores = host_cores()
+        ma = min(n, 1 << 16)
+        done, reps_c, dt = nav_rate(ma, cores, args.cpu_seconds / 2)
+        res["cpu_baseline"]["all_cores"] = {"value": round(done / dt, 1), "cores": cores,
+                                            "sample": "%d x (first %d tracks x %d steps), %d threads"
+                                                      % (reps_c, ma, nsteps, cores)}
     return res
 
 
