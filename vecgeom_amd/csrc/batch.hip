@@ -65,7 +65,13 @@ VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int
       if (o.inside) __builtin_nontemporal_store(int32_t(tess::inside64(s, w, p)), o.inside + i);
       if (o.sin) __builtin_nontemporal_store(tess::safety_in64(s, w, p), o.sin + i);
       if (o.sout) __builtin_nontemporal_store(tess::safety_out64(s, w, p), o.sout + i);
-      if (o.din || o.dout) {
+      if (o.din && o.dout) {
+        const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
+        double din, dout;
+        tess::dist_both64(s, w, p, d, din, dout);
+        __builtin_nontemporal_store(din, o.din + i);
+        __builtin_nontemporal_store(dout, o.dout + i);
+      } else if (o.din || o.dout) {
         const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
         if (o.din) __builtin_nontemporal_store(tess::dist_in64(s, w, p, d), o.din + i);
         if (o.dout) __builtin_nontemporal_store(tess::dist_out64(s, w, p, d), o.dout + i);

@@ -136,16 +136,21 @@ struct RayResult {
 };
 
 // Facet loop of DistanceToSolid: the smallest entering and exiting hits (ties: lowest index).
-// `primaryIn` selects which of the two bounds the search (the other only matters below it).
+// `mode` selects the bound of the search: the entering hit (kHitsIn; the exiting hit only matters
+// below it), the exiting hit (kHitsOut; likewise), or both (kHitsBoth: a subtree is skipped only
+// beyond both current hits, so both come out as the smallest over all facets, and each holds every
+// value the single searches would give it where they use it).
+constexpr int kHitsIn = 0, kHitsOut = 1, kHitsBoth = 2;
+
 template <class W>
-VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool primaryIn)
+VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, int mode)
 {
   const V3d inv(1.0 / NonZero(d.x), 1.0 / NonZero(d.y), 1.0 / NonZero(d.z));
   RayResult r{kInfLength, kInfLength, -1, -1, 0., 0.};
   int j = 0;
   while (j < s.nnodes) {
     const NodeRec nd   = w.node(j);
-    const double bound = primaryIn ? r.dIn : r.dOut;
+    const double bound = mode == kHitsIn ? r.dIn : (mode == kHitsOut ? r.dOut : vmax(r.dIn, r.dOut));
     if (!node_reached(nd.lo, nd.hi, p, inv, bound)) {
       j = nd.escape;
       continue;
@@ -218,7 +223,7 @@ VGMI_DEV double dot_normal(TessellatedData const &s, int slot, V3d const &d) { r
 template <class W>
 VGMI_DEV RayResult test_ray(TessellatedData const &s, W const &w, V3d const &p)
 {
-  return facet_hits(s, w, p, at3(s.testDir), false);
+  return facet_hits(s, w, p, at3(s.testDir), kHitsOut);
 }
 
 VGMI_DEV bool contains_of(TessellatedData const &s, RayResult const &r)
@@ -250,11 +255,9 @@ VGMI_DEV int inside64(TessellatedData const &s, W const &w, V3d const &p)
   return kOutside;
 }
 
-template <class W>
-VGMI_DEV double dist_in64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
+// DistanceToSolid<ToIn = true / false> results from the hits of a search along d.
+VGMI_DEV double dist_in_of(TessellatedData const &s, RayResult const &r, V3d const &d)
 {
-  if (extent_hit(s, p, d) >= kInfLength) return kInfLength;
-  const RayResult r = facet_hits(s, w, p, d, true);
   if (r.sIn < 0) {
     if (r.sOut >= 0 && r.dOut * dot_normal(s, r.sOut, d) > kTolerance) return -1.;
     return kInfLength;
@@ -263,13 +266,33 @@ VGMI_DEV double dist_in64(TessellatedData const &s, W const &w, V3d const &p, V3
   return r.dIn;
 }
 
-template <class W>
-VGMI_DEV double dist_out64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
+VGMI_DEV double dist_out_of(TessellatedData const &s, RayResult const &r, V3d const &d)
 {
-  const RayResult r = facet_hits(s, w, p, d, false);
   if (r.sOut < 0) return -1.;
   if (r.sIn >= 0 && r.dIn < r.dOut && r.dIn * dot_normal(s, r.sIn, d) < -kTolerance) return -1.;
   return r.dOut;
+}
+
+template <class W>
+VGMI_DEV double dist_in64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
+{
+  if (extent_hit(s, p, d) >= kInfLength) return kInfLength;
+  return dist_in_of(s, facet_hits(s, w, p, d, kHitsIn), d);
+}
+
+template <class W>
+VGMI_DEV double dist_out64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
+{
+  return dist_out_of(s, facet_hits(s, w, p, d, kHitsOut), d);
+}
+
+// Both distances along d from one search.
+template <class W>
+VGMI_DEV void dist_both64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, double &din, double &dout)
+{
+  const RayResult r = facet_hits(s, w, p, d, kHitsBoth);
+  din               = extent_hit(s, p, d) >= kInfLength ? kInfLength : dist_in_of(s, r, d);
+  dout              = dist_out_of(s, r, d);
 }
 
 VGMI_DEV double seg_dist2(V3d const &c0, V3d const &c1, V3d const &p)
@@ -382,7 +405,7 @@ VGMI_DEV double safety_bound(TessellatedData const &s, V3d const &p, double tHit
 }
 
 // All six methods of one point with one test-ray walk (Contains, Inside and both safeties),
-// one safety walk (the other safety is -1) and the two directed walks.
+// one safety walk (the other safety is -1) and one walk along d for both distances.
 template <class W>
 VGMI_DEV void query_all(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool &contains,
                         int &inside, double &sin, double &sout, double &din, double &dout)
@@ -410,8 +433,7 @@ VGMI_DEV void query_all(TessellatedData const &s, W const &w, V3d const &p, V3d 
     sout = -1.;
     sin  = vsqrt(safety_sq(s, w, p, true, safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength)));
   }
-  din  = dist_in64(s, w, p, d);
-  dout = dist_out64(s, w, p, d);
+  dist_both64(s, w, p, d, din, dout);
 }
 
 template <class W>
