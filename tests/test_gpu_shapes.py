@@ -93,9 +93,12 @@ def test_fp64_parity(kind, params):
             assert exact >= EXACT_FRACTION_BY_KIND.get(kind, 0.999) * len(ref), (m, exact, len(ref))
 
 
-def test_query_all_matches_single_methods():
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_query_all_matches_single_methods(case):
+    """The six-method launch (the tessellated solid shares its walks across methods) equals the
+    single-method launches bit for bit."""
     from vecgeom_amd import make_shape
-    kind, params = CASES[3]
+    kind, params = CASES[case]
     o, p, d = gpu_util.inputs_for(kind, params, 1 << 14, seed=7)
     s = make_shape(kind, params)
     P, D = gpu_util.to_dev(p), gpu_util.to_dev(d)

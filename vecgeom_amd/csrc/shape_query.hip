@@ -9,6 +9,7 @@
 // is within the near-surface band in FP64, so decisions stay exact.
 #include "vgmi_internal.h"
 #include "dev_shapes.h"
+#include <type_traits>
 
 namespace vgmi {
 
@@ -68,6 +69,23 @@ __global__ __launch_bounds__(kBlock) void k_query_all_fp64(Data s, QueryAllArgs 
   size_t strid = size_t(gridDim.x) * kBlock;
   for (; i < a.n; i += strid) {
     Vec3<double> p = load3<double>(a.px, a.py, a.pz, i);
+    if constexpr (std::is_same<Data, TessellatedData>::value) {
+      // one test-ray walk, one safety walk and one walk along d serve all six methods
+      if (a.contains && a.inside && a.sin && a.sout && a.din && a.dout) {
+        const Vec3<double> d = load3<double>(a.dx, a.dy, a.dz, i);
+        bool c;
+        int in;
+        double si, so, di, dou;
+        tess::query_all(s, tess::pool_walk(s), p, d, c, in, si, so, di, dou);
+        __builtin_nontemporal_store(uint8_t(c), a.contains + i);
+        __builtin_nontemporal_store(int32_t(in), a.inside + i);
+        __builtin_nontemporal_store(si, a.sin + i);
+        __builtin_nontemporal_store(so, a.sout + i);
+        __builtin_nontemporal_store(di, a.din + i);
+        __builtin_nontemporal_store(dou, a.dout + i);
+        continue;
+      }
+    }
     if (a.contains) __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), a.contains + i);
     if (a.inside) __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), a.inside + i);
     if (a.sin) __builtin_nontemporal_store(S::template SafeIn<double>(s, p), a.sin + i);
