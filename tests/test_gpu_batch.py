@@ -109,3 +109,27 @@ def test_batch_tessellated_lds_and_pool_walks():
         for m in METHODS:
             if part[m] is not None:
                 assert torch.equal(part[m], out[m]), (m, tuning)
+
+
+def test_batch_tessellated_deep_points():
+    """Points deep inside the 1024-facet hull (their safety searches are queued for whole
+    wavefronts, and with 512-point tiles the queue overflows so lanes also search themselves)
+    and far outside it: every method equals the oracle bit for bit."""
+    params = workloads.convex_hull_sphere(20.0)
+    s = make_shape("tessellated", params)
+    o = OracleShape("tessellated", params)
+    g = np.random.default_rng(11)
+    n_in, n_out = 3000, 1000
+    r = 6.0 * g.random(n_in) ** (1.0 / 3.0)
+    u = workloads.isotropic_dirs(g, n_in)
+    far = workloads.isotropic_dirs(g, n_out) * (25.0 + 30.0 * g.random(n_out))
+    p = np.ascontiguousarray(np.concatenate([u * r, far], axis=1))
+    d = np.ascontiguousarray(workloads.isotropic_dirs(g, n_in + n_out))
+    P, D = SOA3D.from_array(p), SOA3D.from_array(d)
+    ref = {m: o.query(m, p, d if m.startswith("Distance") else None) for m in METHODS}
+    b = ShapeBatch([s], [n_in + n_out])
+    for tuning in [(512, 512, 1), (256, 256, 4), (64, 64, 2)]:
+        b.set_tuning(*tuning)
+        out = b.query_all(P, D)
+        for m in METHODS:
+            assert np.array_equal(ref[m], out[m].cpu().numpy()), (m, tuning)

@@ -50,46 +50,92 @@ VGMI_DEV void run_tile(Data const &s, int64_t b, int64_t e, BatchOut const &o, i
   }
 }
 
-// The tessellated solid shares one test-ray walk and one safety walk across its six methods;
-// the walks read nodes, planes and candidate boxes through `w` (the pool or the LDS image).
+// Points of a tessellated tile whose safety search is left to a wavefront (deep inside or far
+// from the solid): offset in the tile, side, seed and bound of the search.
+constexpr int kTessQueue = 128;
+struct TessQueue {
+  int n;
+  int slot[kTessQueue]; // tile offset, bit 30 set for SafetyToIn
+  double seed[kTessQueue], lim[kTessQueue];
+};
+
+// The tessellated solid shares one test-ray walk, one safety walk and one walk along d across its
+// six methods; the walks read nodes, planes and candidate boxes through `w` (the pool or the LDS
+// image). With all six outputs, a lane whose point is deep (tess::deep_sq) queues its safety search
+// and the workgroup's wavefronts run the queued searches together afterwards, each over all facets:
+// a deep point's walk would reach most of the tree and hold its wavefront far longer than the
+// others. Called by every thread of the workgroup; q->n is zero on entry.
 template <class W>
 VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int64_t e, BatchOut const &o,
-                            int nthreads)
+                            int nthreads, TessQueue *q)
 {
+  const bool all = o.contains && o.inside && o.sin && o.sout && o.din && o.dout;
+  if (all) {
+    const double deep = tess::deep_sq(s);
+    for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
+      const Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
+                           __builtin_nontemporal_load(o.pz + i));
+      const Vec3<double> d(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
+                           __builtin_nontemporal_load(o.dz + i));
+      bool c, toIn;
+      int in;
+      double si, so, di, dou, seed, lim;
+      if (tess::query_all_or_defer(s, w, p, d, c, in, si, so, di, dou, deep, seed, lim, toIn)) {
+        const int k = atomicAdd(&q->n, 1);
+        if (k < kTessQueue) {
+          q->slot[k] = int(i - b) | (toIn ? 1 << 30 : 0);
+          q->seed[k] = seed;
+          q->lim[k]  = lim;
+        } else { // queue full: the lane searches itself
+          const double sq = vmin(seed, tess::safety_sq_threaded(s, w, p, toIn, lim));
+          __builtin_nontemporal_store(toIn ? vsqrt(sq) : -1., o.sin + i);
+          __builtin_nontemporal_store(toIn ? -1. : vsqrt(sq), o.sout + i);
+        }
+      } else {
+        __builtin_nontemporal_store(si, o.sin + i);
+        __builtin_nontemporal_store(so, o.sout + i);
+      }
+      __builtin_nontemporal_store(uint8_t(c), o.contains + i);
+      __builtin_nontemporal_store(int32_t(in), o.inside + i);
+      __builtin_nontemporal_store(di, o.din + i);
+      __builtin_nontemporal_store(dou, o.dout + i);
+    }
+    __syncthreads();
+    const int nq = vmin(q->n, kTessQueue);
+    __syncthreads(); // every wave holds nq before q is reset for the next tile
+    const int wave = threadIdx.x / 64, nwaves = nthreads / 64;
+    for (int t = wave; t < nq; t += nwaves) {
+      const int sl      = q->slot[t];
+      const int64_t i   = b + (sl & ((1 << 30) - 1));
+      const bool toIn   = (sl >> 30) & 1;
+      const Vec3<double> p(o.px[i], o.py[i], o.pz[i]);
+      const double sq   = vmin(q->seed[t], tess::safety_sq_wave(s, w, p, toIn, q->lim[t]));
+      if (__lane_id() == 0) {
+        o.sin[i]  = toIn ? vsqrt(sq) : -1.;
+        o.sout[i] = toIn ? -1. : vsqrt(sq);
+      }
+    }
+    return;
+  }
+  // a partial output set: the methods one by one
   for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
     const Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
                          __builtin_nontemporal_load(o.pz + i));
-    if (!(o.contains && o.inside && o.sin && o.sout && o.din && o.dout)) {
-      // a partial output set: the methods one by one
-      if (o.contains) __builtin_nontemporal_store(uint8_t(tess::contains64(s, w, p)), o.contains + i);
-      if (o.inside) __builtin_nontemporal_store(int32_t(tess::inside64(s, w, p)), o.inside + i);
-      if (o.sin) __builtin_nontemporal_store(tess::safety_in64(s, w, p), o.sin + i);
-      if (o.sout) __builtin_nontemporal_store(tess::safety_out64(s, w, p), o.sout + i);
-      if (o.din && o.dout) {
-        const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
-        double din, dout;
-        tess::dist_both64(s, w, p, d, din, dout);
-        __builtin_nontemporal_store(din, o.din + i);
-        __builtin_nontemporal_store(dout, o.dout + i);
-      } else if (o.din || o.dout) {
-        const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
-        if (o.din) __builtin_nontemporal_store(tess::dist_in64(s, w, p, d), o.din + i);
-        if (o.dout) __builtin_nontemporal_store(tess::dist_out64(s, w, p, d), o.dout + i);
-      }
-      continue;
+    if (o.contains) __builtin_nontemporal_store(uint8_t(tess::contains64(s, w, p)), o.contains + i);
+    if (o.inside) __builtin_nontemporal_store(int32_t(tess::inside64(s, w, p)), o.inside + i);
+    if (o.sin) __builtin_nontemporal_store(tess::safety_in64(s, w, p), o.sin + i);
+    if (o.sout) __builtin_nontemporal_store(tess::safety_out64(s, w, p), o.sout + i);
+    if (o.din && o.dout) {
+      const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
+      double din, dout;
+      tess::dist_both64(s, w, p, d, din, dout);
+      __builtin_nontemporal_store(din, o.din + i);
+      __builtin_nontemporal_store(dout, o.dout + i);
+    } else if (o.din || o.dout) {
+      const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
+      if (o.din) __builtin_nontemporal_store(tess::dist_in64(s, w, p, d), o.din + i);
+      if (o.dout) __builtin_nontemporal_store(tess::dist_out64(s, w, p, d), o.dout + i);
     }
-    const Vec3<double> d(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
-                         __builtin_nontemporal_load(o.dz + i));
-    bool c;
-    int in;
-    double si, so, di, dou;
-    tess::query_all(s, w, p, d, c, in, si, so, di, dou);
-    __builtin_nontemporal_store(uint8_t(c), o.contains + i);
-    __builtin_nontemporal_store(int32_t(in), o.inside + i);
-    __builtin_nontemporal_store(si, o.sin + i);
-    __builtin_nontemporal_store(so, o.sout + i);
-    __builtin_nontemporal_store(di, o.din + i);
-    __builtin_nontemporal_store(dou, o.dout + i);
   }
 }
 
@@ -99,11 +145,11 @@ VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int
 // results are the pool walk's bit for bit.
 template <int BLOCK>
 VGMI_DEV void tess_tile(TessellatedData const &s, int si, Tile const &tile, BatchOut const &o, double *lds,
-                        int ldsDoubles, int *staged)
+                        int ldsDoubles, int *staged, TessQueue *q)
 {
   const int n = s.imageDoubles;
   if (n == 0 || n > ldsDoubles) {
-    run_tile_tess(s, tess::pool_walk(s), tile.begin, tile.end, o, BLOCK);
+    run_tile_tess(s, tess::pool_walk(s), tile.begin, tile.end, o, BLOCK, q);
     return;
   }
   if (*staged != si) { // block-uniform: written after the barrier that ends the copy
@@ -115,7 +161,7 @@ VGMI_DEV void tess_tile(TessellatedData const &s, int si, Tile const &tile, Batc
     __syncthreads();
     if (threadIdx.x == 0) *staged = si;
   }
-  run_tile_tess(s, tess::lds_walk(s, lds), tile.begin, tile.end, o, BLOCK);
+  run_tile_tess(s, tess::lds_walk(s, lds), tile.begin, tile.end, o, BLOCK, q);
 }
 
 __global__ void k_batch_reset(uint32_t *counter) { *counter = 0u; }
@@ -126,9 +172,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__
 {
   extern __shared__ __align__(16) double lds[];
   __shared__ int s_tile, s_staged;
+  __shared__ TessQueue s_queue;
   if (threadIdx.x == 0) s_staged = -1;
   while (true) {
-    if (threadIdx.x == 0) s_tile = int(atomicAdd(counter, 1u));
+    if (threadIdx.x == 0) {
+      s_tile       = int(atomicAdd(counter, 1u));
+      s_queue.n    = 0;
+    }
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_tile);
     __syncthreads();
@@ -137,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__
     const int si           = __builtin_amdgcn_readfirstlane(tile.shape);
     ShapeRecord const &rec = recs[si];
     if (rec.type == kTessellated) {
-      tess_tile<BLOCK>(rec.u.tess, si, tile, o, lds, ldsDoubles, &s_staged);
+      tess_tile<BLOCK>(rec.u.tess, si, tile, o, lds, ldsDoubles, &s_staged, &s_queue);
       continue;
     }
     switch (rec.type) {
@@ -194,10 +244,12 @@ struct vgmi_batch {
 };
 
 // Dynamic LDS a workgroup may use for a tessellated solid's image: at most what keeps two
-// workgroups on a CU (160 KiB of LDS per CU, less the kernel's 16 bytes of static LDS and the
-// allocation granule); a solid whose image is larger is walked in the pool.
-constexpr int kLdsPerCU   = 160 * 1024;
-constexpr int kTessLdsMax = kLdsPerCU / 2 - 1024;
+// workgroups on a CU (160 KiB of LDS per CU, less the kernel's static LDS -- tile index, staged
+// solid, queue of deferred safety searches -- and the allocation granule); a solid whose image is
+// larger is walked in the pool.
+constexpr int kLdsPerCU     = 160 * 1024;
+constexpr int kStaticLds    = int(sizeof(TessQueue)) + 16 + 512;
+constexpr int kTessLdsMax   = kLdsPerCU / 2 - kStaticLds;
 
 static int tess_lds_bytes(TessellatedData const &t) { return int(sizeof(double)) * t.imageDoubles; }
 
@@ -243,8 +295,8 @@ static vgmi_status batch_launch(vgmi_batch *b, BatchOut const &o, hipStream_t st
   if (b->ntiles == 0) return VGMI_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, st, b->d_counter);
   // resident workgroups per CU are bounded by the dynamic LDS too
-  const int perCU = b->ldsBytes ? std::max(1, std::min(b->blocksPerCU, kLdsPerCU / (b->ldsBytes + 1024)))
-                                : b->blocksPerCU;
+  const int perCU =
+      b->ldsBytes ? std::max(1, std::min(b->blocksPerCU, kLdsPerCU / (b->ldsBytes + kStaticLds))) : b->blocksPerCU;
   const int grid  = std::max(1, std::min(b->ntiles, b->numCU * perCU));
   const size_t lds = size_t(b->ldsBytes);
   const int ldsD   = b->ldsBytes / int(sizeof(double));
@@ -340,9 +392,10 @@ vgmi_status vgmi_batch_destroy(vgmi_batch *b)
 vgmi_status vgmi_batch_set_tuning(vgmi_batch *b, int tile_points, int block, int blocks_per_cu)
 {
   if (!b) return set_error(VGMI_ERR_INVALID, "vgmi_batch_set_tuning: null batch");
-  if (tile_points < 64 || (block != 64 && block != 128 && block != 256 && block != 512) || blocks_per_cu < 1 ||
-      blocks_per_cu > 64)
-    return set_error(VGMI_ERR_INVALID, "vgmi_batch_set_tuning: tile >= 64, block in {64,128,256,512}, 1..64 blocks/CU");
+  if (tile_points < 64 || tile_points > (1 << 24) || (block != 64 && block != 128 && block != 256 && block != 512) ||
+      blocks_per_cu < 1 || blocks_per_cu > 64)
+    return set_error(VGMI_ERR_INVALID,
+                     "vgmi_batch_set_tuning: tile in [64, 2^24], block in {64,128,256,512}, 1..64 blocks/CU");
   b->tile        = tile_points;
   b->block       = block;
   b->blocksPerCU = blocks_per_cu;

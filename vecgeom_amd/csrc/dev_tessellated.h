@@ -366,13 +366,13 @@ VGMI_DEV double safety_sq_threaded(TessellatedData const &s, W const &w, V3d con
 }
 
 // The facets of the leaf reached by always descending into the nearer child give a first minimum
-// (an upper bound of the result), then the threaded walk prunes against it.
+// (an upper bound of the result) that the full search prunes against.
 template <class W>
-VGMI_DEV double safety_sq_seeded(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double bound)
+VGMI_DEV double safety_sq_seed(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double bound)
 {
   double seed = kInfLength;
   int j       = 0;
-  NodeRec nd = w.node(0);
+  NodeRec nd  = w.node(0);
   for (;;) {
     if (nd.count > 0) {
       leaf_safety_sq(s, w, nd, p, toIn, bound, seed);
@@ -383,7 +383,37 @@ VGMI_DEV double safety_sq_seeded(TessellatedData const &s, W const &w, V3d const
     j               = left ? j + 1 : l.escape;
     nd              = left ? l : r;
   }
+  return seed;
+}
+
+template <class W>
+VGMI_DEV double safety_sq_seeded(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double bound)
+{
+  const double seed = safety_sq_seed(s, w, p, toIn, bound);
   return vmin(seed, safety_sq_threaded(s, w, p, toIn, vmin(seed, bound)));
+}
+
+// The smallest per-facet squared safety of p, computed by the 64 lanes of a wavefront together
+// (lane l takes facets l, l + 64, ...; every lane must call it with the same p, toIn and lim);
+// every lane returns it. `lim` bounds it from above (facets whose plane lies beyond are skipped).
+template <class W>
+VGMI_DEV double safety_sq_wave(TessellatedData const &s, W const &w, V3d const &p, bool toIn, double lim)
+{
+  double best = kInfLength;
+  for (int k = int(__lane_id()); k < s.nfacets; k += 64)
+    facet_safety_sq(w.plane(k), facet(s, k), p, toIn, vmin(best, lim), best);
+  for (int off = 32; off > 0; off >>= 1) best = vmin(best, __shfl_xor(best, off));
+  return best;
+}
+
+// Squared seeded minimum beyond which a point counts as deep inside or far from the solid: the
+// walk's boxes bound facet distances poorly there, so such a point's search is left to a wavefront
+// (a quarter of the solid's smallest extent).
+VGMI_DEV double deep_sq(TessellatedData const &s)
+{
+  const double e =
+      0.25 * vmin(s.maxExt[0] - s.minExt[0], vmin(s.maxExt[1] - s.minExt[1], s.maxExt[2] - s.minExt[2]));
+  return e * e;
 }
 
 template <class W>
@@ -434,6 +464,43 @@ VGMI_DEV void query_all(TessellatedData const &s, W const &w, V3d const &p, V3d 
     sin  = vsqrt(safety_sq(s, w, p, true, safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength)));
   }
   dist_both64(s, w, p, d, din, dout);
+}
+
+// query_all, except that a point whose seeded safety minimum exceeds `deep` returns true with
+// sin and sout unset and its search left to the caller: the result is then vmin(seed, the smallest
+// per-facet squared safety below lim) on the side toIn (safety_sq_wave computes it).
+template <class W>
+VGMI_DEV bool query_all_or_defer(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool &contains,
+                                 int &inside, double &sin, double &sout, double &din, double &dout, double deep,
+                                 double &seed, double &lim, bool &toIn)
+{
+  const RayResult r = test_ray(s, w, p);
+  const V3d t       = at3(s.testDir);
+  int sOut          = r.sOut;
+  double dOut       = r.dOut;
+  if (sOut >= 0 && r.sIn >= 0 && r.dIn < dOut && r.dIn * dot_normal(s, r.sIn, t) < -kTolerance) sOut = -1, dOut = -1.;
+  contains = sOut >= 0;
+  if (sOut < 0)
+    inside = kOutside;
+  else if (dOut < 0 || dOut * dot_normal(s, sOut, t) < kTolerance)
+    inside = kSurface;
+  else if (r.sIn < 0 || dOut < r.dIn)
+    inside = kInside;
+  else if (r.dIn < 0 || r.dIn * dot_normal(s, r.sIn, t) > -kTolerance)
+    inside = kSurface;
+  else
+    inside = kOutside;
+  toIn = !contains;
+  const double bound =
+      contains ? safety_bound(s, p, r.dOut) : safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength);
+  seed = safety_sq_seed(s, w, p, toIn, bound);
+  lim  = vmin(seed, bound);
+  dist_both64(s, w, p, d, din, dout);
+  if (lim > deep) return true;
+  const double sq = vmin(seed, safety_sq_threaded(s, w, p, toIn, lim));
+  sin             = contains ? -1. : vsqrt(sq);
+  sout            = contains ? vsqrt(sq) : -1.;
+  return false;
 }
 
 template <class W>
