@@ -83,13 +83,17 @@ def host_cores():
         return max(1, min(16, os.cpu_count() or 1))
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, prefix=None):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (profiles/traffic.json, scaled by a plain-copy calibration), or None."""
+    (profiles/traffic.json, scaled by a plain-copy calibration), or None. With `prefix`, an entry of
+    another instantiation of the same kernel (another block size) stands in when `kernel` has none."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            k = json.load(f)["kernels"].get(kernel)
-        return None if k is None else {"bytes": k["traffic_bytes"], "workload": k["workload"],
+            kernels = json.load(f)["kernels"]
+        name = kernel if kernel in kernels or prefix is None else next(
+            (n for n in kernels if n.startswith(prefix)), kernel)
+        k = kernels.get(name)
+        return None if k is None else {"bytes": k["traffic_bytes"], "workload": k["workload"], "kernel": name,
                                        "source": "profiles/pmc_hbm_traffic.md"}
     except (OSError, ValueError, KeyError):
         return None
@@ -157,7 +161,7 @@ def run_mix(args):
                                          if k not in {s for s, _ in shapes}],
                       "tuning": tuning or batch.info()},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("vgmi::k_batch<256>"),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("vgmi::k_batch<%d>" % batch.info()["block"], "vgmi::k_batch<"),
                         "kernel": "k_batch",
                         "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)}}
     if not args.no_cpu:
