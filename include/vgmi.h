@@ -118,7 +118,10 @@ typedef struct vgmi_batch vgmi_batch;
 VGMI_API vgmi_status vgmi_batch_create(const vgmi_shape *const *shapes, const int64_t *counts, int nshapes,
                                        vgmi_batch **out);
 VGMI_API vgmi_status vgmi_batch_destroy(vgmi_batch *batch);
-/* Launch shape: points per tile, threads per block (64/128/256/512), resident blocks per CU. */
+/* Launch shape: points per tile (64 .. 2^24), threads per block (64/128/256/512), resident blocks per
+ * CU. A batch holding a tessellated solid whose LDS image (nodes, planes, candidate boxes; about
+ * 72 B per facet) fits in 76 KiB reserves that much dynamic LDS per workgroup, which holds the
+ * launch to two resident workgroups per CU. */
 VGMI_API vgmi_status vgmi_batch_set_tuning(vgmi_batch *batch, int tile_points, int block, int blocks_per_cu);
 /* info = nshapes, total points, tiles, tile points, block, blocks per CU */
 VGMI_API vgmi_status vgmi_batch_info(const vgmi_batch *batch, int64_t info[6]);
