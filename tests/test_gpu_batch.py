@@ -133,3 +133,48 @@ def test_batch_tessellated_deep_points():
         out = b.query_all(P, D)
         for m in METHODS:
             assert np.array_equal(ref[m], out[m].cpu().numpy()), (m, tuning)
+
+
+def l_prism_triangles(a=10.0, h=5.0):
+    """Tessellated L-shaped prism (non-convex): the polygon (0,0) (2a,0) (2a,a) (a,a) (a,2a) (0,2a),
+    star-shaped from (0,0) so its faces are fans from there, extruded over z in [-h, h]; vertices
+    anticlockwise seen from outside."""
+    poly = [(0.0, 0.0), (2 * a, 0.0), (2 * a, a), (a, a), (a, 2 * a), (0.0, 2 * a)]
+    tris = []
+    for i in range(1, len(poly) - 1):
+        (x0, y0), (x1, y1), (x2, y2) = poly[0], poly[i], poly[i + 1]
+        tris.append([(x0, y0, h), (x1, y1, h), (x2, y2, h)])
+        tris.append([(x0, y0, -h), (x2, y2, -h), (x1, y1, -h)])
+    for i in range(len(poly)):
+        (ax, ay), (bx, by) = poly[i], poly[(i + 1) % len(poly)]
+        tris.append([(ax, ay, -h), (bx, by, -h), (bx, by, h)])
+        tris.append([(ax, ay, -h), (bx, by, h), (ax, ay, h)])
+    return [float(len(tris))] + [float(c) for t in tris for v in t for c in v]
+
+
+def test_batch_tessellated_non_convex():
+    """A non-convex tessellated solid (L-shaped prism, walked in LDS, deep points queued for
+    wavefronts) next to the convex hull: every method equals the oracle bit for bit."""
+    shapes = [("tessellated", l_prism_triangles()), ("tessellated", workloads.convex_hull_sphere(20.0))]
+    counts = [6000, 2000]
+    gpu = [make_shape(k, p) for k, p in shapes]
+    orc = [OracleShape(k, p) for k, p in shapes]
+    exts = [o.extent() for o in orc]
+    p, d = workloads.mix_inputs(exts, counts, [lambda x, o=o: o.query("Contains", x) for o in orc],
+                                [lambda x, y, o=o: o.query("DistanceToOut", x, y) for o in orc])
+    P, D = SOA3D.from_array(p), SOA3D.from_array(d)
+    refs, off = [], 0
+    for o, n in zip(orc, counts):
+        ps, dsl = np.ascontiguousarray(p[:, off:off + n]), np.ascontiguousarray(d[:, off:off + n])
+        refs.append({m: o.query(m, ps, dsl if m.startswith("Distance") else None) for m in METHODS})
+        off += n
+    assert refs[0]["Contains"].any() and not refs[0]["Contains"].all()
+    b = ShapeBatch(gpu, counts)
+    for tuning in [(512, 512, 2), (256, 256, 4), (64, 64, 1)]:
+        b.set_tuning(*tuning)
+        out = b.query_all(P, D)
+        off = 0
+        for ref, n in zip(refs, counts):
+            for m in METHODS:
+                assert np.array_equal(ref[m], out[m][off:off + n].cpu().numpy()), (m, tuning, off)
+            off += n
