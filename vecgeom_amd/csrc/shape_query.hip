@@ -8,8 +8,7 @@
 // runs the same kernel in FP32 and re-evaluates every point whose FP32 result
 // is within the near-surface band in FP64, so decisions stay exact.
 #include "vgmi_internal.h"
-#include "dev_shapes.h"
-#include <type_traits>
+#include "dev_tess_tile.h"
 
 namespace vgmi {
 
@@ -69,23 +68,6 @@ __global__ __launch_bounds__(kBlock) void k_query_all_fp64(Data s, QueryAllArgs 
   size_t strid = size_t(gridDim.x) * kBlock;
   for (; i < a.n; i += strid) {
     Vec3<double> p = load3<double>(a.px, a.py, a.pz, i);
-    if constexpr (std::is_same<Data, TessellatedData>::value) {
-      // one test-ray walk, one safety walk and one walk along d serve all six methods
-      if (a.contains && a.inside && a.sin && a.sout && a.din && a.dout) {
-        const Vec3<double> d = load3<double>(a.dx, a.dy, a.dz, i);
-        bool c;
-        int in;
-        double si, so, di, dou;
-        tess::query_all(s, tess::pool_walk(s), p, d, c, in, si, so, di, dou);
-        __builtin_nontemporal_store(uint8_t(c), a.contains + i);
-        __builtin_nontemporal_store(int32_t(in), a.inside + i);
-        __builtin_nontemporal_store(si, a.sin + i);
-        __builtin_nontemporal_store(so, a.sout + i);
-        __builtin_nontemporal_store(di, a.din + i);
-        __builtin_nontemporal_store(dou, a.dout + i);
-        continue;
-      }
-    }
     if (a.contains) __builtin_nontemporal_store(uint8_t(S::template Contains<double>(s, p)), a.contains + i);
     if (a.inside) __builtin_nontemporal_store(int32_t(S::template Inside<double>(s, p)), a.inside + i);
     if (a.sin) __builtin_nontemporal_store(S::template SafeIn<double>(s, p), a.sin + i);
@@ -96,6 +78,38 @@ __global__ __launch_bounds__(kBlock) void k_query_all_fp64(Data s, QueryAllArgs 
       if (a.dout) __builtin_nontemporal_store(S::template DistOut<double>(s, p, d), a.dout + i);
     }
   }
+}
+
+// A tessellated solid: tiles of kBlock points, the solid's image in LDS when it fits (copied once
+// per workgroup), one test-ray walk, one safety walk and one walk along d shared by the requested
+// methods, deep points' safety searches run by whole wavefronts (the batched launch's tile code).
+__global__ __launch_bounds__(kBlock) void k_tess_query(TessellatedData s, BatchOut o, int64_t n, int useLds)
+{
+  extern __shared__ __align__(16) double lds[];
+  __shared__ TessQueue q;
+  if (useLds) stage_tess_image(s, lds, kBlock);
+  for (int64_t t = blockIdx.x; t * kBlock < n; t += gridDim.x) {
+    if (threadIdx.x == 0) q.n = 0;
+    __syncthreads();
+    const int64_t b = t * kBlock, e = b + kBlock < n ? b + kBlock : n;
+    if (useLds)
+      run_tile_tess(s, tess::lds_walk(s, lds), b, e, o, kBlock, &q);
+    else
+      run_tile_tess(s, tess::pool_walk(s), b, e, o, kBlock, &q);
+    __syncthreads();
+  }
+}
+
+static hipError_t launch_tess(TessellatedData const &s, BatchOut const &o, size_t n, hipStream_t st)
+{
+  const bool useLds = s.imageDoubles > 0 && s.imageDoubles * int(sizeof(double)) <= kTessLdsMax;
+  int dev = 0, ncu = 256;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t tiles = (n + kBlock - 1) / kBlock, cap = size_t(ncu) * (useLds ? 2 : 8);
+  const size_t lds   = useLds ? size_t(s.imageDoubles) * sizeof(double) : 0;
+  hipLaunchKernelGGL(k_tess_query, dim3(unsigned(tiles < cap ? tiles : cap)), dim3(kBlock), lds, st, s, o, int64_t(n),
+                     int(useLds));
+  return hipGetLastError();
 }
 
 static unsigned grid_for(size_t n)
@@ -157,11 +171,24 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
   switch (shape->rec.type) {
+  case kTessellated: {
+    BatchOut o{px, py, pz, dx, dy, dz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    switch (op) {
+    case kOpContains: o.contains = (uint8_t *)out; break;
+    case kOpInside: o.inside = (int32_t *)out; break;
+    case kOpDistIn: o.din = (double *)out; break;
+    case kOpDistOut: o.dout = (double *)out; break;
+    case kOpSafetyIn: o.sin = (double *)out; break;
+    default: o.sout = (double *)out; break;
+    }
+    e = launch_tess(shape->rec.u.tess, o, n, st);
+    break;
+  }
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
     e = launch_query(shape->rec.u.M, op, a, st);                                                                     \
     break;
-    VGMI_FOR_EACH_SHAPE(X)
+    VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)
 #undef X
   default:
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query: shape type %d has no kernel", shape->rec.type);
@@ -186,11 +213,16 @@ vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision, const d
   hipStream_t st = (hipStream_t)stream;
   dim3 g(grid_for(n)), b(kBlock);
   switch (shape->rec.type) {
+  case kTessellated: {
+    const BatchOut o{px, py, pz, dx, dy, dz, contains, inside, dist_in, dist_out, safety_in, safety_out};
+    (void)launch_tess(shape->rec.u.tess, o, n, st);
+    break;
+  }
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
     hipLaunchKernelGGL((k_query_all_fp64<T>), g, b, 0, st, shape->rec.u.M, a);                                      \
     break;
-    VGMI_FOR_EACH_SHAPE(X)
+    VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)
 #undef X
   default:
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_shape_query_all: shape type %d has no kernel", shape->rec.type);
