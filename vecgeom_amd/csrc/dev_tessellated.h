@@ -434,38 +434,6 @@ VGMI_DEV double safety_bound(TessellatedData const &s, V3d const &p, double tHit
   return b * b * (1. + 1e-9);
 }
 
-// All six methods of one point with one test-ray walk (Contains, Inside and both safeties),
-// one safety walk (the other safety is -1) and one walk along d for both distances.
-template <class W>
-VGMI_DEV void query_all(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool &contains,
-                        int &inside, double &sin, double &sout, double &din, double &dout)
-{
-  const RayResult r = test_ray(s, w, p);
-  const V3d t       = at3(s.testDir);
-  int sOut          = r.sOut;
-  double dOut       = r.dOut;
-  if (sOut >= 0 && r.sIn >= 0 && r.dIn < dOut && r.dIn * dot_normal(s, r.sIn, t) < -kTolerance) sOut = -1, dOut = -1.;
-  contains = sOut >= 0;
-  if (sOut < 0)
-    inside = kOutside;
-  else if (dOut < 0 || dOut * dot_normal(s, sOut, t) < kTolerance)
-    inside = kSurface;
-  else if (r.sIn < 0 || dOut < r.dIn)
-    inside = kInside;
-  else if (r.dIn < 0 || r.dIn * dot_normal(s, r.sIn, t) > -kTolerance)
-    inside = kSurface;
-  else
-    inside = kOutside;
-  if (contains) {
-    sin  = -1.;
-    sout = vsqrt(safety_sq(s, w, p, false, safety_bound(s, p, r.dOut)));
-  } else {
-    sout = -1.;
-    sin  = vsqrt(safety_sq(s, w, p, true, safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength)));
-  }
-  dist_both64(s, w, p, d, din, dout);
-}
-
 // query_all, except that a point whose seeded safety minimum exceeds `deep` returns true with
 // sin and sout unset and its search left to the caller: the result is then vmin(seed, the smallest
 // per-facet squared safety below lim) on the side toIn (safety_sq_wave computes it).
@@ -501,6 +469,17 @@ VGMI_DEV bool query_all_or_defer(TessellatedData const &s, W const &w, V3d const
   sin             = contains ? -1. : vsqrt(sq);
   sout            = contains ? vsqrt(sq) : -1.;
   return false;
+}
+
+// All six methods of one point with one test-ray walk (Contains, Inside and both safeties),
+// one safety walk (the other safety is -1) and one walk along d for both distances.
+template <class W>
+VGMI_DEV void query_all(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool &contains,
+                        int &inside, double &sin, double &sout, double &din, double &dout)
+{
+  double seed, lim;
+  bool toIn;
+  (void)query_all_or_defer(s, w, p, d, contains, inside, sin, sout, din, dout, __builtin_inf(), seed, lim, toIn);
 }
 
 template <class W>
