@@ -45,3 +45,18 @@ def test_mixed_decisions_exact_and_errors_bounded(kind, params):
                                                                  np.percentile(err, 99) if err.size else 0, flips))
         assert flips <= max(2, len(ref) // 5000), (m, flips)
         assert np.count_nonzero(err > bound) <= max(2, len(ref) // 5000), (m, float(err.max(initial=0)))
+
+
+def test_mixed_tessellated_is_fp64():
+    """The tessellated solid has no FP32 pass: a MIXED query returns the FP64 results bit for bit."""
+    from vecgeom_amd import FP64, MIXED, make_shape
+    from vecgeom_amd import workloads
+    params = workloads.convex_hull_sphere(20.0)
+    o, p, d = gpu_util.inputs_for("tessellated", params, 1 << 13, seed=5)
+    s = make_shape("tessellated", params)
+    P, D = gpu_util.to_dev(p), gpu_util.to_dev(d)
+    for m in ("Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"):
+        args = (P, D) if m.startswith("Distance") else (P,)
+        a = getattr(s, m)(*args, precision=FP64)
+        b = getattr(s, m)(*args, precision=MIXED)
+        assert a.equal(b), m

@@ -166,7 +166,9 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
   if (!px || !py || !pz || !out) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: null point/output pointer");
   if ((op == kOpDistIn || op == kOpDistOut) && (!dx || !dy || !dz))
     return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: DistanceToIn/Out need directions");
-  if (precision == VGMI_MIXED) return mixed_shape_query(shape, op, px, py, pz, dx, dy, dz, out, n, stream);
+  // the tessellated solid has no FP32 pass (it answers every point in FP64), so MIXED takes the FP64 launch
+  if (precision == VGMI_MIXED && shape->rec.type != kTessellated)
+    return mixed_shape_query(shape, op, px, py, pz, dx, dy, dz, out, n, stream);
   QueryArgs a{px, py, pz, dx, dy, dz, out, n};
   hipStream_t st = (hipStream_t)stream;
   hipError_t e;
