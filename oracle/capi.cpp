@@ -4,7 +4,9 @@
 // Shape parameter lists (same order as the reference constructors):
 //   0 box         dx dy dz                                   (volumes/UnplacedBox.h)
 //   1 tube        rmin rmax dz sphi dphi                     (volumes/UnplacedTube.h:29)
-//   5 trapezoid   dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2 (volumes/UnplacedTrapezoid.h:56)
+//   5 trapezoid   dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2 (volumes/UnplacedTrapezoid.h:56);
+//                 also the reduced forms of 3, 4, 5, 6 parameters and 8 corners (24 doubles, the
+//                 TrapCorners order of source/UnplacedTrapezoid.cpp:447-465)
 //  10 hype        rmin rmax stIn stOut dz                    (volumes/UnplacedHype.h:56)
 //   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
 //                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
@@ -16,6 +18,43 @@
 using namespace vgo;
 
 namespace vgo {
+// dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2 of the reduced trapezoid constructors:
+//  3: xbox ybox zbox (source/UnplacedTrapezoid.cpp:176-190)
+//  4: dx1 dx2 dy dz (:148-154)
+//  5: dx1 dx2 dy1 dy2 dz (:156-162)
+//  6: dx dy dz alpha theta phi (:164-174)
+// 11: the full list (volumes/UnplacedTrapezoid.h:56-63)
+bool TrapezoidFullParams(const double *a, int n, double f[11])
+{
+  switch (n) {
+  case 3: {
+    double g[11] = {a[2], 0., 0., a[1], a[0], a[0], 0., a[1], a[0], a[0], 0.};
+    for (int i = 0; i < 11; ++i) f[i] = g[i];
+    return true;
+  }
+  case 4: {
+    double g[11] = {a[3], 0., 0., a[2], a[0], a[0], 0., a[2], a[1], a[1], 0.};
+    for (int i = 0; i < 11; ++i) f[i] = g[i];
+    return true;
+  }
+  case 5: {
+    double g[11] = {a[4], 0., 0., a[2], a[0], a[0], 0., a[3], a[1], a[1], 0.};
+    for (int i = 0; i < 11; ++i) f[i] = g[i];
+    return true;
+  }
+  case 6: {
+    double g[11] = {a[2], a[4], a[5], a[1], a[0], a[0], a[3], a[1], a[0], a[0], a[3]};
+    for (int i = 0; i < 11; ++i) f[i] = g[i];
+    return true;
+  }
+  case 11:
+    for (int i = 0; i < 11; ++i) f[i] = a[i];
+    return true;
+  default:
+    return false;
+  }
+}
+
 Shape *MakeShape(int type, const double *a, int n)
 {
   switch (type) {
@@ -56,8 +95,18 @@ Shape *MakeShape(int type, const double *a, int n)
     }
     return t;
   }
-  case 5:
-    return n >= 11 ? new Trapezoid(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]) : nullptr;
+  case 5: {
+    // the UnplacedTrapezoid constructors by parameter count (volumes/UnplacedTrapezoid.h:56-102,
+    // source/UnplacedTrapezoid.cpp:129-190); all but the corner form reduce to the full one
+    double f[11];
+    if (!TrapezoidFullParams(a, n, f)) {
+      if (n != 24) return nullptr;
+      V3 pt[8];
+      for (int i = 0; i < 8; ++i) pt[i] = V3(a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+      return new Trapezoid(pt);
+    }
+    return new Trapezoid(f[0], f[1], f[2], f[3], f[4], f[5], f[6], f[7], f[8], f[9], f[10]);
+  }
   default:
     return nullptr;
   }

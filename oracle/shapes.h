@@ -127,6 +127,8 @@ public:
   double A[4], B[4], C[4], D[4];
   Trapezoid(double dz, double theta, double phi, double dy1, double dx1, double dx2, double alpha1, double dy2,
             double dx3, double dx4, double alpha2);
+  explicit Trapezoid(V3 const pt[8]);
+  void SetPlanes(V3 const pt[8]);
   bool Contains(V3 const &p) const override;
   int Inside(V3 const &p) const override;
   double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;

@@ -23,7 +23,33 @@ Trapezoid::Trapezoid(double dz_, double theta_, double phi_, double dy1_, double
   pt[5] = V3(+dxdzDz - dxdyDy2 + dx3, +dydzDz - dy2, +dz);
   pt[6] = V3(+dxdzDz + dxdyDy2 - dx4, +dydzDz + dy2, +dz);
   pt[7] = V3(+dxdzDz + dxdyDy2 + dx4, +dydzDz + dy2, +dz);
-  // source/UnplacedTrapezoid.cpp:630-666 and MakeAPlane :575-619
+  SetPlanes(pt);
+}
+
+// UnplacedTrapezoid(TrapCorners): source/UnplacedTrapezoid.cpp:129-134 → fromCornersToParameters :496-523
+// (parameters recovered from the corners), then the side planes of the given corners (MakePlanes(pt)).
+Trapezoid::Trapezoid(V3 const pt[8])
+{
+  dz             = pt[7].z;
+  double DzRecip = 1.0 / dz;
+  dy1            = 0.50 * (pt[2].y - pt[0].y);
+  dx1            = 0.50 * (pt[1].x - pt[0].x);
+  dx2            = 0.50 * (pt[3].x - pt[2].x);
+  tanAlpha1      = 0.25 * (pt[2].x + pt[3].x - pt[1].x - pt[0].x) / dy1;
+  dy2            = 0.50 * (pt[6].y - pt[4].y);
+  dx3            = 0.50 * (pt[5].x - pt[4].x);
+  dx4            = 0.50 * (pt[7].x - pt[6].x);
+  tanAlpha2      = 0.25 * (pt[6].x + pt[7].x - pt[5].x - pt[4].x) / dy2;
+  tthetaCphi     = (pt[4].x + dy2 * tanAlpha2 + dx3) * DzRecip;
+  tthetaSphi     = (pt[4].y + dy2) * DzRecip;
+  theta          = std::atan(std::sqrt(tthetaSphi * tthetaSphi + tthetaCphi * tthetaCphi));
+  phi            = std::atan2(tthetaSphi, tthetaCphi);
+  SetPlanes(pt);
+}
+
+// source/UnplacedTrapezoid.cpp:630-666 and MakeAPlane :575-619
+void Trapezoid::SetPlanes(V3 const pt[8])
+{
   auto plane = [&](int i, V3 const &p1, V3 const &p2, V3 const &p3, V3 const &p4) {
     V3 n       = (p3 - p1).Cross(p4 - p2);
     double inv = 1. / n.Mag();

@@ -55,52 +55,107 @@ def _norm_expr(s):
 
 
 CURRENT_VECS = {}
+_VEC_CTOR = re.compile(r"(?:Vec_t|Vector3D\s*<\s*\w+\s*>|Vec3D_t)\s*\(")
 
 
-def eval_scalar(expr, scalars):
+def _dot(a, b):
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]  # Vector3D::Dot, base/Vector3D.h
+
+
+def eval_expr(expr, scalars, vecs=None):
+    """Value of a C++ arithmetic expression over scalars and Vector3D values (lists of 3).
+    Vector operations follow base/Vector3D.h: componentwise +, -, scalar * and /; Mag2 = Dot(v, v);
+    Mag = sqrt(Mag2); Unit = v / sqrt(Mag2 + kMinimum); Normalized = v * (1 / Mag)."""
+    vecs = CURRENT_VECS if vecs is None else vecs
     expr = _norm_expr(expr)
-    # vector components of known vectors: name.x() / name.y() / name.z()
-    def comp(m):
-        v = CURRENT_VECS.get(m.group(1))
-        if v is None:
-            raise Unsupported(m.group(0))
-        return repr(v["xyz".index(m.group(2))])
-    expr = re.sub(r"\b(\w+)\.([xyz])\(\)", comp, expr)
+    expr = _VEC_CTOR.sub("__V(", expr)
     try:
         tree = ast.parse(expr, mode="eval")
     except SyntaxError as e:
         raise Unsupported(expr) from e
 
+    def isvec(v):
+        return isinstance(v, list)
+
     def ev(n):
         if isinstance(n, ast.Expression):
             return ev(n.body)
-        if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)):
+        if isinstance(n, ast.Constant) and isinstance(n.value, (int, float)) and not isinstance(n.value, bool):
             return float(n.value)
         if isinstance(n, ast.UnaryOp) and isinstance(n.op, (ast.USub, ast.UAdd)):
             v = ev(n.operand)
-            return -v if isinstance(n.op, ast.USub) else v
+            if isinstance(n.op, ast.UAdd):
+                return v
+            return [-c for c in v] if isvec(v) else -v
         if isinstance(n, ast.BinOp):
             a, b = ev(n.left), ev(n.right)
-            if isinstance(n.op, ast.Add):
-                return a + b
-            if isinstance(n.op, ast.Sub):
-                return a - b
+            if isinstance(n.op, (ast.Add, ast.Sub)):
+                if isvec(a) != isvec(b):
+                    raise Unsupported(expr)
+                sgn = 1.0 if isinstance(n.op, ast.Add) else -1.0
+                if isvec(a):
+                    return [a[i] + b[i] if sgn > 0 else a[i] - b[i] for i in range(3)]
+                return a + b if sgn > 0 else a - b
             if isinstance(n.op, ast.Mult):
+                if isvec(a) and isvec(b):
+                    raise Unsupported(expr)
+                if isvec(a):
+                    return [c * b for c in a]
+                if isvec(b):
+                    return [a * c for c in b]
                 return a * b
             if isinstance(n.op, ast.Div):
-                return a / b
+                if isvec(b):
+                    raise Unsupported(expr)
+                return [c / b for c in a] if isvec(a) else a / b
             raise Unsupported(expr)
         if isinstance(n, ast.Name):
             if n.id in CONSTS:
                 return CONSTS[n.id]
             if n.id in scalars:
                 return scalars[n.id]
+            if n.id in vecs:
+                return list(vecs[n.id])
             raise Unsupported(n.id)
-        if isinstance(n, ast.Call) and isinstance(n.func, ast.Name) and n.func.id in FUNCS:
-            return float(FUNCS[n.func.id](*[ev(a) for a in n.args]))
+        if isinstance(n, ast.Call) and isinstance(n.func, ast.Name):
+            if n.func.id == "__V":
+                if len(n.args) != 3:
+                    raise Unsupported(expr)
+                vals = [ev(a) for a in n.args]
+                if any(isvec(v) for v in vals):
+                    raise Unsupported(expr)
+                return vals
+            if n.func.id in FUNCS:
+                args = [ev(a) for a in n.args]
+                if any(isvec(v) for v in args):
+                    raise Unsupported(expr)
+                return float(FUNCS[n.func.id](*args))
+        if isinstance(n, ast.Call) and isinstance(n.func, ast.Attribute) and not n.args:
+            v = ev(n.func.value)
+            if not isvec(v):
+                raise Unsupported(expr)
+            at = n.func.attr
+            if at in ("x", "y", "z"):
+                return v["xyz".index(at)]
+            if at in ("Mag2", "Length2"):
+                return _dot(v, v)
+            if at in ("Mag", "Length"):
+                return math.sqrt(_dot(v, v))
+            if at == "Unit":
+                return unit(v)
+            if at == "Normalized":
+                f = 1.0 / math.sqrt(_dot(v, v))
+                return [c * f for c in v]
         raise Unsupported(expr)
 
     return ev(tree)
+
+
+def eval_scalar(expr, scalars):
+    v = eval_expr(expr, scalars)
+    if isinstance(v, list):
+        raise Unsupported(expr)
+    return v
 
 
 def split_args(s):
@@ -127,23 +182,10 @@ def unit(v):
 
 
 def eval_vec(expr, vecs, scalars, vec_type):
-    e = expr.strip()
-    m = re.fullmatch(r"\((.*)\)", e)
-    if m and e.count("(") == 1:
-        e = m.group(1).strip()
-    if e.endswith(".Unit()"):
-        return unit(eval_vec(e[: -len(".Unit()")], vecs, scalars, vec_type))
-    if e.startswith("-") and re.fullmatch(r"-\w+", e) and e[1:] in vecs:
-        return [-c for c in vecs[e[1:]]]
-    if e in vecs:
-        return list(vecs[e])
-    m = re.fullmatch(r"(?:%s|Vector3D<\w+>|Vec3D_t)\s*\((.*)\)" % vec_type, e)
-    if m:
-        args = split_args(m.group(1))
-        if len(args) != 3:
-            raise Unsupported(e)
-        return [eval_scalar(a, scalars) for a in args]
-    raise Unsupported(e)
+    v = eval_expr(expr, scalars, vecs)
+    if not isinstance(v, list):
+        raise Unsupported(expr)
+    return v
 
 
 def statements(text):
@@ -187,9 +229,131 @@ def trd_like_triangles(x1, x2, y1, y2, z):
     return [float(len(tris))] + [float(c) for t in tris for v in t for c in v]
 
 
-def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
+def _strip_parens(e):
+    e = e.strip()
+    while e.startswith("(") and _match_paren(e, 0) == len(e) - 1:
+        e = e[1:-1].strip()
+    return e
+
+
+def _match_paren(e, i):
+    """Index of the parenthesis closing the one at e[i], or -1."""
+    depth = 0
+    for j in range(i, len(e)):
+        if e[j] == "(":
+            depth += 1
+        elif e[j] == ")":
+            depth -= 1
+            if depth == 0:
+                return j
+    return -1
+
+
+def split_call(e):
+    """'obj->Method(args) rest' -> (obj, Method, args, rest), else None."""
+    m = re.match(r"\s*(\w+)\s*(?:\.|->)\s*(%s)\s*\(" % METHODS, e)
+    if not m:
+        return None
+    j = _match_paren(e, m.end() - 1)
+    if j < 0:
+        return None
+    return m.group(1), m.group(2), e[m.end():j], e[j + 1:].strip()
+
+
+def _query_point_dir(meth, argstr, vecs, scalars, vec_type):
+    a = split_args(argstr)
+    p = eval_vec(a[0], vecs, scalars, vec_type)
+    d = eval_vec(a[1], vecs, scalars, vec_type) if meth in ("DistanceToIn", "DistanceToOut") else None
+    return p, d
+
+
+def direct_query_assert(body, shapes, vecs, scalars, vec_type, ln):
+    """Asserts that call the shape inside the assert:
+    fabs(s->M(args) [- v]) < tol  -> absdiff
+    s->M(args) == v               -> exact (Contains: == true/false -> bool)
+    s->M(args) <,<=,>,>= 0        -> negative / nonpositive / positive / nonnegative
+    [!]s->Contains(p)             -> bool
+    s->Inside(p) == EInside::kX   -> inside
+    Returns a record tuple, or None when the body has none of these forms (or the shape is unknown)."""
+    b = _strip_parens(body)
+    if "||" in b or "&&" in b:
+        return None
+    m = re.match(r"(?:std::)?fabs\s*\(", b)
+    if m:
+        j = _match_paren(b, m.end() - 1)
+        if j < 0:
+            return None
+        inner, rest = _strip_parens(b[m.end():j]), b[j + 1:].strip()
+        c = split_call(inner)
+        mt = re.fullmatch(r"<\s*(.+)", rest)
+        if not c or not mt or c[0] not in shapes:
+            return None
+        sh, meth, argstr, tail = c
+        p, d = _query_point_dir(meth, argstr, vecs, scalars, vec_type)
+        val = 0.0
+        if tail:
+            mv = re.fullmatch(r"-\s*(.+)", tail)
+            if not mv:
+                return None
+            val = eval_scalar(mv.group(1), scalars)
+        return (sh, meth, p, d, ln, {"kind": "absdiff", "value": val, "tol": eval_scalar(mt.group(1), scalars)})
+    neg = b.startswith("!")
+    c = split_call(b[1:] if neg else b)
+    if not c or c[0] not in shapes:
+        return None
+    sh, meth, argstr, rest = c
+    p, d = _query_point_dir(meth, argstr, vecs, scalars, vec_type)
+    if meth == "Contains":
+        if rest == "":
+            return (sh, meth, p, None, ln, {"kind": "bool", "value": not neg})
+        mb = re.fullmatch(r"==\s*(true|false)", rest)
+        if mb and not neg:
+            return (sh, meth, p, None, ln, {"kind": "bool", "value": mb.group(1) == "true"})
+        return None
+    if neg:
+        return None
+    if meth == "Inside":
+        mi = re.fullmatch(r"==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", rest)
+        return (sh, meth, p, None, ln, {"kind": "inside", "value": mi.group(1)}) if mi else None
+    mc = re.fullmatch(r"(==|<=|<|>=|>)\s*(.+)", rest)
+    if not mc:
+        return None
+    op, rhs = mc.groups()
+    if op == "==":
+        return (sh, meth, p, d, ln, {"kind": "exact", "value": eval_scalar(rhs, scalars)})
+    if eval_scalar(rhs, scalars) != 0.0:
+        return None
+    kind = {"<": "negative", "<=": "nonpositive", ">": "positive", ">=": "nonnegative"}[op]
+    return (sh, meth, p, d, ln, {"kind": kind})
+
+
+def skip_reason(body, shapes, pending, unknown_results):
+    """Why an assert produced no record (for kat_stats.json)."""
+    c = None
+    for mcall in re.finditer(r"(\w+)\s*(?:\.|->)\s*(%s)\s*\(" % METHODS, body):
+        c = mcall
+        break
+    names = set(re.findall(r"\b(\w+)\b", body))
+    if c is not None:
+        if c.group(1) not in shapes:
+            return "shape not reconstructed by the extractor"
+        return "query form not recognised"
+    res = [n for n in names if n in pending or n in unknown_results]
+    if res:
+        why = unknown_results.get(res[0])
+        if why:
+            return why
+        return "comparison on a query result not recognised"
+    if re.search(r"\b(?:normal|Normal|valid|norm\w*)\b", body):
+        return "normal vector or validity flag (Normal() is not on the path)"
+    return "not a query assert (capacity, surface area, getters, bounding box)"
+
+
+def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
     text = open(path).read()
-    vecs, scalars, shapes, pending, arrays, extents = {}, {}, {}, {}, {}, {}
+    vecs, scalars, shapes, pending, arrays, extents, logicals = {}, {}, {}, {}, {}, {}, {}
+    unknown_results = {}
+    vec_arrays = set()
     global CURRENT_VECS
     CURRENT_VECS = vecs
     records = []
@@ -219,6 +383,35 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                 break
             st = st2
         st = re.sub(r"^\s*(?:const\s+)?", "", st)
+        st = re.sub(r"^(?:vecgeom::(?:cxx::)?|VECGEOM_NAMESPACE::)(?=[A-Za-z_])", "", st)
+        # shapes made by GeoManager::MakeInstance<UnplacedX>(args)
+        m = re.match(r"(?:auto|\w+)\s*\*?\s*(\w+)\s*=\s*(?:vecgeom::)?GeoManager::MakeInstance\s*<\s*"
+                     r"(?:vecgeom::)?(?:cxx::)?(\w+)\s*>\s*\((.*)\)$", st)
+        if m:
+            if unplaced and re.fullmatch(unplaced, m.group(2)):
+                try:
+                    shapes[m.group(1)] = shape_params(m.group(3))
+                except Unsupported:
+                    shapes.pop(m.group(1), None)
+            else:
+                shapes.pop(m.group(1), None)
+            continue
+        # identity placements: (new LogicalVolume("n", s))->Place([new Transformation3D()]),
+        # or LogicalVolume l("n", s) followed by auto p = l.Place()
+        m = re.match(r"LogicalVolume\s+(\w+)\s*\(\s*\"[^\"]*\"\s*,\s*&?\s*(\w+)\s*\)$", st)
+        if m:
+            logicals[m.group(1)] = m.group(2)
+            continue
+        m = re.match(r"(?:VPlacedVolume\s+const\s*\*|VPlacedVolume\s*\*|auto\s*\*?)\s*(\w+)\s*=\s*"
+                     r"(?:\(\s*new\s+LogicalVolume\s*\(\s*\"[^\"]*\"\s*,\s*&?\s*(\w+)\s*\)\s*\)|(\w+))\s*(?:->|\.)\s*"
+                     r"Place\s*\(\s*(?:new\s+Transformation3D\s*\(\s*\))?\s*\)$", st)
+        if m:
+            src = m.group(2) or logicals.get(m.group(3))
+            if src in shapes:
+                shapes[m.group(1)] = shapes[src]
+            else:
+                shapes.pop(m.group(1), None)
+            continue
         # tessellated solids made by the test's CreateTrdLikeTessellated helper
         m = re.match(r"(?:vecgeom::)?SimpleTessellated\s*&\s*(\w+)\s*=\s*\*\s*CreateTrdLikeTessellated\s*<\s*\w+\s*>"
                      r"\s*\((.*)\)$", st)
@@ -241,6 +434,27 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                     shapes[dm.group(1)] = shape_params(dm.group(2))
                 except Unsupported:
                     shapes.pop(dm.group(1), None)
+            continue
+        # arrays of vectors (trapezoid corners): Vec_t name[8] = {Vec_t(..), ...}; Vec_t name[8]; name[i] = ..
+        m = re.match(r"(?:%s)\s+(\w+)\s*\[\s*(\w+)\s*\]\s*(?:=\s*\{(.*)\})?$" % vec_type, st)
+        if m:
+            try:
+                if m.group(3) is not None:
+                    arrays[m.group(1)] = [c for e in split_args(m.group(3))
+                                          for c in eval_vec(e, vecs, scalars, vec_type)]
+                else:
+                    arrays[m.group(1)] = [None] * (3 * int(eval_scalar(m.group(2), scalars)))
+                vec_arrays.add(m.group(1))
+            except (Unsupported, ValueError):
+                arrays.pop(m.group(1), None)
+            continue
+        m = re.match(r"(\w+)\s*\[\s*(\d+)\s*\]\s*=\s*([^=].*)$", st)
+        if m and m.group(1) in vec_arrays and m.group(1) in arrays:
+            try:
+                i = int(m.group(2))
+                arrays[m.group(1)][3 * i:3 * i + 3] = eval_vec(m.group(3), vecs, scalars, vec_type)
+            except (Unsupported, IndexError):
+                arrays.pop(m.group(1), None)
             continue
         # array definitions: T name[] = {a, b, ...} (flattened into shape parameter lists)
         m = re.match(r"(?:Precision|double)\s+(\w+)\s*\[\s*\w*\s*\]\s*=\s*\{(.*)\}$", st)
@@ -314,6 +528,7 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
         if m:
             var, sh, meth, args = m.groups()
             pending.pop(var, None)
+            unknown_results.pop(var, None)
             if sh in shapes:
                 a = split_args(args)
                 try:
@@ -321,7 +536,9 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                     d = eval_vec(a[1], vecs, scalars, vec_type) if meth in ("DistanceToIn", "DistanceToOut") else None
                     pending[var] = (sh, meth, p, d, ln)
                 except (Unsupported, IndexError):
-                    pass
+                    unknown_results[var] = "argument or expected value not evaluable"
+            else:
+                unknown_results[var] = "shape not reconstructed by the extractor"
             continue
         # shape->Extent(lo, hi): remember which vector holds which corner
         m = re.fullmatch(r"(\w+)\s*(?:\.|->)\s*Extent\s*\(\s*(\w+)\s*,\s*(\w+)\s*\)", st)
@@ -340,44 +557,29 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
             rec = None
             me = re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,\s*(.+)\)", body)
             if me and me.group(1) in extents:
+                ok = False
                 try:
                     sh, which = extents[me.group(1)]
                     v = eval_vec(me.group(2), vecs, scalars, vec_type)
                     records.append({"line": "%s:%d" % (fname, ln), "shape": shapes[sh], "method": "Extent",
                                     "p": None, "d": None, "expect": {"kind": "extent", "which": which, "value": v}})
+                    ok = True
                 except Unsupported:
                     pass
+                if stats is not None:
+                    stats["asserts"] += 1
+                    if ok:
+                        stats["extracted"] += 1
+                    else:
+                        stats["skipped"].setdefault(
+                            "not a query assert (capacity, surface area, getters, bounding box)", []).append(ln)
                 continue
-            # fabs(shape->Method(args) - value) < tol, fabs(shape->Method(args)) < tol
-            mq = re.fullmatch(r"(?:std::)?fabs\s*\(\s*\(?\s*(\w+)\s*(?:\.|->)\s*(%s)\s*\(([^()]*(?:\([^()]*\))?[^()]*)\)"
-                              r"\s*(?:-\s*([-+\w.()*/ ]+?))?\s*\)?\s*\)\s*<\s*(\w+)" % METHODS, body)
-            # shape->Contains(p) / !shape->Contains(p)
-            mc = re.fullmatch(r"(!?)\s*(\w+)\s*(?:\.|->)\s*Contains\s*\((.*)\)", body)
-            mm = re.fullmatch(r"(\w+)\s*(?:\.|->)\s*Inside\s*\((.*)\)\s*==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", body)
-            if mq and mq.group(1) in shapes and "||" not in body:
-                try:
-                    a = split_args(mq.group(3))
-                    p = eval_vec(a[0], vecs, scalars, vec_type)
-                    meth = mq.group(2)
-                    d = eval_vec(a[1], vecs, scalars, vec_type) if meth in ("DistanceToIn", "DistanceToOut") else None
-                    val = eval_scalar(mq.group(4), scalars) if mq.group(4) else 0.0
-                    tol = eval_scalar(mq.group(5), scalars)
-                    rec = (mq.group(1), meth, p, d, ln, {"kind": "absdiff", "value": val, "tol": tol})
-                except (Unsupported, IndexError):
-                    rec = None
-            elif mc and mc.group(2) in shapes:
-                try:
-                    p = eval_vec(mc.group(3), vecs, scalars, vec_type)
-                    rec = (mc.group(2), "Contains", p, None, ln, {"kind": "bool", "value": mc.group(1) != "!"})
-                except Unsupported:
-                    rec = None
-            elif mm and mm.group(1) in shapes:
-                try:
-                    p = eval_vec(mm.group(2), vecs, scalars, vec_type)
-                    rec = (mm.group(1), "Inside", p, None, ln, {"kind": "inside", "value": mm.group(3)})
-                except Unsupported:
-                    rec = None
-            else:
+            rec, why = None, None
+            try:
+                rec = direct_query_assert(body, shapes, vecs, scalars, vec_type, ln)
+            except (Unsupported, IndexError, ValueError, ZeroDivisionError):
+                rec, why = None, "argument or expected value not evaluable"
+            if rec is None and why is None:
                 if "&&" in body and "||" not in body:
                     # a conjunction: every scalar ApproxEqual(var, value) term on a stored result is
                     # a requirement of its own (the other terms, e.g. on normals, are skipped)
@@ -409,10 +611,33 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
                     if mm and mm.group(1) in pending:
                         sh, meth, p, d, _ = pending[mm.group(1)]
                         rec = (sh, meth, p, d, ln, {"kind": "nonpositive"})
+                    mm = re.fullmatch(r"(\w+)\s*(>|>=)\s*0(?:\.0*)?", body)
+                    if mm and mm.group(1) in pending:
+                        sh, meth, p, d, _ = pending[mm.group(1)]
+                        rec = (sh, meth, p, d, ln, {"kind": "positive" if mm.group(2) == ">" else "nonnegative"})
+                    # d == value (exact), d >= kInfLength (a miss)
+                    mm = re.fullmatch(r"(\w+)\s*(==|>=)\s*(.+)", body)
+                    if rec is None and mm and mm.group(1) in pending and pending[mm.group(1)][1] != "Inside":
+                        try:
+                            v = eval_scalar(mm.group(3), scalars)
+                            sh, meth, p, d, _ = pending[mm.group(1)]
+                            if mm.group(2) == "==":
+                                rec = (sh, meth, p, d, ln, {"kind": "exact", "value": v})
+                            elif v == DBL_MAX:
+                                rec = (sh, meth, p, d, ln, {"kind": "exact", "value": v})
+                        except Unsupported:
+                            why = "argument or expected value not evaluable"
             if rec:
                 sh, meth, p, d, line, exp = rec
                 records.append({"line": "%s:%d" % (fname, line), "shape": shapes[sh], "method": meth,
                                 "p": p, "d": d, "expect": exp})
+            if stats is not None:
+                stats["asserts"] += 1
+                if rec:
+                    stats["extracted"] += 1
+                else:
+                    why = why or skip_reason(body, shapes, pending, unknown_results)
+                    stats["skipped"].setdefault(why, []).append(ln)
             continue
         # plain re-assignment of a scalar or vector: name = expr
         m = re.fullmatch(r"(\w+)\s*=\s*([^=].*)", st)
@@ -430,6 +655,34 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
             except Unsupported:
                 vecs.pop(name, None)
             continue
+        # in-place vector updates: v.Normalize(); v op= expr; v.x() = expr
+        m = re.fullmatch(r"(\w+)\s*\.\s*Normalize\s*\(\s*\)", st)
+        if m and m.group(1) in vecs:
+            v = vecs[m.group(1)]
+            f = 1.0 / math.sqrt(_dot(v, v))  # Vector3D::Normalize, base/Vector3D.h:198
+            vecs[m.group(1)] = [c * f for c in v]
+            continue
+        m = re.fullmatch(r"(\w+)\s*([-+*/])=\s*(.+)", st)
+        if m and m.group(1) in vecs:
+            try:
+                rhs = eval_expr(m.group(3), scalars, vecs)
+                v, op = vecs[m.group(1)], m.group(2)
+                if op in "+-" and isinstance(rhs, list):
+                    vecs[m.group(1)] = [v[i] + rhs[i] if op == "+" else v[i] - rhs[i] for i in range(3)]
+                elif op in "*/" and not isinstance(rhs, list):
+                    vecs[m.group(1)] = [c * rhs if op == "*" else c / rhs for c in v]
+                else:
+                    vecs.pop(m.group(1), None)
+            except Unsupported:
+                vecs.pop(m.group(1), None)
+            continue
+        m = re.fullmatch(r"(\w+)\s*\.\s*([xyz])\s*\(\s*\)\s*=\s*([^=].*)", st)
+        if m and m.group(1) in vecs:
+            try:
+                vecs[m.group(1)]["xyz".index(m.group(2))] = eval_scalar(m.group(3), scalars)
+            except Unsupported:
+                vecs.pop(m.group(1), None)
+            continue
         # anything else that mutates a known vector or scalar makes it unknown
         m = re.match(r"(\w+)\s*(?:\.\s*\w+\(\))?\s*(?:[-+*/]?=)", st)
         if m:
@@ -440,35 +693,55 @@ def extract(path, shape_t, vec_type="Vec_t", nparams=None, fixed_arrays=None):
 
 
 SHAPES = {
-    # fixture name: (file, shape class pattern, accepted ctor parameter counts)
-    "box": ("TestBox.cpp", r"Box_t", (3,)),
-    "tube": ("TestTube.cpp", r"Tube_t", (5,)),
-    "trapezoid": ("TestTrap.cpp", r"Trap_t", (11,)),
-    "cone": ("TestCons.cpp", r"Cone_t", (7,)),
-    "orb": ("TestOrb.cpp", r"Orb_t", (1,)),
-    "polycone": ("TestPolycone.cpp", r"Polycone_t", None),
-    "sphere": ("TestSphere.cpp", r"Sphere_t", (6,)),
-    "polyhedron": ("TestPolyhedra.cpp", r"Polyhedra_t", "polyhedron"),
-    "torus2": ("TestTorus2.cpp", r"Torus_t", (5,)),
-    "tessellated": ("TestTessellated.cpp", r"SimpleTessellated", "tessellated"),
-    "hype": ("TestHype.cpp", r"Hype_t", (5,)),
+    # fixture name: (file, shape class pattern, accepted ctor parameter counts, unplaced class pattern)
+    "box": ("TestBox.cpp", r"Box_t", (3,), r"UnplacedBox"),
+    "tube": ("TestTube.cpp", r"Tube_t", (5,), r"(?:S?Unplaced|G)Tube\w*"),
+    "trapezoid": ("TestTrap.cpp", r"Trap_t", (3, 4, 5, 6, 11, 24), r"UnplacedTrapezoid"),
+    "cone": ("TestCons.cpp", r"Cone_t", (7,), r"S?UnplacedCone\w*"),
+    "orb": ("TestOrb.cpp", r"Orb_t", (1,), r"UnplacedOrb"),
+    "polycone": ("TestPolycone.cpp", r"Polycone_t", None, r"UnplacedPolycone"),
+    "sphere": ("TestSphere.cpp", r"Sphere_t", (6,), r"UnplacedSphere"),
+    "polyhedron": ("TestPolyhedra.cpp", r"Polyhedra_t", "polyhedron", r"UnplacedPolyhedron"),
+    "torus2": ("TestTorus2.cpp", r"Torus_t", (5,), r"UnplacedTorus2"),
+    "tessellated": ("TestTessellated.cpp", r"SimpleTessellated", "tessellated", None),
+    "hype": ("TestHype.cpp", r"Hype_t", (5,), r"S?UnplacedHype\w*"),
 }
+
+
+def params_ok(counts, shape):
+    if counts is None:  # sphi dphi nz z[nz] rmin[nz] rmax[nz]
+        return len(shape) >= 3 and len(shape) == 3 + 3 * int(shape[2])
+    if counts == "tessellated":  # ntri, 9 doubles per triangle
+        return len(shape) == 1 + 9 * int(shape[0])
+    if counts == "polyhedron":  # sphi dphi nsides nz z[nz] rmin[nz] rmax[nz]
+        return len(shape) >= 4 and len(shape) == 4 + 3 * int(shape[3])
+    return len(shape) in counts
+
 
 if __name__ == "__main__":
     REF_ROOT = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     out_dir = os.path.dirname(os.path.abspath(__file__))
     only = sys.argv[2:] or list(SHAPES)
+    stats_path = os.path.join(out_dir, "kat_stats.json")
+    all_stats = json.load(open(stats_path)) if os.path.exists(stats_path) else {}
     for name in only:
-        fn, pat, counts = SHAPES[name]
-        recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat)
-        if counts is None:  # sphi dphi nz z[nz] rmin[nz] rmax[nz]
-            recs = [r for r in recs if len(r["shape"]) >= 3 and len(r["shape"]) == 3 + 3 * int(r["shape"][2])]
-        elif counts == "tessellated":  # ntri, 9 doubles per triangle
-            recs = [r for r in recs if len(r["shape"]) == 1 + 9 * int(r["shape"][0])]
-        elif counts == "polyhedron":  # sphi dphi nsides nz z[nz] rmin[nz] rmax[nz]
-            recs = [r for r in recs if len(r["shape"]) >= 4 and len(r["shape"]) == 4 + 3 * int(r["shape"][3])]
-        else:
-            recs = [r for r in recs if len(r["shape"]) in counts]
+        fn, pat, counts, unplaced = SHAPES[name]
+        st = {"source": "test/unit_tests/%s" % fn, "asserts": 0, "extracted": 0, "skipped": {}}
+        recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat, unplaced=unplaced, stats=st)
+        kept = []
+        for r in recs:
+            if params_ok(counts, r["shape"]):
+                kept.append(r)
+            else:
+                st["extracted"] -= 1
+                st["skipped"].setdefault("constructor form not supported (%d parameters)" % len(r["shape"]),
+                                         []).append(int(r["line"].rsplit(":", 1)[1]))
+        st["records"] = len(kept)
+        st["skipped"] = {k: sorted(v) for k, v in sorted(st["skipped"].items())}
+        all_stats[name] = st
         with open(os.path.join(out_dir, "kat_%s.json" % name), "w") as f:
-            json.dump({"source": "test/unit_tests/%s" % fn, "records": recs}, f, indent=0)
-        print("%-12s %4d records" % (name, len(recs)))
+            json.dump({"source": "test/unit_tests/%s" % fn, "records": kept}, f, indent=0)
+        print("%-12s %4d records  %4d asserts, %4d extracted, %4d skipped" % (
+            name, len(kept), st["asserts"], st["extracted"], sum(len(v) for v in st["skipped"].values())))
+    with open(stats_path, "w") as f:
+        json.dump(all_stats, f, indent=1, sort_keys=True)

@@ -40,6 +40,12 @@ def check(rec, got):
         return got < 0
     if e["kind"] == "nonpositive":
         return got <= 0
+    if e["kind"] == "positive":
+        return got > 0
+    if e["kind"] == "nonnegative":
+        return got >= 0
+    if e["kind"] == "exact":  # assert(s->M(..) == v): the reference's own value, bit for bit
+        return got == e["value"]
     if e["kind"] == "absdiff":
         return abs(got - e["value"]) < e["tol"]
     if e["kind"] == "bool":

@@ -121,29 +121,72 @@ int MakeTube(double const *a, int n, ShapeRecord &r, std::string &err)
 // Trapezoid: volumes/UnplacedTrapezoid.h:56-63 (tan(alpha)), TrapezoidStruct::CalculateCached,
 // corners source/UnplacedTrapezoid.cpp:447-465, side planes from corner diagonals :575-619
 // (outward unit normal (p3-p1)x(p4-p2), offset through the corner centroid).
+// Parameter counts select the reference constructor: 11 = the full list (dz theta phi dy1 dx1 dx2 alpha1
+// dy2 dx3 dx4 alpha2); 3 = xbox ybox zbox; 4 = dx1 dx2 dy dz; 5 = dx1 dx2 dy1 dy2 dz; 6 = dx dy dz alpha
+// theta phi (source/UnplacedTrapezoid.cpp:148-190, each the full form with zero or shared angles);
+// 24 = eight corners in TrapCorners order (:129-134, parameters recovered by fromCornersToParameters
+// :496-523, planes through the given corners).
 int MakeTrapezoid(double const *a, int n, ShapeRecord &r, std::string &err)
 {
-  if (n < 11) return err = "trapezoid needs 11 parameters (dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2)", -1;
-  if (!(a[0] > 0 && a[3] > 0 && a[4] > 0 && a[5] > 0 && a[7] > 0 && a[8] > 0 && a[9] > 0))
-    return err = "trapezoid half-lengths must be positive", -1;
-  r                = ShapeRecord();
-  r.type           = kTrapezoid;
-  TrapezoidData &t = r.u.trap;
-  const double dz = a[0], theta = a[1], phi = a[2], dy1 = a[3], dx1 = a[4], dx2 = a[5];
-  const double ta1 = std::tan(a[6]), dy2 = a[7], dx3 = a[8], dx4 = a[9], ta2 = std::tan(a[10]);
-  const double tthetaCphi = std::tan(theta) * std::cos(phi);
-  const double tthetaSphi = std::tan(theta) * std::sin(phi);
-  t.dz = dz;
-  double prm[11] = {dz, theta, phi, dy1, dx1, dx2, ta1, dy2, dx3, dx4, ta2};
-  for (int i = 0; i < 11; ++i) t.params[i] = prm[i];
   struct P3 {
     double x, y, z;
   };
-  const double dxdyDy1 = ta1 * dy1, dxdyDy2 = ta2 * dy2, dxdzDz = tthetaCphi * dz, dydzDz = tthetaSphi * dz;
-  P3 c[8] = {{-dxdzDz - dxdyDy1 - dx1, -dydzDz - dy1, -dz}, {-dxdzDz - dxdyDy1 + dx1, -dydzDz - dy1, -dz},
-             {-dxdzDz + dxdyDy1 - dx2, -dydzDz + dy1, -dz}, {-dxdzDz + dxdyDy1 + dx2, -dydzDz + dy1, -dz},
-             {+dxdzDz - dxdyDy2 - dx3, +dydzDz - dy2, +dz}, {+dxdzDz - dxdyDy2 + dx3, +dydzDz - dy2, +dz},
-             {+dxdzDz + dxdyDy2 - dx4, +dydzDz + dy2, +dz}, {+dxdzDz + dxdyDy2 + dx4, +dydzDz + dy2, +dz}};
+  double f[11];
+  P3 c[8];
+  double prm[11];
+  if (n == 24) {
+    for (int i = 0; i < 8; ++i) c[i] = {a[3 * i], a[3 * i + 1], a[3 * i + 2]};
+    const double dz = c[7].z, dzRecip = 1.0 / dz;
+    const double dy1 = 0.50 * (c[2].y - c[0].y), dx1 = 0.50 * (c[1].x - c[0].x), dx2 = 0.50 * (c[3].x - c[2].x);
+    const double ta1 = 0.25 * (c[2].x + c[3].x - c[1].x - c[0].x) / dy1;
+    const double dy2 = 0.50 * (c[6].y - c[4].y), dx3 = 0.50 * (c[5].x - c[4].x), dx4 = 0.50 * (c[7].x - c[6].x);
+    const double ta2 = 0.25 * (c[6].x + c[7].x - c[5].x - c[4].x) / dy2;
+    const double tc = (c[4].x + dy2 * ta2 + dx3) * dzRecip, ts = (c[4].y + dy2) * dzRecip;
+    if (!(dz > 0 && dy1 > 0 && dx1 > 0 && dx2 > 0 && dy2 > 0 && dx3 > 0 && dx4 > 0))
+      return err = "trapezoid corners must give positive half-lengths (TrapCorners order)", -1;
+    double g[11] = {dz, std::atan(std::sqrt(ts * ts + tc * tc)), std::atan2(ts, tc), dy1, dx1, dx2, ta1, dy2,
+                    dx3, dx4, ta2};
+    for (int i = 0; i < 11; ++i) prm[i] = g[i];
+  } else {
+    if (n == 11) {
+      for (int i = 0; i < 11; ++i) f[i] = a[i];
+    } else if (n == 3) {
+      double g[11] = {a[2], 0., 0., a[1], a[0], a[0], 0., a[1], a[0], a[0], 0.};
+      for (int i = 0; i < 11; ++i) f[i] = g[i];
+    } else if (n == 4) {
+      double g[11] = {a[3], 0., 0., a[2], a[0], a[0], 0., a[2], a[1], a[1], 0.};
+      for (int i = 0; i < 11; ++i) f[i] = g[i];
+    } else if (n == 5) {
+      double g[11] = {a[4], 0., 0., a[2], a[0], a[0], 0., a[3], a[1], a[1], 0.};
+      for (int i = 0; i < 11; ++i) f[i] = g[i];
+    } else if (n == 6) {
+      double g[11] = {a[2], a[4], a[5], a[1], a[0], a[0], a[3], a[1], a[0], a[0], a[3]};
+      for (int i = 0; i < 11; ++i) f[i] = g[i];
+    } else {
+      return err = "trapezoid takes 11 parameters (dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2), "
+                   "3, 4, 5 or 6 (reduced forms) or 24 (eight corners)",
+             -1;
+    }
+    if (!(f[0] > 0 && f[3] > 0 && f[4] > 0 && f[5] > 0 && f[7] > 0 && f[8] > 0 && f[9] > 0))
+      return err = "trapezoid half-lengths must be positive", -1;
+    const double dz = f[0], theta = f[1], phi = f[2], dy1 = f[3], dx1 = f[4], dx2 = f[5];
+    const double ta1 = std::tan(f[6]), dy2 = f[7], dx3 = f[8], dx4 = f[9], ta2 = std::tan(f[10]);
+    const double tthetaCphi = std::tan(theta) * std::cos(phi);
+    const double tthetaSphi = std::tan(theta) * std::sin(phi);
+    double g[11] = {dz, theta, phi, dy1, dx1, dx2, ta1, dy2, dx3, dx4, ta2};
+    for (int i = 0; i < 11; ++i) prm[i] = g[i];
+    const double dxdyDy1 = ta1 * dy1, dxdyDy2 = ta2 * dy2, dxdzDz = tthetaCphi * dz, dydzDz = tthetaSphi * dz;
+    P3 cc[8] = {{-dxdzDz - dxdyDy1 - dx1, -dydzDz - dy1, -dz}, {-dxdzDz - dxdyDy1 + dx1, -dydzDz - dy1, -dz},
+                {-dxdzDz + dxdyDy1 - dx2, -dydzDz + dy1, -dz}, {-dxdzDz + dxdyDy1 + dx2, -dydzDz + dy1, -dz},
+                {+dxdzDz - dxdyDy2 - dx3, +dydzDz - dy2, +dz}, {+dxdzDz - dxdyDy2 + dx3, +dydzDz - dy2, +dz},
+                {+dxdzDz + dxdyDy2 - dx4, +dydzDz + dy2, +dz}, {+dxdzDz + dxdyDy2 + dx4, +dydzDz + dy2, +dz}};
+    for (int i = 0; i < 8; ++i) c[i] = cc[i];
+  }
+  r                = ShapeRecord();
+  r.type           = kTrapezoid;
+  TrapezoidData &t = r.u.trap;
+  t.dz             = prm[0];
+  for (int i = 0; i < 11; ++i) t.params[i] = prm[i];
   auto plane = [&](int i, int i1, int i2, int i3, int i4) {
     P3 const &p1 = c[i1], &p2 = c[i2], &p3 = c[i3], &p4 = c[i4];
     double ux = p3.x - p1.x, uy = p3.y - p1.y, uz = p3.z - p1.z;
