@@ -22,6 +22,26 @@ def inputs_for(kind, params, n, seed, near_fraction=0.5):
     return o, p, d
 
 
+def grazing_rays(kind, params, n, seed):
+    """Rays that skim the surface far from their start: two nearby surface points (within 1e-6 mm of
+    the surface, neighbours along a random projection) define a direction close to tangent or
+    across an edge, and the start point lies 5-60 mm back along it. These are the rays whose hit or
+    miss depends on rounding."""
+    o = OracleShape(kind, params)
+    lo, hi = o.extent()
+    g = workloads.rng(seed)
+    s = workloads.near_surface(g, n, lo, hi, lambda q: o.query("Contains", q),
+                               lambda q, e: o.query("DistanceToOut", q, e), band=1e-6)
+    key = np.argsort(s[0] * 0.37 + s[1] * 0.71 + s[2] * 0.6)
+    s1 = s[:, key]
+    e = np.roll(s1, 1, axis=1) - s1
+    nrm = np.linalg.norm(e, axis=0)
+    ok = nrm > 1e-9
+    e, s1 = e[:, ok] / nrm[ok], s1[:, ok]
+    p = s1 - e * g.uniform(5.0, 60.0, size=s1.shape[1])
+    return o, np.ascontiguousarray(p), np.ascontiguousarray(e)
+
+
 def to_dev(a):
     return SOA3D.from_array(a, device="cuda")
 

@@ -89,6 +89,7 @@ def test_fp64_parity(kind, params):
             assert np.array_equal(ref, got), (m, int((ref != got).sum()))
         else:
             bad, exact, maxerr = gpu_util.compare_float(ref, got, TOL_BY_KIND.get(kind, TOL))
+            print("%-11s %-13s max |gpu - oracle| %.3g mm, bit-exact %d of %d" % (kind, m, maxerr, exact, len(ref)))
             assert bad == 0, (m, bad, maxerr)
             assert exact >= EXACT_FRACTION_BY_KIND.get(kind, 0.999) * len(ref), (m, exact, len(ref))
 

@@ -167,6 +167,22 @@ def margin_polyhedron(params, p):
     return np.minimum(profile_margin(proj, p[2], z, rmin, rmax), phi_margin(p, sphi, dphi))
 
 
+def margin_polyhedron_ref(params, p):
+    """The solid the reference's Inside / Contains describe: the phi quadrilaterals are tested as two
+    half-spaces (Planes::Inside / Contains on segment.phi, PolyhedronImplementation.h:728,826), an
+    intersection also when the phi range exceeds pi (where the wedge is their union)."""
+    sphi, dphi, nsides, nz = params[0], params[1], int(params[2]), int(params[3])
+    z = params[4:4 + nz]
+    dside = dphi / nsides
+    centres = sphi + (np.arange(nsides) + 0.5) * dside
+    proj = np.max(np.cos(centres)[:, None] * p[0][None, :] + np.sin(centres)[:, None] * p[1][None, :], axis=0)
+    m = profile_margin(proj, p[2], z, params[4 + nz:4 + 2 * nz], params[4 + 2 * nz:4 + 3 * nz])
+    e = sphi + dphi
+    d1 = -math.sin(sphi) * p[0] + math.cos(sphi) * p[1]
+    d2 = math.sin(e) * p[0] - math.cos(e) * p[1]
+    return np.minimum(m, np.minimum(d1, d2))
+
+
 def trapezoid_corners(params):
     """The eight corners of G4Trap parameters (dz theta phi dy1 dx1 dx2 alpha1 dy2 dx3 dx4 alpha2)."""
     dz, th, ph, dy1, dx1, dx2, a1, dy2, dx3, dx4, a2 = params
@@ -213,6 +229,12 @@ CASES = [
      margin_polyhedron),
     ("polyhedron", [math.radians(15), math.radians(120), 3, 2, -10.0, 10.0, 3.0, 4.0, 9.0, 12.0],
      margin_polyhedron),
+]
+# Inside / Contains only: a polyhedron with a phi range above pi, whose reference answer is the
+# solid of margin_polyhedron_ref (its distances follow the faces themselves)
+INSIDE_ONLY = [
+    ("polyhedron", [0.5, 4.0, 8, 4, -20.0, -10.0, 10.0, 20.0, 0.0, 5.0, 3.0, 0.0, 10.0, 15.0, 12.0, 8.0],
+     margin_polyhedron_ref),
 ]
 # planar side faces need (dx2 - dx1) / dy1 + 2 tan(alpha1) == (dx4 - dx3) / dy2 + 2 tan(alpha2)
 _TRAPS = [
@@ -304,7 +326,7 @@ def first_crossing(margin, params, p, d, tmax, entering, max_iter=4000):
     return out, certain
 
 
-@pytest.mark.parametrize("case", CASES, ids=IDS)
+@pytest.mark.parametrize("case", CASES + INSIDE_ONLY, ids=IDS + ["%s-ref-%d" % (c[0], i) for i, c in enumerate(INSIDE_ONLY)])
 def test_inside_and_contains_match_margin(case):
     kind, params, margin = case
     s = OracleShape(kind, params)

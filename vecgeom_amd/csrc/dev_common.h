@@ -136,6 +136,35 @@ VGMI_DEV R QuadRoot(R b, R D, R ainv, R c, int sgn)
   return (-b + R(sgn) * sq) * ainv;
 }
 
+// ---------------------------------------------------------------- certified FP32 (mixed precision)
+// The FP32 pass of a directed query returns either NaN (undecided: evaluate in FP64) or a distance
+// with the FP64 kernel's hit/miss decision and within kMixDistErr of its value. Every FP32 quantity
+// that feeds a decision carries a rounding bound: with u = 2^-24, a length computed from FP32
+// copies of the inputs by a few roundings is within c*u*(sum of the magnitudes involved) of its
+// FP64 value (c counted per expression, rounded up). A test is decided only when the margin exceeds
+// its bound; parameters along the ray are carried as intervals [lo, hi] holding the FP64 value.
+constexpr float kU32        = 5.9604645e-8f;
+constexpr float kMixDistErr = float(kMixedBand) * 0.25f;
+
+struct FInt {
+  float lo, hi;
+};
+VGMI_DEV FInt fint(float v, float e) { return FInt{v - e, v + e}; }
+VGMI_DEV FInt fmaxi(FInt a, FInt b) { return FInt{a.lo < b.lo ? b.lo : a.lo, a.hi < b.hi ? b.hi : a.hi}; }
+VGMI_DEV FInt fmini(FInt a, FInt b) { return FInt{b.lo < a.lo ? b.lo : a.lo, b.hi < a.hi ? b.hi : a.hi}; }
+VGMI_DEV float Undecided() { return __builtin_nanf(""); }
+// three-valued tests of certified FP32 code: a > b is decided (kYes / kNo) when |a - b| exceeds e,
+// the bound of the rounding error of a - b
+enum { kNo = 0, kYes = 1, kUnsure = 2 };
+VGMI_DEV int cmpGT(float a, float b, float e)
+{
+  const float m = a - b;
+  return m > e ? kYes : (m < -e ? kNo : kUnsure);
+}
+VGMI_DEV int and3(int a, int b) { return (a == kNo || b == kNo) ? kNo : ((a == kYes && b == kYes) ? kYes : kUnsure); }
+VGMI_DEV int or3(int a, int b) { return (a == kYes || b == kYes) ? kYes : ((a == kNo && b == kNo) ? kNo : kUnsure); }
+VGMI_DEV int not3(int a) { return a == kUnsure ? kUnsure : (a == kYes ? kNo : kYes); }
+
 template <class R>
 VGMI_DEV R InfLen();
 template <>

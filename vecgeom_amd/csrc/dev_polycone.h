@@ -160,9 +160,22 @@ VGMI_DEV R DistanceToOut(PolyconeData const &P, Vec3<R> const &point, Vec3<R> co
   return total;
 }
 
+// The safeties take the section of the point's z and change abruptly across a z plane, even far
+// from the surface: an FP32 z within kMixedPlaneBand of a plane answers NaN (evaluated in FP64).
+template <class R>
+VGMI_DEV bool NearZPlaneF32(PolyconeData const &P, R z)
+{
+  if constexpr (sizeof(R) == sizeof(float)) {
+    for (int i = 0; i <= P.nsec; ++i)
+      if (vabs(z - R(zb(P, i))) < R(kMixedPlaneBand)) return true;
+  }
+  return false;
+}
+
 template <class R>
 VGMI_DEV R SafetyToIn(PolyconeData const &P, Vec3<R> const &p)
 {
+  if (NearZPlaneF32(P, p.z)) return R(__builtin_nanf(""));
   int index  = SectionIndex(P, p.z);
   bool needZ = false;
   if (index < 0) {
@@ -193,6 +206,7 @@ VGMI_DEV R SafetyToIn(PolyconeData const &P, Vec3<R> const &p)
 template <class R>
 VGMI_DEV R SafetyToOut(PolyconeData const &P, Vec3<R> const &p)
 {
+  if (NearZPlaneF32(P, p.z)) return R(__builtin_nanf(""));
   bool cin, cout;
   ContainsInside(P, p, true, cin, cout);
   if (cout) return R(-1);

@@ -575,14 +575,43 @@ VGMI_DEV R Safety(PolyhedronData const &P, Vec3<R> const &p, bool ptInside)
   return vsqrt(safety);
 }
 
+// The safeties take the z segment of the point's z and change abruptly across a z plane: an FP32
+// z within kMixedPlaneBand of a plane answers NaN (evaluated in FP64).
+template <class R>
+VGMI_DEV bool NearZPlaneF32(PolyhedronData const &P, R z)
+{
+  if constexpr (sizeof(R) == sizeof(float)) {
+    for (int k = 0; k < P.nz; ++k)
+      if (vabs(z - R(zp(P, k))) < R(kMixedPlaneBand)) return true;
+  }
+  return false;
+}
+
+// The reference's Inside and Contains test the phi quadrilaterals as two half-spaces (Planes::Inside
+// and Contains on segment.phi, PolyhedronImplementation.h:728,826), so for a phi range above pi the
+// extended phi planes also pass through the solid's interior and the answer changes there. An FP32
+// point within kMixedPlaneBand of a phi plane or of a z plane is left to FP64.
+VGMI_DEV bool F32PointUndecided(PolyhedronData const &P, Vec3<float> const &p)
+{
+  if (NearZPlaneF32(P, p.z)) return true;
+  if (P.hasPhiCutout) {
+    const QSet ph = qset(P, 0, 1);
+    for (int i = 0; i < ph.n; ++i)
+      if (vabs(planeDist(ph.q + kQuadDoubles * i, p)) < float(kMixedPlaneBand)) return true;
+  }
+  return false;
+}
+
 template <class R>
 VGMI_DEV R SafetyToIn(PolyhedronData const &P, Vec3<R> const &p)
 {
+  if (NearZPlaneF32(P, p.z)) return R(__builtin_nanf(""));
   return Safety(P, p, false);
 }
 template <class R>
 VGMI_DEV R SafetyToOut(PolyhedronData const &P, Vec3<R> const &p)
 {
+  if (NearZPlaneF32(P, p.z)) return R(__builtin_nanf(""));
   return Safety(P, p, true);
 }
 

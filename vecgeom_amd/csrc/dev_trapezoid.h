@@ -48,9 +48,128 @@ VGMI_DEV int Inside(TrapezoidData const &t, Vec3<R> const &p)
   return ci ? kInside : (co ? kOutside : kSurface);
 }
 
+// Certified FP32 paths (see dev_common.h). A plane distance n.p + D of FP32 copies is within
+// 4u (|a x| + |b y| + |c z| + |D|) of its FP64 value, a projection n.d within 4u (|a dx| + |b dy| +
+// |c dz|), and their quotient -pd / pr within (epd + |v| epr) / |pr| + u |v|. Sign tests whose
+// operand lies within its bound, and interval comparisons that overlap, answer NaN.
+struct TrapPlaneF {
+  float pd, epd, pr, epr;
+};
+VGMI_DEV TrapPlaneF trap_plane_f32(TrapezoidData const &t, int i, Vec3<float> const &p, Vec3<float> const &d)
+{
+  const float a = float(t.a[i]), b = float(t.b[i]), c = float(t.c[i]), dd = float(t.d[i]);
+  TrapPlaneF r;
+  r.pd  = a * p.x + b * p.y + c * p.z + dd;
+  r.epd = 4.f * kU32 * (vabs(a * p.x) + vabs(b * p.y) + vabs(c * p.z) + vabs(dd));
+  r.pr  = a * d.x + b * d.y + c * d.z;
+  r.epr = 4.f * kU32 * (vabs(a * d.x) + vabs(b * d.y) + vabs(c * d.z));
+  return r;
+}
+
+VGMI_DEV FInt trap_vdist_f32(TrapPlaneF const &q)
+{
+  const float v = -q.pd / NonZero(q.pr);
+  return fint(v, (q.epd + vabs(v) * q.epr) / vabs(q.pr) + 2.f * kU32 * vabs(v));
+}
+
+VGMI_DEV float DistanceToInF32(TrapezoidData const &t, Vec3<float> const &p, Vec3<float> const &d)
+{
+  const float dz = float(t.dz), ht = float(kHalfTolerance);
+  const float signZ = Sign(d.z);
+  const float mx    = signZ * dz - p.z;
+  const float ez    = 3.f * kU32 * (dz + vabs(p.z));
+  if (signZ * mx + ez < ht) return InfLen<float>();
+  if (signZ * mx - ez < ht) return Undecided();
+  const float invdir = 1.f / NonZero(d.z);
+  const float a0 = mx * invdir, a1 = -(signZ * dz + p.z) * invdir;
+  FInt smax = fint(a0, ez * vabs(invdir) + 2.f * kU32 * vabs(a0));
+  FInt smin = fint(a1, ez * vabs(invdir) + 2.f * kU32 * vabs(a1));
+  float vsmin = a1;
+  TrapPlaneF q[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    q[i] = trap_plane_f32(t, i, p, d);
+    // done = (pd > ht && pr >= 0) || (pd > -ht && pr > 0): the sign tests must be decided
+    if (vabs(q[i].pd) <= q[i].epd + ht || vabs(q[i].pr) <= q[i].epr) return Undecided();
+    if (q[i].pd > 0.f && q[i].pr > 0.f) return InfLen<float>();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool posPoint = q[i].pd > 0.f, posDir = q[i].pr > 0.f;
+    const bool fromIn = !posPoint && posDir, fromOut = posPoint && !posDir;
+    if (!fromIn && !fromOut) continue;
+    const FInt v = trap_vdist_f32(q[i]);
+    if (fromIn) {
+      if (v.hi < smin.lo) return InfLen<float>();
+      if (v.lo < smin.hi) return Undecided();
+      smax = fmini(smax, v);
+    } else {
+      if (v.lo > smax.hi) return InfLen<float>();
+      if (v.hi > smax.lo) return Undecided();
+      vsmin = vmax(vsmin, -q[i].pd / NonZero(q[i].pr)); // the estimate stays inside the interval
+      smin  = fmaxi(smin, v);
+    }
+  }
+  if (smin.hi <= smax.lo) return smin.hi - smin.lo <= kMixDistErr ? vsmin : Undecided();
+  if (smin.lo > smax.hi) return InfLen<float>();
+  return Undecided();
+}
+
+VGMI_DEV float DistanceToOutF32(TrapezoidData const &t, Vec3<float> const &p, Vec3<float> const &d)
+{
+  // wrong side: beyond the z planes the reference answers -1 at once. Outside a side plane it sets
+  // -1 at every such plane and then keeps the smaller vd of the later planes with pr > 0, so the
+  // answer is min(-1, vd of those planes): negative, which is all the reference's convention gives a
+  // wrong-side result; its FP32 magnitude is returned once every plane's side is decided.
+  const float dz = float(t.dz), ht = float(kHalfTolerance);
+  const float mz = vabs(p.z) - dz, ez = 3.f * kU32 * (dz + vabs(p.z));
+  if (mz - ez > ht) return -1.f;
+  if (mz + ez > ht) return Undecided();
+  {
+    float neg = 0.f;
+    bool outside = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const TrapPlaneF q = trap_plane_f32(t, i, p, d);
+      if (vabs(q.pd - ht) <= q.epd) return Undecided();
+      if (q.pd > ht) {
+        outside = true;
+        neg     = -1.f;
+      }
+      if (outside && q.pr > 0.f) neg = vmin(neg, -q.pd / NonZero(q.pr));
+    }
+    if (outside) return neg;
+  }
+  FInt out{FLT_MAX, FLT_MAX};
+  float vout = FLT_MAX;
+  if (d.z != 0.f) {
+    const float inv = 1.f / NonZero(d.z);
+    const float v   = (Sign(d.z) * dz - p.z) * inv;
+    out             = fint(v, 3.f * kU32 * (dz + vabs(p.z)) * vabs(inv) + 2.f * kU32 * vabs(v));
+    vout            = v;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const TrapPlaneF q = trap_plane_f32(t, i, p, d);
+    if (q.pd + q.epd > float(-kHalfTolerance)) return Undecided(); // not certainly inside this plane
+    if (q.pr + q.epr <= 0.f) continue;                              // certainly not exiting through it
+    if (q.pr - q.epr <= 0.f) {
+      // exit direction undecided: only harmless when even the nearest possible crossing is beyond
+      const float lo = (-q.pd - q.epd) / (vabs(q.pr) + q.epr);
+      if (lo > out.hi) continue;
+      return Undecided();
+    }
+    const FInt v = trap_vdist_f32(q);
+    vout         = vmin(vout, -q.pd / NonZero(q.pr));
+    out          = fmini(out, v);
+  }
+  return out.hi - out.lo <= kMixDistErr ? vout : Undecided();
+}
+
 template <class R>
 VGMI_DEV R DistanceToIn(TrapezoidData const &t, Vec3<R> const &p, Vec3<R> const &d)
 {
+  if constexpr (sizeof(R) == sizeof(float)) return DistanceToInF32(t, p, d);
   const R dz = R(t.dz);
   R signZ    = Sign(d.z);
   R mx       = signZ * dz - p.z;
@@ -85,6 +204,7 @@ VGMI_DEV R DistanceToIn(TrapezoidData const &t, Vec3<R> const &p, Vec3<R> const 
 template <class R>
 VGMI_DEV R DistanceToOut(TrapezoidData const &t, Vec3<R> const &p, Vec3<R> const &d)
 {
+  if constexpr (sizeof(R) == sizeof(float)) return DistanceToOutF32(t, p, d);
   const R dz = R(t.dz);
   if (vabs(p.z) > dz + R(kHalfTolerance)) return R(-1);
   R dist = InfLen<R>();

@@ -202,10 +202,271 @@ VGMI_DEV R DistanceToInKernel(TubeData const &t, Vec3<R> const &p, Vec3<R> const
   return dist;
 }
 
+// ---------------------------------------------------------------- certified FP32 paths
+// (see dev_common.h) The same decisions as DistanceToInKernel / DistanceToOut, each taken only when
+// its margin exceeds the rounding bound of the FP32 operands; ep bounds the error of the FP32 point.
+// FP32 forms 1 - dz^2 as dx^2 + dy^2 (the same value for a unit direction) so that rays close to
+// the axis keep a small relative error.
+
+// PointInCyclicalSector of a computed point (x, y), position error ex
+VGMI_DEV int InSectorF(TubeData const &t, float x, float y, float ex, bool includeSurface)
+{
+  const float a1x = float(t.a1x), a1y = float(t.a1y), a2x = float(t.a2x), a2y = float(t.a2y);
+  const float s1 = -x * a1y + y * a1x, s2 = -a2x * y + a2y * x;
+  const float e1 = 1.5f * ex + 4.f * kU32 * (vabs(x * a1y) + vabs(y * a1x));
+  const float e2 = 1.5f * ex + 4.f * kU32 * (vabs(a2x * y) + vabs(a2y * x));
+  const float tol = includeSurface ? float(-kHalfTolerance) : float(kHalfTolerance);
+  const int c1 = cmpGT(s1, tol, e1), c2 = cmpGT(s2, tol, e2); // >= and > agree once decided
+  return t.phiLePi ? and3(c1, c2) : or3(c1, c2);
+}
+
+// CircleTrajectoryIntersection with b, c carrying errors eb, ec
+VGMI_DEV int CircleHitF(TubeData const &t, float b, float eb, float c, float ec, Vec3<float> const &p,
+                        Vec3<float> const &d, float ep, bool largest, bool sectorCheck, float &dist, float &edist)
+{
+  const float delta = b * b - c;
+  const float ed    = 2.f * vabs(b) * eb + ec + 3.f * kU32 * (b * b + vabs(c));
+  if (delta < -ed) return kNo;
+  if (delta <= ed) return kUnsure; // tangent within rounding (largest also accepts delta == 0)
+  const float sq  = vsqrt(delta);
+  const float esq = ed / (sq + vsqrt(vmax(delta - ed, 0.f))) + kU32 * sq;
+  dist            = largest ? -b + sq : -b - sq;
+  edist           = eb + esq + 2.f * kU32 * vabs(dist);
+  const int ok    = cmpGT(dist, float(-2 * kTolerance), edist);
+  if (ok != kYes) return ok;
+  if (!sectorCheck) return kYes;
+  const float hitz = p.z + dist * d.z;
+  const float eh   = edist + 3.f * kU32 * (vabs(p.x) + vabs(p.y) + vabs(p.z) + vabs(dist)) + ep;
+  const int zok    = cmpGT(float(t.z), vabs(hitz), eh + kU32 * float(t.z));
+  if (zok != kYes || !t.hasPhi) return zok;
+  return InSectorF(t, p.x + dist * d.x, p.y + dist * d.y, eh, true);
+}
+
+// PhiPlaneTrajectoryIntersection (PositiveDirectionOfPhiVector)
+VGMI_DEV int PhiPlaneHitF(TubeData const &t, float ax, float ay, float nx, float ny, Vec3<float> const &p,
+                          Vec3<float> const &d, float ep, bool toIn, float &dist, float &edist)
+{
+  const float dn  = d.x * nx + d.y * ny;
+  const float edn = 4.f * kU32 * (vabs(d.x * nx) + vabs(d.y * ny));
+  const int dirok = toIn ? cmpGT(dn, 0.f, edn) : cmpGT(0.f, dn, edn);
+  if (dirok != kYes) return dirok;
+  const float dxy  = d.y * ax - d.x * ay;
+  const float edxy = 4.f * kU32 * (vabs(d.y * ax) + vabs(d.x * ay));
+  if (vabs(dxy) <= edxy) return kUnsure;
+  const float num  = ay * p.x - ax * p.y;
+  const float enm  = 4.f * kU32 * (vabs(ay * p.x) + vabs(ax * p.y)) + 1.5f * ep;
+  dist             = num / NonZero(dxy);
+  edist            = (enm + vabs(dist) * edxy) / (vabs(dxy) - edxy) + 2.f * kU32 * vabs(dist);
+  int ok           = cmpGT(dist * vabs(dn), float(-kHalfTolerance), edist * vabs(dn) + vabs(dist) * edn);
+  if (ok != kYes) return ok;
+  const float hx = p.x + dist * d.x, hy = p.y + dist * d.y;
+  const float eh = edist + 3.f * kU32 * (vabs(p.x) + vabs(p.y) + vabs(p.z) + vabs(dist)) + ep;
+  if (toIn) {
+    const float hz = p.z + dist * d.z;
+    ok             = cmpGT(float(t.z + kHalfTolerance), vabs(hz), eh + kU32 * float(t.z));
+    const float r2 = hx * hx + hy * hy;
+    const float er = 2.f * vsqrt(r2) * eh + eh * eh + 4.f * kU32 * r2;
+    ok             = and3(ok, cmpGT(r2, float(t.tolOrmin2), er + kU32 * float(t.tolOrmin2)));
+    ok             = and3(ok, cmpGT(float(t.tolOrmax2), r2, er + kU32 * float(t.tolOrmax2)));
+  }
+  const float ha = hx * ax + hy * ay;
+  return and3(ok, cmpGT(ha, 0.f, 1.5f * eh + 3.f * kU32 * (vabs(hx * ax) + vabs(hy * ay))));
+}
+
+VGMI_DEV float DistanceToInF32(TubeData const &t, Vec3<float> const &pt, Vec3<float> const &d)
+{
+  Vec3<float> p = pt;
+  float ep      = 2.f * kU32 * (vabs(p.x) + vabs(p.y) + vabs(p.z));
+  float move = 0.f, emove = 0.f;
+  const float ptDist = vsqrt(p.Mag2());
+  if (ptDist > 100.f * float(t.maxVal)) { // the far-point shift (TubeImplementation.h:475-502)
+    move  = ptDist - 100.f * float(t.maxVal);
+    p     = p + move * d;
+    emove = 4.f * kU32 * ptDist;
+    ep += 8.f * kU32 * ptDist;
+  }
+  const float z = float(t.z), ht = float(kHalfTolerance);
+  const float distz = vabs(p.z) - z;
+  const float edz   = 2.f * kU32 * (vabs(p.z) + z) + ep;
+  if (p.z * d.z >= 0.f) {
+    const int c = cmpGT(distz, ht, edz);
+    if (c == kYes) return InfLen<float>();
+    if (c == kUnsure) return Undecided();
+  }
+  const float rsq   = p.x * p.x + p.y * p.y;
+  const float ersq  = 4.f * kU32 * rsq + 2.f * vsqrt(rsq) * ep + ep * ep;
+  const float rdotn = p.x * d.x + p.y * d.y;
+  const float erd   = 4.f * kU32 * (vabs(p.x * d.x) + vabs(p.y * d.y)) + ep;
+  const float irmax2 = float(t.tolIrmax2);
+  {
+    int c = cmpGT(rsq, irmax2, ersq + kU32 * irmax2);
+    if (c != kNo) c = and3(c, not3(cmpGT(0.f, rdotn, erd)));
+    if (c == kYes) return InfLen<float>();
+    if (c == kUnsure) return Undecided();
+  }
+  {
+    int in = and3(cmpGT(-ht, distz, edz), cmpGT(irmax2, rsq, ersq + kU32 * irmax2));
+    if (t.hasRmin) in = and3(in, cmpGT(rsq, float(t.tolIrmin2), ersq + kU32 * float(t.tolIrmin2)));
+    if (t.hasPhi && in != kNo) in = and3(in, InSectorF(t, p.x, p.y, ep, false));
+    if (in == kYes) return -1.f;
+    if (in == kUnsure) return Undecided();
+  }
+  // entry through a z plane
+  {
+    const float adz = NonZeroAbs(d.z);
+    const float tz  = distz / adz;
+    const float etz = edz / adz + 2.f * kU32 * vabs(tz);
+    int okz         = (p.z * d.z < 0.f) ? cmpGT(tz, -ht, etz) : kNo;
+    if (okz != kNo) {
+      const float hx = p.x + tz * d.x, hy = p.y + tz * d.y;
+      const float eh = etz + 3.f * kU32 * (vabs(p.x) + vabs(p.y) + vabs(tz)) + ep;
+      const float r2 = hx * hx + hy * hy;
+      const float er = 2.f * vsqrt(r2) * eh + eh * eh + 4.f * kU32 * r2;
+      okz            = and3(okz, cmpGT(float(t.rmax2), r2, er + kU32 * float(t.rmax2)));
+      if (t.hasRmin) okz = and3(okz, cmpGT(r2, float(t.rmin2), er + kU32 * float(t.rmin2)));
+      if (t.hasPhi && okz != kNo) okz = and3(okz, InSectorF(t, hx, hy, eh, true));
+    }
+    if (okz == kUnsure) return Undecided();
+    if (okz == kYes) return etz + emove <= kMixDistErr ? tz + move : Undecided();
+  }
+  // (a start point on a cylindrical surface is within kMixedBand of the surface: never here)
+  const float n2   = d.x * d.x + d.y * d.y;
+  const float inv  = 1.f / NonZero(n2);
+  const float rel  = 6.f * kU32;
+  const float b    = inv * rdotn;
+  const float eb   = vabs(b) * rel + erd * vabs(inv);
+  FInt best{FLT_MAX, FLT_MAX};
+  float vbest = FLT_MAX;
+  {
+    const float cr = rsq - float(t.rmax2);
+    const float c  = inv * cr;
+    const float ec = vabs(c) * rel + (ersq + 2.f * kU32 * (rsq + float(t.rmax2))) * vabs(inv);
+    float dr, edr;
+    const int h = CircleHitF(t, b, eb, c, ec, p, d, ep, false, true, dr, edr);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) return edr + emove <= kMixDistErr ? dr + move : Undecided();
+  }
+  if (t.hasRmin) {
+    const float cr = rsq - float(t.rmin2);
+    const float c  = inv * cr;
+    const float ec = vabs(c) * rel + (ersq + 2.f * kU32 * (rsq + float(t.rmin2))) * vabs(inv);
+    float dr, edr;
+    const int h = CircleHitF(t, b, eb, c, ec, p, d, ep, true, true, dr, edr);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dr, edr));
+      vbest = vmin(vbest, dr);
+    }
+  }
+  if (t.hasPhi) {
+    float dp, edp;
+    int h = PhiPlaneHitF(t, float(t.a1x), float(t.a1y), float(t.n1x), float(t.n1y), p, d, ep, true, dp, edp);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dp, edp));
+      vbest = vmin(vbest, dp);
+    }
+    h = PhiPlaneHitF(t, float(t.a2x), float(t.a2y), float(t.n2x), float(t.n2y), p, d, ep, true, dp, edp);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dp, edp));
+      vbest = vmin(vbest, dp);
+    }
+  }
+  if (vbest >= FLT_MAX) return InfLen<float>();
+  return best.hi - best.lo + 2.f * emove <= kMixDistErr ? vbest + move : Undecided();
+}
+
+VGMI_DEV float DistanceToOutF32(TubeData const &t, Vec3<float> const &p, Vec3<float> const &d)
+{
+  const float ep = 2.f * kU32 * (vabs(p.x) + vabs(p.y) + vabs(p.z));
+  const float z = float(t.z), ht = float(kHalfTolerance);
+  // wrong side: every outside point gets -1
+  {
+    const float distz = z - vabs(p.z);
+    int out = cmpGT(-ht, distz, 2.f * kU32 * (vabs(p.z) + z) + ep);
+    const float rsq  = p.x * p.x + p.y * p.y;
+    const float ersq = 4.f * kU32 * rsq + 2.f * vsqrt(rsq) * ep + ep * ep;
+    out = or3(out, cmpGT(rsq - float(t.rmax2), float(2.0 * kTolerance * t.rmax),
+                         ersq + 2.f * kU32 * (rsq + float(t.rmax2))));
+    if (t.hasRmin)
+      out = or3(out, cmpGT(float(-2.0 * kTolerance * t.rmin), rsq - float(t.rmin2),
+                           ersq + 2.f * kU32 * (rsq + float(t.rmin2))));
+    if (t.hasPhi) {
+      const float sc = -p.x * float(t.a1y) + p.y * float(t.a1x), ec = -float(t.a2x) * p.y + float(t.a2y) * p.x;
+      const float e1 = 1.5f * ep + 4.f * kU32 * (vabs(p.x) + vabs(p.y));
+      const int o1 = cmpGT(0.f, sc, e1), o2 = cmpGT(0.f, ec, e1);
+      int wo = t.wedgeLtPi ? or3(o1, o2) : and3(o1, o2);
+      // on a phi plane (|n.p| < kTolerance on the plane's half) is within kMixedBand of the surface
+      // for a point inside the z and r ranges: such a point never reaches the FP32 pass
+      out = or3(out, wo);
+    }
+    if (out == kYes) return -1.f;
+    if (out == kUnsure) return Undecided();
+  }
+  const float rsq   = p.x * p.x + p.y * p.y;
+  const float ersq  = 4.f * kU32 * rsq + 2.f * vsqrt(rsq) * ep + ep * ep;
+  const float rdotn = d.x * p.x + d.y * p.y;
+  const float erd   = 4.f * kU32 * (vabs(p.x * d.x) + vabs(p.y * d.y)) + ep;
+  FInt best{FLT_MAX, FLT_MAX};
+  float vbest = FLT_MAX;
+  if (d.z != 0.f) {
+    const float inv = 1.f / NonZero(d.z);
+    const float v   = (d.z < 0.f ? (-z - p.z) : (z - p.z)) * inv;
+    best            = fint(v, (2.f * kU32 * (z + vabs(p.z)) + ep) * vabs(inv) + 2.f * kU32 * vabs(v));
+    vbest           = v;
+  }
+  const float n2  = d.x * d.x + d.y * d.y;
+  const float inv = 1.f / NonZero(n2);
+  const float rel = 6.f * kU32;
+  const float b   = inv * rdotn;
+  const float eb  = vabs(b) * rel + erd * vabs(inv);
+  if (t.hasRmin) {
+    const float c  = inv * (rsq - float(t.rmin2));
+    const float ec = vabs(c) * rel + (ersq + 2.f * kU32 * (rsq + float(t.rmin2))) * vabs(inv);
+    float dr, edr;
+    const int h = CircleHitF(t, b, eb, c, ec, p, d, ep, false, false, dr, edr);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dr, edr));
+      vbest = vmin(vbest, dr);
+    }
+  }
+  {
+    const float c  = inv * (rsq - float(t.rmax2));
+    const float ec = vabs(c) * rel + (ersq + 2.f * kU32 * (rsq + float(t.rmax2))) * vabs(inv);
+    float dr, edr;
+    const int h = CircleHitF(t, b, eb, c, ec, p, d, ep, true, false, dr, edr);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dr, edr));
+      vbest = vmin(vbest, dr);
+    }
+  }
+  if (t.hasPhi) {
+    float dp, edp;
+    int h = PhiPlaneHitF(t, float(t.a1x), float(t.a1y), float(t.n1x), float(t.n1y), p, d, ep, false, dp, edp);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dp, edp));
+      vbest = vmin(vbest, dp);
+    }
+    h = PhiPlaneHitF(t, float(t.a2x), float(t.a2y), float(t.n2x), float(t.n2y), p, d, ep, false, dp, edp);
+    if (h == kUnsure) return Undecided();
+    if (h == kYes) {
+      best  = fmini(best, fint(dp, edp));
+      vbest = vmin(vbest, dp);
+    }
+  }
+  if (vbest >= FLT_MAX) return InfLen<float>();
+  return best.hi - best.lo <= kMixDistErr ? vbest : Undecided();
+}
+
 // DistanceToIn with the far-point shift (TubeImplementation.h:475-502)
 template <class R>
 VGMI_DEV R DistanceToIn(TubeData const &t, Vec3<R> const &pt, Vec3<R> const &d)
 {
+  if constexpr (sizeof(R) == sizeof(float)) return DistanceToInF32(t, pt, d);
   Vec3<R> p = pt;
   R ptDist  = vsqrt(p.Mag2());
   R move    = R(0);
@@ -220,6 +481,7 @@ VGMI_DEV R DistanceToIn(TubeData const &t, Vec3<R> const &pt, Vec3<R> const &d)
 template <class R>
 VGMI_DEV R DistanceToOut(TubeData const &t, Vec3<R> const &p, Vec3<R> const &d)
 {
+  if constexpr (sizeof(R) == sizeof(float)) return DistanceToOutF32(t, p, d);
   const R z = R(t.z);
   R distz   = z - vabs(p.z);
   if (distz < R(-kHalfTolerance)) return R(-1);
