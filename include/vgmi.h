@@ -89,8 +89,11 @@ VGMI_API vgmi_status vgmi_shape_extent(const vgmi_shape *shape, double lo[3], do
  * step_max*, out*) and SafetyToOut(SOA3D, out*) (volumes/UnplacedVolume.h:144,179)
  * and the VPlacedVolume SOA overloads Contains/Inside/DistanceToIn/SafetyToIn
  * (volumes/PlacedVolume.h:273,299,332,372,405,421) for an identity placement.
- * dx/dy/dz may be NULL for Contains/Inside/Safety*; step_max may be NULL
- * (kInfLength). out is uint8*, int32* or double* by op.
+ * dx/dy/dz may be NULL for Contains/Inside/Safety*. step_max (per point) may be NULL
+ * (kInfLength); as in the reference only the tessellated solid uses it
+ * (VecGeom/volumes/kernel/TessellatedImplementation.h:191-199, 213-216, 250: the extent distance is
+ * returned when it is not below the limit, facet clusters beyond it are not searched); the other
+ * shapes' kernels ignore it. out is uint8*, int32* or double* by op.
  */
 VGMI_API vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, const double *px, const double *py,
                              const double *pz, const double *dx, const double *dy, const double *dz,
@@ -287,7 +290,8 @@ VGMI_API vgmi_status vgmi_server_submit(vgmi_server *server, const double *px, c
                                         int64_t n, double *px_out, double *py_out, double *pz_out,
                                         uint32_t *nav_out, int32_t *steps_out, uint64_t *ticket);
 /* Blocks until the request is done; latency_us (may be NULL) = submit-to-completion time.
- * Each ticket is waited for exactly once. */
+ * Each ticket is waited for exactly once. Returns VGMI_ERR_CAPACITY for a request the server was
+ * destroyed before launching. */
 VGMI_API vgmi_status vgmi_server_wait(vgmi_server *server, uint64_t ticket, double *latency_us);
 /* stats[0..3] = launches, tracks done, steps done, requests refused */
 VGMI_API vgmi_status vgmi_server_stats(vgmi_server *server, int64_t stats[4]);
@@ -295,7 +299,9 @@ VGMI_API vgmi_status vgmi_server_stats(vgmi_server *server, int64_t stats[4]);
  * power-of-two track-count bucket from 4096 up to lane_tracks; the kernels read the actual count
  * from device memory). stats[0] = graph launches so far, stats[1] = graphs per lane. */
 VGMI_API vgmi_status vgmi_server_graph_stats(vgmi_server *server, int64_t stats[2]);
-/* Stops the threads after the launches in flight complete; queued requests are dropped. */
+/* Stops the threads after the launches in flight complete. Requests still queued are never launched:
+ * their waiters (threads blocked in vgmi_server_wait) return VGMI_ERR_CAPACITY, and destroy returns
+ * once every waiter has left. No other call on the server may start once destroy has begun. */
 VGMI_API vgmi_status vgmi_server_destroy(vgmi_server *server);
 
 #ifdef __cplusplus

@@ -99,3 +99,24 @@ def test_server_rejects_bad_arguments_without_gpu():
     assert L.vgmi_server_submit(None, *([None] * 7), 1, *([None] * 5), C.byref(t)) == -1
     assert L.vgmi_server_wait(None, 1, None) == -1
     assert L.vgmi_server_destroy(None) == 0
+
+
+def test_tessellated_facet_count_is_bounded():
+    """The facet count is checked before any size arithmetic (at most 2^23 facets)."""
+    import math
+    import pytest
+    from vecgeom_amd import VgmiError, make_shape
+    for bad in (float(1 << 24), 1e300, math.nan, 3.0, -5.0):
+        with pytest.raises(VgmiError, match="triangles"):
+            make_shape("tessellated", [bad] + [0.0] * 36)
+
+
+def test_device_array_check():
+    """The wrappers' tensor check: device, dtype, contiguity and length (host tensors here)."""
+    import pytest
+    import torch
+    from vecgeom_amd.soa import device_array
+    with pytest.raises(ValueError, match="device tensor"):
+        device_array(torch.zeros(4, dtype=torch.float64), torch.float64, 4, "x")
+    with pytest.raises(ValueError, match="torch tensor"):
+        device_array([0.0] * 4, torch.float64, 4, "x")

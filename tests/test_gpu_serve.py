@@ -143,3 +143,70 @@ def test_poisson_load_completes(geo):
     assert s["requests"] > 20 and refused == 0
     assert st["tracks"] == s["tracks"] and st["steps"] == s["nav_steps"]
     assert s["p99_ms"] > 0
+
+
+@pytest.fixture(scope="module")
+def calo():
+    from vecgeom_amd.navigation import Geometry
+    g = geometry.calorimeter()
+    return g, Geometry(g)
+
+
+@pytest.mark.parametrize("precision", [FP64, MIXED])
+def test_full_calorimeter_requests(calo, precision):
+    """The serving workload's own geometry (the 23 081-touchable calorimeter) and request sizes
+    (1k, 64k and 200k tracks of U{1..100} steps): each request's result is the same alone and in
+    the mix, and a sample of its tracks matches the oracle's transport loop step for step."""
+    g, G = calo
+    print("calorimeter requests, precision", precision, flush=True)
+    reqs = _requests(g, [1000, 65536, 200000], seed=41)
+    with Server(G, precision=precision, lane_tracks=1 << 18, lanes=3) as S:
+        alone = [S.wait(S.submit(*r)) for r in reqs]
+        mixed = [None] * len(reqs)
+
+        def run(i):
+            mixed[i] = S.wait(S.submit(*reqs[i]))
+
+        th = [threading.Thread(target=run, args=(i,)) for i in range(len(reqs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.p_out, m.p_out)
+        assert np.array_equal(a.nav, m.nav)
+        assert np.array_equal(a.taken, m.taken)
+    O = OracleGeometry(g)
+    rng = np.random.default_rng(5)
+    for (p, d, nsteps), r in zip(reqs, alone):
+        sel = np.sort(rng.choice(len(nsteps), size=min(256, len(nsteps)), replace=False))
+        nav0 = O.locate(np.ascontiguousarray(p[:, sel]), threads=8)
+        for j, i in enumerate(sel):
+            rp, rn, rl, rt = O.propagate(np.ascontiguousarray(p[:, i:i + 1]), np.ascontiguousarray(d[:, i:i + 1]),
+                                         10.0, nav0[j:j + 1], np.zeros(1, np.uint32), int(nsteps[i]))
+            assert r.nav[i] == rn[0] and r.taken[i] == rt, (i, int(nsteps[i]))
+            assert np.max(np.abs(r.p_out[:, i] - rp[:, 0])) <= 1e-9, i
+
+
+def test_destroy_with_queued_requests_and_a_blocked_waiter(geo):
+    """Destroying a server whose queue still holds requests: their waiters return an error (the
+    request was never launched) instead of blocking, and destroy returns once they have left."""
+    g, G = geo
+    reqs = _requests(g, [30000] * 6, seed=37)
+    S = Server(G, lane_tracks=1 << 15, lanes=1)
+    tickets = [S.submit(*r) for r in reqs]
+    outcome = []
+
+    def wait_last():
+        try:
+            S.wait(tickets[-1])
+            outcome.append("done")
+        except VgmiError as e:
+            outcome.append(str(e))
+
+    t = threading.Thread(target=wait_last)
+    t.start()
+    S.close()
+    t.join(timeout=60)
+    assert not t.is_alive()
+    assert outcome and (outcome[0] == "done" or "never launched" in outcome[0]), outcome

@@ -67,10 +67,11 @@ CASES = [
     ("hype", [10.0, 20.0, 0.0, math.pi / 4, 50.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
-# from the host C library in the last bit, and the Newton polishing stops once a step is below
-# kTolerance, so torus results agree within a few kTolerance rather than bit for bit.
-TOL_BY_KIND = {"torus2": 1e-8}
-EXACT_FRACTION_BY_KIND = {"torus2": 0.5}
+# from the host C library in the last bit, and the Newton polishing then stops at a neighbouring
+# iterate, so torus results are held to the kTolerance bar (measured: at most 2.2e-10 mm on these
+# cases) with at least 90 % bit-identical (measured 95-98 %) rather than all.
+TOL_BY_KIND = {"torus2": TOL}
+EXACT_FRACTION_BY_KIND = {"torus2": 0.9}
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 
 
@@ -144,3 +145,52 @@ def test_empty_and_ragged_sizes():
         got = s.DistanceToIn(P, D).cpu().numpy()
         ref = o.query("DistanceToIn", p, d)
         assert np.array_equal(ref, got), n
+
+
+@pytest.mark.parametrize("case", [33, 34, 36])
+def test_tessellated_step_max(case):
+    """A finite step_max bounds the tessellated search as in the reference
+    (TessellatedImplementation.h:191-199, 213-216, 250): the extent distance when it is not below
+    the limit, no facet cluster entered beyond it. Limits from 0 to beyond the solid, per point."""
+    from vecgeom_amd import make_shape
+    import torch
+    kind, params = CASES[case]
+    assert kind == "tessellated"
+    o, p, d = gpu_util.inputs_for(kind, params, 1 << 13, seed=11)
+    lo, hi = o.extent()
+    span = float(np.max(hi - lo))
+    g = np.random.default_rng(3)
+    step = g.uniform(0.0, 1.5 * span, size=p.shape[1])
+    step[::7] = 1e300  # no limit on some points
+    s = make_shape(kind, params)
+    P, D = gpu_util.to_dev(p), gpu_util.to_dev(d)
+    SM = torch.as_tensor(step, dtype=torch.float64, device="cuda")
+    for m in ("DistanceToIn", "DistanceToOut"):
+        ref = o.query(m, p, d, step=step)
+        got = getattr(s, m)(P, D, step_max=SM).cpu().numpy()
+        assert np.array_equal(ref, got), (m, int((ref != got).sum()))
+        # the limit changes answers (otherwise this would not test it)
+        free = o.query(m, p, d)
+        assert (free != ref).sum() > 10, m
+
+
+def test_wrappers_reject_bad_tensors():
+    """Tensors of the wrong dtype, length or device are refused before any launch."""
+    from vecgeom_amd import make_shape
+    from vecgeom_amd.batch import ShapeBatch
+    import torch
+    s = make_shape("box", [10.0, 20.0, 30.0])
+    o, p, d = gpu_util.inputs_for("box", [10.0, 20.0, 30.0], 1024, seed=1)
+    P, D = gpu_util.to_dev(p), gpu_util.to_dev(d)
+    short = gpu_util.to_dev(d[:, :1000])
+    with pytest.raises(ValueError, match="differ in length"):
+        s.DistanceToIn(P, short)
+    with pytest.raises(ValueError, match="float64"):
+        s.SafetyToIn(P, out=torch.empty(1024, dtype=torch.float32, device="cuda"))
+    with pytest.raises(ValueError, match="elements"):
+        s.Inside(P, out=torch.empty(100, dtype=torch.int32, device="cuda"))
+    with pytest.raises(ValueError, match="device"):
+        s.DistanceToIn(P, D, step_max=torch.zeros(1024, dtype=torch.float64))
+    b = ShapeBatch([s], [1024])
+    with pytest.raises(ValueError, match="batch"):
+        b.query_all(P, short)

@@ -13,6 +13,7 @@ import torch
 
 from ._lib import check, lib
 from .shapes import FP64, _stream
+from .soa import device_array
 
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "vecgeom_amd", "batch_tuning.json")
@@ -84,6 +85,14 @@ class ShapeBatch:
                 **{m: torch.empty(n, dtype=torch.float64, device=device) for m in METHODS[2:]}}
 
     def _args(self, points, dirs, outs):
+        n = self.total
+        if len(points) != n or len(dirs) != n:
+            raise ValueError("points (%d) and directions (%d) must hold the batch's %d points"
+                             % (len(points), len(dirs), n))
+        dt = {"Contains": torch.uint8, "Inside": torch.int32}
+        for m in METHODS:
+            if outs.get(m) is not None:
+                device_array(outs[m], dt.get(m, torch.float64), n, m)
         ptr = lambda m: outs[m].data_ptr() if outs.get(m) is not None else None  # noqa: E731
         return [*points.ptrs(), *dirs.ptrs()] + [ptr(m) for m in METHODS]
 

@@ -242,7 +242,7 @@ vgmi_status vgmi_batch_create(const vgmi_shape *const *shapes, const int64_t *co
     b->recs.push_back(shapes[s]->rec);
     if (b->recs.back().type == kTessellated && counts[s] > 0) {
       const int need = tess_lds_bytes(b->recs.back().u.tess);
-      if (need > 0 && need <= kTessLdsMax) b->ldsBytes = std::max(b->ldsBytes, need);
+      if (need > 0 && need <= tess_lds_limit()) b->ldsBytes = std::max(b->ldsBytes, need);
     }
     b->offsets.push_back(b->total);
     b->counts.push_back(counts[s]);

@@ -82,10 +82,13 @@ VGMI_DEV LdsWalk lds_walk(TessellatedData const &s, const double *img)
 }
 
 // BoxImplementation::IntersectCachedKernel2<float, float> on the facet's single-precision box as
-// HybridNavigator2::GetHitCandidates_v calls it (:111-150): t0 = 0, t1 = FLT_MAX.
+// HybridNavigator2::GetHitCandidates_v calls it (:111-150): t0 = 0, t1 = maxstep, the step limit
+// of the query as a float (FLT_MAX without one; BVHSortedIntersectionsLooper's limit,
+// TessellatedImplementation.h:250).
 // The reference's early exits are folded into one conjunction (same value: every test is a
 // comparison of values computed the same way whether or not an earlier test failed).
-VGMI_DEV bool box_candidate(FBox const &b, V3d const &p, V3d const &inv)
+VGMI_DEV bool box_candidate(FBox const &b, V3d const &p, V3d const &inv, float maxstep = FLT_MAX,
+                            double stepMax = kInfLength)
 {
   const bool sx = inv.x < 0, sy = inv.y < 0, sz = inv.z < 0;
   const float px = float(p.x), py = float(p.y);
@@ -103,8 +106,9 @@ VGMI_DEV bool box_candidate(FBox const &b, V3d const &p, V3d const &inv)
   tmax              = tmax < tzmax ? tmax : tzmax;
   bool ok = xy;
   ok &= z;
-  ok &= tmin <= FLT_MAX && tmax > float(0.0 - kTolerance);
-  ok &= tmin < FLT_MAX;
+  ok &= tmin <= float(double(maxstep) + kTolerance) && tmax > float(0.0 - kTolerance);
+  ok &= tmin < maxstep;
+  ok &= !(double(tmin) > stepMax); // the hook stops at boxes entered beyond stepMax (:213-216)
   return ok;
 }
 
@@ -143,14 +147,19 @@ struct RayResult {
 constexpr int kHitsIn = 0, kHitsOut = 1, kHitsBoth = 2;
 
 template <class W>
-VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, int mode)
+VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, int mode,
+                              float maxstep = FLT_MAX, double stepMax = kInfLength)
 {
   const V3d inv(1.0 / NonZero(d.x), 1.0 / NonZero(d.y), 1.0 / NonZero(d.z));
   RayResult r{kInfLength, kInfLength, -1, -1, 0., 0.};
+  // a node entered beyond the step limit holds no candidate box (every facet box inside it is
+  // entered later; the margin covers the float rounding of the candidate test)
+  const double stepBound = maxstep < FLT_MAX ? double(maxstep) * (1. + 1e-5) + 1e-5 : kInfLength;
   int j = 0;
   while (j < s.nnodes) {
     const NodeRec nd   = w.node(j);
-    const double bound = mode == kHitsIn ? r.dIn : (mode == kHitsOut ? r.dOut : vmax(r.dIn, r.dOut));
+    const double bound =
+        vmin(stepBound, mode == kHitsIn ? r.dIn : (mode == kHitsOut ? r.dOut : vmax(r.dIn, r.dOut)));
     if (!node_reached(nd.lo, nd.hi, p, inv, bound)) {
       j = nd.escape;
       continue;
@@ -172,7 +181,7 @@ VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p
       if (!vin && !vout) continue;
       const double dist = -saf / ndd;
       if (vin ? dist > r.dIn : dist > r.dOut) continue;
-      if (!box_candidate(B, p, inv)) continue;
+      if (!box_candidate(B, p, inv, maxstep, stepMax)) continue;
       const double *F = facet(s, k);
       const V3d v0 = at3(F + 7), s0 = at3(F + 10), v1 = at3(F + 13), s1 = at3(F + 16), v2 = at3(F + 19),
                 s2     = at3(F + 22);
@@ -284,6 +293,25 @@ template <class W>
 VGMI_DEV double dist_out64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d)
 {
   return dist_out_of(s, facet_hits(s, w, p, d, kHitsOut), d);
+}
+
+// DistanceToSolid with a finite step limit (TessellatedImplementation.h:191-199, 213-216, 250): the
+// extent box distance is returned at once when it is not below the limit (DistanceToIn), and facet
+// boxes entered beyond the limit are no candidates.
+template <class W>
+VGMI_DEV double dist_in_step64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, double stepMax)
+{
+  const double e = extent_hit(s, p, d);
+  if (e >= stepMax) return e;
+  const float maxstep = stepMax < double(FLT_MAX) ? float(stepMax) : FLT_MAX;
+  return dist_in_of(s, facet_hits(s, w, p, d, kHitsIn, maxstep, stepMax), d);
+}
+
+template <class W>
+VGMI_DEV double dist_out_step64(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, double stepMax)
+{
+  const float maxstep = stepMax < double(FLT_MAX) ? float(stepMax) : FLT_MAX;
+  return dist_out_of(s, facet_hits(s, w, p, d, kHitsOut, maxstep, stepMax), d);
 }
 
 // Both distances along d from one search.

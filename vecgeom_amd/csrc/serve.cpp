@@ -79,6 +79,7 @@ struct vgmi_server {
   std::unordered_map<uint64_t, Request *> requests;
   uint64_t nextTicket = 1;
   bool stop           = false;
+  int waiters         = 0; // threads inside vgmi_server_wait (destroy waits for them to leave)
   std::thread dispatcher, completer;
   int device = 0;
   // statistics
@@ -384,6 +385,10 @@ vgmi_status vgmi_server_submit(vgmi_server *S, const double *px, const double *p
                         steps_out, Clock::now(), Clock::time_point(), 0, 0};
   {
     std::lock_guard<std::mutex> lk(S->m);
+    if (S->stop) {
+      delete r;
+      return set_error(VGMI_ERR_INVALID, "vgmi_server_submit: the server is being destroyed");
+    }
     if (n > S->lanes[0].cap || S->queuedTracks + n > S->queueCap) {
       S->rejected++;
       delete r;
@@ -407,13 +412,19 @@ vgmi_status vgmi_server_wait(vgmi_server *S, uint64_t ticket, double *latency_us
   if (it == S->requests.end()) return set_error(VGMI_ERR_INVALID, "vgmi_server_wait: unknown ticket %llu",
                                                 (unsigned long long)ticket);
   Request *r = it->second;
+  ++S->waiters;
   S->cvDone.wait(lk, [&] { return r->state == 2; });
+  --S->waiters;
   if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->tDone - r->tSubmit).count();
   const int st = r->status;
   S->requests.erase(it);
   delete r;
-  return st == VGMI_OK ? VGMI_OK : set_error(st, "vgmi_server_wait: the launch carrying ticket %llu failed",
-                                             (unsigned long long)ticket);
+  if (S->stop) S->cvDone.notify_all(); // vgmi_server_destroy waits for the last waiter to leave
+  if (st == VGMI_OK) return VGMI_OK;
+  if (st == VGMI_ERR_CAPACITY)
+    return set_error(st, "vgmi_server_wait: ticket %llu was never launched (the server was destroyed)",
+                     (unsigned long long)ticket);
+  return set_error(st, "vgmi_server_wait: the launch carrying ticket %llu failed", (unsigned long long)ticket);
 }
 
 vgmi_status vgmi_server_stats(vgmi_server *S, int64_t stats[4])
@@ -446,6 +457,21 @@ vgmi_status vgmi_server_destroy(vgmi_server *S)
   S->cvWork.notify_all();
   if (S->dispatcher.joinable()) S->dispatcher.join();
   if (S->completer.joinable()) S->completer.join();
+  {
+    // requests still queued are never launched: finish them with an error, wake their waiters and
+    // let every waiter leave before the server's memory goes away
+    std::unique_lock<std::mutex> lk(S->m);
+    const auto now = Clock::now();
+    for (auto &kv : S->requests)
+      if (kv.second->state != 2) {
+        kv.second->state  = 2;
+        kv.second->status = VGMI_ERR_CAPACITY;
+        kv.second->tDone  = now;
+      }
+    S->queue.clear();
+    S->cvDone.notify_all();
+    S->cvDone.wait(lk, [&] { return S->waiters == 0; });
+  }
   for (auto &L : S->lanes) {
     if (L.stream) (void)hipStreamSynchronize(L.stream);
     for (auto &g : L.graphs) (void)hipGraphExecDestroy(g.second);

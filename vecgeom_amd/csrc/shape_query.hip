@@ -9,6 +9,8 @@
 // is within the near-surface band in FP64, so decisions stay exact.
 #include "vgmi_internal.h"
 #include "dev_tess_tile.h"
+#include <algorithm>
+#include <atomic>
 
 namespace vgmi {
 
@@ -100,9 +102,42 @@ __global__ __launch_bounds__(kBlock) void k_tess_query(TessellatedData s, BatchO
   }
 }
 
+// A tessellated solid's DistanceToIn / DistanceToOut with a step limit per point
+// (TessellatedImplementation.h:191-199, 213-216, 250): one lane per point over the pool.
+__global__ __launch_bounds__(kBlock) void k_tess_dist_step(TessellatedData s, QueryArgs a,
+                                                           const double *__restrict__ step_max, int toIn)
+{
+  size_t i     = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  size_t strid = size_t(gridDim.x) * kBlock;
+  for (; i < a.n; i += strid) {
+    const Vec3<double> p = load3<double>(a.px, a.py, a.pz, i), d = load3<double>(a.dx, a.dy, a.dz, i);
+    const tess::V3d pp(p.x, p.y, p.z), dd(d.x, d.y, d.z);
+    const double sm = __builtin_nontemporal_load(step_max + i);
+    const double r  = toIn ? tess::dist_in_step64(s, tess::pool_walk(s), pp, dd, sm)
+                           : tess::dist_out_step64(s, tess::pool_walk(s), pp, dd, sm);
+    __builtin_nontemporal_store(r, (double *)a.out + i);
+  }
+}
+
+int tess_lds_limit()
+{
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return kTessLdsMax;
+  int v = cache[dev].load(std::memory_order_relaxed);
+  if (v > 0) return v;
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
+    lds = kLdsPerCU;
+  v = std::min(kTessLdsMax, lds / 2 - kStaticLds);
+  v = v > 1 ? v : 1;
+  cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
+
 static hipError_t launch_tess(TessellatedData const &s, BatchOut const &o, size_t n, hipStream_t st)
 {
-  const bool useLds = s.imageDoubles > 0 && s.imageDoubles * int(sizeof(double)) <= kTessLdsMax;
+  const bool useLds = s.imageDoubles > 0 && s.imageDoubles * int64_t(sizeof(double)) <= tess_lds_limit();
   int dev = 0, ncu = 256;
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
   const size_t tiles = (n + kBlock - 1) / kBlock, cap = size_t(ncu) * (useLds ? 2 : 8);
@@ -157,7 +192,6 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
                              const double *pz, const double *dx, const double *dy, const double *dz,
                              const double *step_max, void *out, size_t n, void *stream)
 {
-  (void)step_max; // none of the supported shapes uses step_max (it only bounds iterative solvers)
   if (!shape) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: null shape");
   if (op < 0 || op >= kNumOps) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: unknown op %d", op);
   if (precision != VGMI_FP64 && precision != VGMI_MIXED)
@@ -166,6 +200,16 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
   if (!px || !py || !pz || !out) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: null point/output pointer");
   if ((op == kOpDistIn || op == kOpDistOut) && (!dx || !dy || !dz))
     return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: DistanceToIn/Out need directions");
+  // step_max: of the supported shapes only the tessellated solid uses it (the reference's other
+  // kernels take the argument and ignore it); the tessellated solid answers every point in FP64
+  if (step_max && shape->rec.type == kTessellated && (op == kOpDistIn || op == kOpDistOut)) {
+    QueryArgs a{px, py, pz, dx, dy, dz, out, n};
+    hipLaunchKernelGGL(k_tess_dist_step, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                       shape->rec.u.tess, a, step_max, int(op == kOpDistIn));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_shape_query launch: %s", hipGetErrorString(e));
+    return VGMI_OK;
+  }
   // the tessellated solid has no FP32 pass (it answers every point in FP64), so MIXED takes the FP64 launch
   if (precision == VGMI_MIXED && shape->rec.type != kTessellated)
     return mixed_shape_query(shape, op, px, py, pz, dx, dy, dz, out, n, stream);
@@ -217,8 +261,10 @@ vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision, const d
   switch (shape->rec.type) {
   case kTessellated: {
     const BatchOut o{px, py, pz, dx, dy, dz, contains, inside, dist_in, dist_out, safety_in, safety_out};
-    (void)launch_tess(shape->rec.u.tess, o, n, st);
-    break;
+    const hipError_t e = launch_tess(shape->rec.u.tess, o, n, st);
+    if (e != hipSuccess)
+      return set_error(VGMI_ERR_DEVICE, "vgmi_shape_query_all launch: %s", hipGetErrorString(e));
+    return VGMI_OK;
   }
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \

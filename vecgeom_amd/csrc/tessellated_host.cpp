@@ -151,8 +151,11 @@ void Build(std::vector<HostFacet> &f, int b, int e, std::vector<BuildNode> &node
 int MakeTessellated(double const *a, int n, ShapeRecord &r, std::vector<double> &extra, std::string &err)
 {
   if (n < 1) return err = "tessellated needs ntri followed by 9 doubles per triangle", -1;
+  // at most 2^23 facets: every facet-array offset (facets x kFacetDoubles) stays an int
+  if (!(a[0] >= 4.0 && a[0] <= double(1 << 23)))
+    return err = "tessellated needs between 4 and 2^23 triangles", -1;
   const int ntri = int(a[0]);
-  if (ntri < 4 || n < 1 + 9 * ntri) return err = "tessellated needs at least 4 triangles of 9 doubles each", -1;
+  if (int64_t(n) < 1 + 9 * int64_t(ntri)) return err = "tessellated needs 9 doubles per triangle", -1;
   std::vector<HostFacet> f;
   for (int i = 0; i < ntri; ++i) {
     const double *q = a + 1 + 9 * i;

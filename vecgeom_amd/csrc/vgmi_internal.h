@@ -40,4 +40,7 @@ int set_error(int code, const char *fmt, ...);
 vgmi_status mixed_shape_query(const vgmi_shape *shape, int op, const double *px, const double *py, const double *pz,
                               const double *dx, const double *dy, const double *dz, void *out, size_t n,
                               void *stream);
+// Bytes of dynamic LDS a workgroup may give a tessellated solid's image on the current device: half
+// of the device's LDS per CU (two workgroups per CU) less the kernels' static LDS.
+int tess_lds_limit();
 } // namespace vgmi

@@ -12,6 +12,15 @@ import torch
 
 from ._lib import check, lib
 from .shapes import FP64, _stream
+from .soa import device_array, same_length
+
+_U32 = (torch.int32, torch.uint32) if hasattr(torch, "uint32") else (torch.int32,)
+
+
+def _limit(step_limit, n, dev):
+    if not torch.is_tensor(step_limit):
+        return torch.full((n,), float(step_limit), dtype=torch.float64, device=dev)
+    return device_array(step_limit, torch.float64, n, "step_limit")
 
 HOST_ONLY = 1
 
@@ -115,6 +124,7 @@ class Geometry:
         n = len(points)
         if out is None:
             out = torch.empty(n, dtype=torch.int32, device=points.x.device)
+        device_array(out, _U32, n, "out")
         check(lib().vgmi_nav_locate(self._h, precision, *points.ptrs(), out.data_ptr(), n, _stream()),
               "vgmi_nav_locate")
         return out
@@ -123,8 +133,11 @@ class Geometry:
         """Returns (step, nav_out, last_out, boundary) device tensors."""
         n = len(points)
         dev = points.x.device
-        if not torch.is_tensor(step_limit):
-            step_limit = torch.full((n,), float(step_limit), dtype=torch.float64, device=dev)
+        same_length(points, dirs)
+        step_limit = _limit(step_limit, n, dev)
+        device_array(nav_in, _U32, n, "nav_in")
+        if last_in is not None:
+            device_array(last_in, _U32, n, "last_in")
         step = torch.empty(n, dtype=torch.float64, device=dev)
         nav_out = torch.empty(n, dtype=torch.int32, device=dev)
         last_out = torch.empty(n, dtype=torch.int32, device=dev)
@@ -141,14 +154,20 @@ class Geometry:
         boundary, safety); safety is that of the start point where calc_safety (default: all)."""
         n = len(points)
         dev = points.x.device
-        if not torch.is_tensor(step_limit):
-            step_limit = torch.full((n,), float(step_limit), dtype=torch.float64, device=dev)
+        same_length(points, dirs)
+        step_limit = _limit(step_limit, n, dev)
+        device_array(nav_in, _U32, n, "nav_in")
+        if last_in is not None:
+            device_array(last_in, _U32, n, "last_in")
         step = torch.empty(n, dtype=torch.float64, device=dev)
         safety = torch.empty(n, dtype=torch.float64, device=dev)
         nav_out = torch.empty(n, dtype=torch.int32, device=dev)
         last_out = torch.empty(n, dtype=torch.int32, device=dev)
         bnd = torch.empty(n, dtype=torch.uint8, device=dev)
-        calc = calc_safety.to(torch.uint8) if calc_safety is not None else None
+        calc = None
+        if calc_safety is not None:
+            device_array(calc_safety, (torch.uint8, torch.bool), n, "calc_safety")
+            calc = calc_safety.to(torch.uint8)
         check(lib().vgmi_nav_step_safety(self._h, precision, *points.ptrs(), *dirs.ptrs(), step_limit.data_ptr(),
                                          nav_in.data_ptr(), last_in.data_ptr() if last_in is not None else None,
                                          calc.data_ptr() if calc is not None else None, nav_out.data_ptr(),
@@ -158,6 +177,7 @@ class Geometry:
 
     def ComputeSafety(self, points, nav, precision=FP64):
         n = len(points)
+        device_array(nav, _U32, n, "nav")
         out = torch.empty(n, dtype=torch.float64, device=points.x.device)
         check(lib().vgmi_nav_safety(self._h, precision, *points.ptrs(), nav.data_ptr(), out.data_ptr(), n,
                                     _stream()), "vgmi_nav_safety")
@@ -166,6 +186,11 @@ class Geometry:
     def Propagate(self, points, dirs, step_limit, nav, last, nsteps, taken=None, precision=FP64):
         """In-place transport loop on the device (points, nav, last are updated)."""
         n = len(points)
+        same_length(points, dirs)
+        device_array(nav, _U32, n, "nav")
+        device_array(last, _U32, n, "last")
+        if taken is not None:
+            device_array(taken, torch.int32, n, "taken")
         check(lib().vgmi_nav_propagate(self._h, precision, *points.ptrs(), *dirs.ptrs(), float(step_limit),
                                        nav.data_ptr(), last.data_ptr(),
                                        taken.data_ptr() if taken is not None else None, n, int(nsteps), _stream()),

@@ -10,7 +10,7 @@ import ctypes
 import torch
 
 from ._lib import check, lib
-from .soa import SOA3D
+from .soa import SOA3D, device_array, same_length
 
 BOX, TUBE, CONE, POLYCONE, POLYHEDRON, TRAPEZOID, SPHERE, TORUS2, ORB, TESSELLATED, HYPE = range(11)
 OPS = {"Contains": 0, "Inside": 1, "DistanceToIn": 2, "DistanceToOut": 3, "SafetyToIn": 4, "SafetyToOut": 5}
@@ -55,9 +55,16 @@ class UnplacedShape:
     def _query(self, op, points, dirs=None, step_max=None, out=None, precision=FP64):
         n = len(points)
         code = OPS[op]
+        if code in (2, 3) and dirs is None:
+            raise ValueError("%s needs directions" % op)
+        same_length(points, dirs)
+        odt = _OUT_DTYPE.get(code, torch.float64)
         if out is None:
-            out = torch.empty(n, dtype=_OUT_DTYPE.get(code, torch.float64), device=points.x.device)
+            out = torch.empty(n, dtype=odt, device=points.x.device)
+        device_array(out, odt, n, "out")
         d = dirs.ptrs() if dirs is not None else (None, None, None)
+        if step_max is not None:
+            device_array(step_max, torch.float64, n, "step_max")
         sm = step_max.data_ptr() if step_max is not None else None
         check(lib().vgmi_shape_query(self._h, code, precision, *points.ptrs(), *d, sm, out.data_ptr(), n,
                                      _stream()), "vgmi_shape_query(%s)" % op)
@@ -84,6 +91,7 @@ class UnplacedShape:
     def QueryAll(self, points, dirs, precision=FP64):
         """All six methods in one pass; returns a dict of device tensors."""
         n, dev = len(points), points.x.device
+        same_length(points, dirs)
         res = {"Contains": torch.empty(n, dtype=torch.uint8, device=dev),
                "Inside": torch.empty(n, dtype=torch.int32, device=dev)}
         for k in ("DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"):
