@@ -244,6 +244,20 @@ VGMI_API vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, do
                                         void *stream);
 
 /*
+ * vgmi_nav_propagate that allocates nothing. No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). The regrouping workspace is a fixed device array of the library (8.6 MB per GPU, held
+ * until the process ends) instead of a per-call allocation, so at most 2^21 tracks and 2^16 logical
+ * volumes (VGMI_ERR_CAPACITY beyond). Every call on one GPU uses that GPU's one array: calls that
+ * overlap on one GPU (other streams, other threads) race and must be serialised by the caller.
+ * Results are those of vgmi_nav_propagate.
+ */
+VGMI_API vgmi_status vgmi_nav_propagate_noalloc(const vgmi_geom *geom, int precision, double *px, double *py,
+                                                double *pz, const double *dx, const double *dy, const double *dz,
+                                                double step_limit, uint32_t *nav, uint32_t *last,
+                                                int32_t *steps_taken, size_t n, int nsteps, void *stream);
+
+/*
  * Regrouping of the fused transport loops (vgmi_nav_propagate, the serving loop) of at least
  * 65536 tracks: every chunk_steps steps the tracks are counting-sorted on the device by the
  * logical volume they are in, so the lanes of a wavefront run the same volume's code; 0

@@ -1,6 +1,7 @@
 // Navigation kernels and their C ABI entry points (one lane per track).
 #include "dev_nav.h"
 #include <algorithm>
+#include <mutex>
 #include "geom_internal.h"
 #include "vgmi_internal.h"
 
@@ -446,6 +447,73 @@ vgmi_status vgmi_nav_propagate(const vgmi_geom *geom, int precision, double *px,
   const vgmi_status rc = launch_transport(geom, a, work, st);
   if (work) (void)hipFreeAsync(work, st);
   return rc;
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------ transport loop without allocation
+// Workspace of vgmi_nav_propagate_noalloc: the regrouping histogram and permutation (nlogical + 1 + n
+// words) live in this fixed device array instead of a per-call allocation. Each device holds its own
+// copy; host code addresses it through the pointer hipGetSymbolAddress returns for the current device
+// (never through the array's name, which on the host is only a stand-in).
+namespace {
+constexpr size_t kNoAllocMaxTracks  = size_t(1) << 21;
+constexpr size_t kNoAllocMaxLogical = size_t(1) << 16;
+constexpr size_t kNoAllocWords      = kNoAllocMaxTracks + kNoAllocMaxLogical + 1;
+constexpr int kMaxDevices           = 64;
+__device__ __align__(256) uint32_t scratch[kNoAllocWords];
+std::mutex g_scratch_mutex;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// This device's copy of `scratch`, resolved once per device with that device current.
+uint32_t *scratch_for_current_device(vgmi_status &rc)
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    rc = set_error(VGMI_ERR_DEVICE, "vgmi_nav_propagate_noalloc: no current device (or device index >= %d)",
+                   kMaxDevices);
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr           = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      rc = set_error(VGMI_ERR_DEVICE, "vgmi_nav_propagate_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<uint32_t *>(ptr);
+  }
+  rc = VGMI_OK;
+  return g_scratch_ptr[dev];
+}
+} // namespace
+
+extern "C" {
+
+vgmi_status vgmi_nav_propagate_noalloc(const vgmi_geom *geom, int precision, double *px, double *py, double *pz,
+                                       const double *dx, const double *dy, const double *dz, double step_limit,
+                                       uint32_t *nav, uint32_t *last, int32_t *steps_taken, size_t n, int nsteps,
+                                       void *stream)
+{
+  if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_propagate_noalloc")) return s;
+  if (n == 0 || nsteps <= 0) return VGMI_OK;
+  if (!px || !py || !pz || !dx || !dy || !dz || !nav || !last)
+    return set_error(VGMI_ERR_INVALID, "vgmi_nav_propagate_noalloc: null pointer");
+  if (n > kNoAllocMaxTracks)
+    return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_propagate_noalloc: %zu tracks, at most %zu", n, kNoAllocMaxTracks);
+  const size_t words = transport_work_words(geom, n);
+  if (words > kNoAllocWords)
+    return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_propagate_noalloc: %d logical volumes, at most %zu",
+                     geom->nlogical, kNoAllocMaxLogical);
+  uint32_t *work = nullptr;
+  if (words) {
+    vgmi_status rc = VGMI_OK;
+    work           = scratch_for_current_device(rc);
+    if (!work) return rc;
+  }
+  TransportArgs a{px, py, pz, dx, dy, dz, step_limit, nav, last, steps_taken, nullptr, n, nsteps, precision, nullptr};
+  return launch_transport(geom, a, work, (hipStream_t)stream);
 }
 
 vgmi_status vgmi_geom_set_lanes_per_wave(vgmi_geom *geom, int lanes)
