@@ -114,7 +114,10 @@ static double BoxHit(V3 const &lo, V3 const &hi, V3 const &p, V3 const &inv, dou
 // HybridNavigator2::GetHitCandidates_v calls it (:111-150): x and y differences in float,
 // the z difference in double, each product by the double inverse direction rounded to float;
 // t0 = 0, t1 = maxstep (float). Returns whether the box is a candidate.
-static bool FacetBoxCandidate(TriFacet const &t, V3 const &p, V3 const &inv, float maxstep)
+// The user hook then stops at the first box (in distance order) entered beyond
+// min(stepMax, distance) (TessellatedImplementation.h:213-216): a box entered beyond stepMax is
+// no candidate (boxes beyond the current distance cannot change the result).
+static bool FacetBoxCandidate(TriFacet const &t, V3 const &p, V3 const &inv, float maxstep, double stepMax)
 {
   const int sx = inv.x < 0, sy = inv.y < 0, sz = inv.z < 0;
   const float cx[2] = {t.blo[0], t.bhi[0]}, cy[2] = {t.blo[1], t.bhi[1]}, cz[2] = {t.blo[2], t.bhi[2]};
@@ -132,7 +135,7 @@ static bool FacetBoxCandidate(TriFacet const &t, V3 const &p, V3 const &inv, flo
   tmin = tmin > tzmin ? tmin : tzmin;
   tmax = tmax < tzmax ? tmax : tzmax;
   if (!(tmin <= float(double(maxstep) + kTolerance) && tmax > float(0.0 - kTolerance))) return false;
-  return tmin < maxstep; // "hit = distance < maxstep"
+  return tmin < maxstep && !(double(tmin) > stepMax); // "hit = distance < maxstep", then the hook
 }
 
 // TessellatedCluster::DistanceToCluster for one facet (:232-283)
@@ -166,7 +169,7 @@ void Tessellated::DistanceToSolid(bool toIn, V3 const &p, V3 const &d, double st
   int sIn = -1, sOut = -1;
   const float maxstep = std::min(float(stepMax), FLT_MAX);
   for (int k = 0; k < int(f.size()); ++k) {
-    if (!FacetBoxCandidate(f[k], p, inv, maxstep)) continue;
+    if (!FacetBoxCandidate(f[k], p, inv, maxstep, stepMax)) continue;
     double ci, co;
     bool vin, vout;
     DistanceToFacet(f[k], p, d, ci, co, vin, vout);
