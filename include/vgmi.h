@@ -193,6 +193,35 @@ VGMI_API vgmi_status vgmi_nav_locate(const vgmi_geom *geom, int precision, const
                                      const double *pz, uint32_t *nav_out, size_t n, void *stream);
 
 /*
+ * Level locate per point: VLevelLocator::LevelLocate(lvol, localpoint, pvol, daughterlocalpoint) and
+ * LevelLocateExclVol(lvol, exclvol, ...) of the BVH level locator (navigation/BVHLevelLocator.h:44-77,
+ * base/BVH.h:219-344). logical[i] is the logical volume the point (in its frame) is in; excl_placed
+ * (may be NULL) the placed volume to skip, -1 for none. daughter_out[i] is the placed volume of the first
+ * daughter (BVH order) containing the point, -1 when none does or logical[i] is out of range; dpx/dpy/dpz
+ * (all or none NULL) receive the point in that daughter's frame (the input point when none).
+ */
+VGMI_API vgmi_status vgmi_nav_level_locate(const vgmi_geom *geom, int precision, const int32_t *logical,
+                                           const int32_t *excl_placed, const double *px, const double *py,
+                                           const double *pz, int32_t *daughter_out, double *dpx, double *dpy,
+                                           double *dpz, size_t n, void *stream);
+
+/*
+ * Placed-volume SOA queries: VPlacedVolume::Contains / Inside / DistanceToIn / DistanceToOut /
+ * SafetyToIn / SafetyToOut over SOA3D (volumes/PlacedVolume.h:273,299,332,372,405,421) and
+ * PlacedDistanceToOut (op VGMI_OP_PLACED_DIST_OUT), dispatched as CommonSpecializedVolImplHelper
+ * (volumes/SpecializedPlacedVolImplHelper.h:98-179): Contains, Inside, DistanceToIn, SafetyToIn and
+ * PlacedDistanceToOut take points (and directions) in the frame the volume is placed in and transform
+ * them with its placement; DistanceToOut and SafetyToOut take points in the volume's own frame
+ * (PlacedVolume.h:335-345, 408-416). placed_id is a placement index of the description, or the
+ * world's id (vgmi_geom_info counts[4]). Other arguments as vgmi_shape_query.
+ */
+#define VGMI_OP_PLACED_DIST_OUT 6
+VGMI_API vgmi_status vgmi_placed_query(const vgmi_geom *geom, int placed_id, int op, int precision,
+                                       const double *px, const double *py, const double *pz, const double *dx,
+                                       const double *dy, const double *dz, const double *step_max, void *out,
+                                       size_t n, void *stream);
+
+/*
  * One navigation step per track: VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState
  * (relocate = 1) or ComputeStep (relocate = 0) (navigation/VNavigator.h:436-540).
  * Inputs are global points/directions, per-track step limits and in-states;

@@ -100,12 +100,13 @@ public:
                      bool relocate) const;
   double ComputeSafety(V3 const &gp, NavState const &in) const;
   uint32_t LocateGlobalPoint(int pvol, V3 const &p, uint32_t path, bool top) const;
+  // BVH::LevelLocate with an excluded placed volume (VecGeom/base/BVH.h:219-344)
+  bool LevelLocate(LVol const &lv, int excl, V3 const &lp, int &pvol, V3 &dlp) const;
 
 private:
   void BuildBVH(LVol &lv);
   void ComputeNodes(BVHo &b, unsigned id, int *first, int *last, unsigned nodes);
   void CheckDaughters(LVol const &lv, V3 const &lp, V3 const &ld, double &step, int last, int &hit) const;
-  bool LevelLocate(LVol const &lv, int excl, V3 const &lp, int &pvol, V3 &dlp) const;
   uint32_t LocateExcl(int pvol, int excl, V3 const &p, uint32_t path) const;
   uint32_t RelocateForceDifferent(V3 const &lp, uint32_t path) const;
   double SafetyToDaughters(LVol const &lv, V3 const &lp, double safety) const;

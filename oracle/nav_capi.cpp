@@ -183,4 +183,61 @@ int64_t vgo_nav_propagate(void *gp, double *px, double *py, double *pz, const do
   return tot;
 }
 
+// VLevelLocator::LevelLocate / LevelLocateExclVol (VecGeom/navigation/BVHLevelLocator.h:44-77) per
+// point: lv[i] the logical volume, excl[i] the placed volume to skip (-1: none; excl may be null).
+int vgo_nav_level_locate(void *gp, const int32_t *lv, const int32_t *excl, const double *px, const double *py,
+                         const double *pz, int32_t *out, double *dx, double *dy, double *dz, int64_t n, int threads)
+{
+  Geometry *g = (Geometry *)gp;
+  parallel_for(n, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      const V3 lp(px[i], py[i], pz[i]);
+      int pv = -1;
+      V3 dlp = lp;
+      const int l = lv[i];
+      if (l >= 0 && l < int(g->lvs.size()) && !g->lvs[l].daughters.empty()) {
+        if (!g->LevelLocate(g->lvs[l], excl ? excl[i] : -1, lp, pv, dlp)) pv = -1, dlp = lp;
+      }
+      out[i] = pv;
+      if (dx) dx[i] = dlp.x, dy[i] = dlp.y, dz[i] = dlp.z;
+    }
+  });
+  return 0;
+}
+
+// Placed-volume queries as CommonSpecializedVolImplHelper dispatches them
+// (VecGeom/volumes/SpecializedPlacedVolImplHelper.h:98-179): Contains (0), Inside (1), DistanceToIn (2),
+// SafetyToIn (4) and PlacedDistanceToOut (6) transform the point (Transformation3D::Transform) and the
+// direction (TransformDirection); DistanceToOut (3) and SafetyToOut (5) take local coordinates
+// (VecGeom/volumes/PlacedVolume.h:335-345, 408-416).
+int vgo_placed_query(void *gp, int pvol, int op, const double *px, const double *py, const double *pz,
+                     const double *dx, const double *dy, const double *dz, const double *step, void *out, int64_t n,
+                     int threads)
+{
+  Geometry *g = (Geometry *)gp;
+  if (pvol < 0 || pvol >= int(g->pvols.size()) || op < 0 || op > 6) return -1;
+  Transform const &tr = g->pvols[pvol].tr;
+  Shape const &s      = g->ShapeOf(pvol);
+  const bool xf       = op != 3 && op != 5;
+  parallel_for(n, threads, [&](int64_t b, int64_t e) {
+    for (int64_t i = b; i < e; ++i) {
+      V3 p(px[i], py[i], pz[i]);
+      if (xf) p = tr.Transform3(p);
+      switch (op) {
+      case 0: static_cast<uint8_t *>(out)[i] = s.Contains(p) ? 1 : 0; break;
+      case 1: static_cast<int32_t *>(out)[i] = s.Inside(p); break;
+      case 4: static_cast<double *>(out)[i] = s.SafetyToIn(p); break;
+      case 5: static_cast<double *>(out)[i] = s.SafetyToOut(p); break;
+      default: {
+        V3 d(dx[i], dy[i], dz[i]);
+        if (xf) d = tr.TransformDirection(d);
+        const double sm = step ? step[i] : kInfLength;
+        static_cast<double *>(out)[i] = op == 2 ? s.DistanceToIn(p, d, sm) : s.DistanceToOut(p, d, sm);
+      }
+      }
+    }
+  });
+  return 0;
+}
+
 } // extern "C"

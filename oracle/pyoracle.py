@@ -110,6 +110,10 @@ class OracleGeometry:
             L.vgo_nav_step.argtypes = [vp] + [vp] * 13 + [i64, ci, ci]
             L.vgo_nav_safety.restype = ci
             L.vgo_nav_safety.argtypes = [vp] + [vp] * 5 + [i64, ci]
+            L.vgo_nav_level_locate.restype = ci
+            L.vgo_nav_level_locate.argtypes = [vp] + [vp] * 9 + [i64, ci]
+            L.vgo_placed_query.restype = ci
+            L.vgo_placed_query.argtypes = [vp, ci, ci] + [vp] * 8 + [i64, ci]
             L.vgo_nav_propagate.restype = i64
             L.vgo_nav_propagate.argtypes = [vp] + [vp] * 6 + [ctypes.c_double, vp, vp, i64, ci, ci]
             L._geom_declared = True
@@ -183,6 +187,39 @@ class OracleGeometry:
         out = np.zeros(n)
         px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
         lib().vgo_nav_safety(self._h, _np_ptr(px), _np_ptr(py), _np_ptr(pz), _np_ptr(nav), _np_ptr(out), n, threads)
+        return out
+
+    def level_locate(self, lv, p, excl=None, threads=1):
+        """BVH LevelLocate per point in logical volume lv[i]: (placed daughter or -1, daughter-frame point)."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        lv = np.ascontiguousarray(np.broadcast_to(np.asarray(lv, dtype=np.int32), (n,)))
+        ex = None if excl is None else np.ascontiguousarray(np.broadcast_to(np.asarray(excl, dtype=np.int32), (n,)))
+        out = np.zeros(n, np.int32)
+        dl = np.zeros((3, n))
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        dx, dy, dz = (np.ascontiguousarray(dl[i]) for i in range(3))
+        lib().vgo_nav_level_locate(self._h, _np_ptr(lv), _np_ptr(ex) if ex is not None else None, _np_ptr(px),
+                                   _np_ptr(py), _np_ptr(pz), _np_ptr(out), _np_ptr(dx), _np_ptr(dy), _np_ptr(dz), n,
+                                   threads)
+        return out, np.stack([dx, dy, dz])
+
+    def placed_query(self, pvol, op, p, d=None, step=None, threads=1):
+        """Placed-volume query (op as OPS, or 6 = PlacedDistanceToOut) on SOA (3, n) arrays."""
+        code = OPS[op] if isinstance(op, str) else int(op)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        out = np.empty(n, {0: np.uint8, 1: np.int32}.get(code, np.float64))
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        dx = dy = dz = None
+        if d is not None:
+            d = np.ascontiguousarray(d, dtype=np.float64)
+            dx, dy, dz = (np.ascontiguousarray(d[i]) for i in range(3))
+        st = None if step is None else np.ascontiguousarray(step, dtype=np.float64)
+        rc = lib().vgo_placed_query(self._h, int(pvol), code, _np_ptr(px), _np_ptr(py), _np_ptr(pz),
+                                    _dptr(dx), _dptr(dy), _dptr(dz), _dptr(st), _np_ptr(out), n, threads)
+        if rc != 0:
+            raise ValueError("vgo_placed_query rejected placed volume %d / op %d" % (pvol, code))
         return out
 
     def propagate(self, p, d, limit, nav, last, nsteps, threads=1):

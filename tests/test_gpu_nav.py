@@ -157,3 +157,101 @@ def test_propagate_matches_oracle(geo, regroup, lanes, prec):
     assert np.array_equal(ls.cpu().numpy().astype(np.uint32), rl)
     assert int(taken.sum()) == rtaken
     assert np.array_equal(P.cpu(), rp), float(np.max(np.abs(P.cpu() - rp)))
+
+
+def _placed_cases(g):
+    """Placement ids of a layer tube, a rotated trapezoid cell and a box plate, and the world."""
+    from vecgeom_amd.shapes import BOX, TRAPEZOID, TUBE
+    want, out = {TUBE: None, TRAPEZOID: None, BOX: None}, []
+    for k, (m, d, t) in enumerate(g.placements):
+        st = g.shapes[g.logicals[d]][0]
+        rotated = any(abs(a - b) > 1e-12 for a, b in zip(t[3:], geometry.IDENTITY[3:]))
+        if st in want and want[st] is None and (st != TRAPEZOID or rotated):
+            want[st] = k
+    return [k for k in want.values() if k is not None] + [len(g.placements)]
+
+
+def _placed_points(g, pid, n, seed):
+    """Points in the placed volume's frame (half in 1.3x its extent, half near its surface) and the
+    same points in its mother's frame (master = R local + t)."""
+    from oracle.pyoracle import OracleShape, SHAPE_TYPES
+    from vecgeom_amd import workloads
+    names = {v: k for k, v in SHAPE_TYPES.items()}
+    lv = g.world if pid == len(g.placements) else g.placements[pid][1]
+    st, params = g.shapes[g.logicals[lv]]
+    o = OracleShape(names[st], params)
+    lo, hi = o.extent()
+    rg = workloads.rng(seed)
+    half = n // 2
+    loc = np.concatenate([workloads.uniform_in_box(rg, half, lo, hi, scale=1.3),
+                          workloads.near_surface(rg, n - half, lo, hi, lambda q: o.query("Contains", q),
+                                                 lambda q, e: o.query("DistanceToOut", q, e))], axis=1)
+    d = workloads.isotropic_dirs(rg, loc.shape[1])
+    t = np.array(geometry.IDENTITY if pid == len(g.placements) else g.placements[pid][2])
+    R = t[3:].reshape(3, 3)
+    master = t[:3, None] + R @ loc
+    dm = R @ d
+    return np.ascontiguousarray(loc), np.ascontiguousarray(d), np.ascontiguousarray(master), np.ascontiguousarray(dm)
+
+
+@PRECISIONS
+def test_placed_queries(geo, prec):
+    """VPlacedVolume SOA queries (SpecializedPlacedVolImplHelper.h:98-179) against the oracle's placed
+    dispatch: FP64 bit for bit; mixed: decisions identical, certified distances within 2.5e-4 mm."""
+    g, G, O = geo
+    for pid in _placed_cases(g):
+        loc, dl, mas, dm = _placed_points(g, pid, 1 << 13, seed=40 + pid % 97)
+        print("placed volume", pid, flush=True)
+        for op in ("Contains", "Inside", "DistanceToIn", "SafetyToIn", "PlacedDistanceToOut", "DistanceToOut",
+                   "SafetyToOut"):
+            local = op in ("DistanceToOut", "SafetyToOut")
+            p, d = (loc, dl) if local else (mas, dm)
+            directed = "Distance" in op
+            got = G.PlacedQuery(pid, op, SOA3D.from_array(p), SOA3D.from_array(d) if directed else None,
+                                precision=prec).cpu().numpy()
+            ref = O.placed_query(pid, 6 if op == "PlacedDistanceToOut" else op, p, d if directed else None)
+            if prec == FP64 or op in ("Contains", "Inside"):
+                assert np.array_equal(got, ref), (pid, op, int((got != ref).sum()))
+            else:
+                assert np.array_equal(got < 0, ref < 0) and np.array_equal(got >= 1e300, ref >= 1e300), (pid, op)
+                fin = (ref < 1e300) & (ref >= 0)
+                assert np.max(np.abs(got[fin] - ref[fin]), initial=0) <= 2.5e-4, (pid, op)
+        # the placed dispatch equals the unplaced shape on the local points (pins the transform)
+        from oracle.pyoracle import OracleShape, SHAPE_TYPES
+        lv = g.world if pid == len(g.placements) else g.placements[pid][1]
+        st, params = g.shapes[g.logicals[lv]]
+        un = OracleShape({v: k for k, v in SHAPE_TYPES.items()}[st], params)
+        ins_loc = un.query("Inside", loc)
+        ins_mas = O.placed_query(pid, "Inside", mas)
+        assert (ins_loc == ins_mas).mean() > 0.999, pid  # equal up to the rounding of the transform
+
+
+@PRECISIONS
+def test_level_locate(geo, prec):
+    """BVHLevelLocator::LevelLocate and LevelLocateExclVol per point against the oracle: the daughter
+    found and the point in its frame, bit for bit; with the found daughter excluded, the next one."""
+    from oracle.pyoracle import OracleShape, SHAPE_TYPES
+    from vecgeom_amd import workloads
+    g, G, O = geo
+    names = {v: k for k, v in SHAPE_TYPES.items()}
+    mothers = sorted({m for m, _, _ in g.placements})
+    lvs = [g.world] + [m for m in mothers if m != g.world][:3]
+    for k, lv in enumerate(lvs):
+        st, params = g.shapes[g.logicals[lv]]
+        lo, hi = OracleShape(names[st], params).extent()
+        p = workloads.uniform_in_box(workloads.rng(60 + k), 1 << 14, lo, hi, scale=1.0)
+        P = SOA3D.from_array(p)
+        got, dl = G.LevelLocate(lv, P, precision=prec)
+        ref, rdl = O.level_locate(lv, p)
+        got = got.cpu().numpy()
+        assert np.array_equal(got, ref), (lv, int((got != ref).sum()))
+        assert (ref >= 0).mean() > 0.01, lv
+        assert np.array_equal(dl.cpu(), rdl), lv
+        excl = torch.as_tensor(ref, dtype=torch.int32, device="cuda")
+        got2, dl2 = G.LevelLocate(lv, P, excl=excl, precision=prec)
+        ref2, rdl2 = O.level_locate(lv, p, excl=ref)
+        assert np.array_equal(got2.cpu().numpy(), ref2), lv
+        assert np.array_equal(dl2.cpu(), rdl2), lv
+    # an out-of-range logical volume finds nothing
+    bad, _ = G.LevelLocate(len(g.logicals) + 5, SOA3D.from_array(np.zeros((3, 4))), precision=prec)
+    assert (bad.cpu().numpy() == -1).all()

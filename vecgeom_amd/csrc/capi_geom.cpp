@@ -130,6 +130,7 @@ int upload(vgmi_geom &G)
     ShapeRecord *recs  = reinterpret_cast<ShapeRecord *>(staging.data() + parts[0].off);
     for (size_t i = 0; i < h.shapes.size(); ++i)
       if (h.extraOffset[i] >= 0) SetShapeExtraBase(recs[i], pool + h.extraOffset[i]);
+    G.devShapes.assign(recs, recs + h.shapes.size());
   }
   e = hipMemcpy(blk, staging.data(), total, hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -155,6 +156,7 @@ int upload(vgmi_geom &G)
   v.slots32        = static_cast<const NavSlot32 *>(at(15));
   v.worldPvol      = h.worldPvol;
   v.maxDepth       = h.maxDepth;
+  v.nlogical       = int32_t(h.lvs.size());
   G.device_block   = blk;
   G.device_bytes   = total;
   return VGMI_OK;

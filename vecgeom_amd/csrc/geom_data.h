@@ -95,6 +95,7 @@ struct GeomView {
   const NavSlot32 *slots32;
   int32_t worldPvol;
   int32_t maxDepth;
+  int32_t nlogical;
 };
 
 } // namespace vgmi

@@ -34,6 +34,7 @@ struct vgmi_geom {
   void *device_block = nullptr;
   size_t device_bytes = 0;
   int device = -1;
+  std::vector<vgmi::ShapeRecord> devShapes; // shape records as uploaded (variable data at device addresses)
   unsigned typeMask = 0; // bit t set when a shape of ShapeType t is present
   int nlogical = 0;
   int regroupSteps = 10; // transport loops regroup tracks by volume every this many steps (0: never)

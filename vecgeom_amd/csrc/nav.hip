@@ -36,6 +36,43 @@ __global__ __launch_bounds__(kNavBlock) void k_locate(GeomView g, const double *
   out[i] = locate_global<TM>(g, V3(px[i], py[i], pz[i]), stk);
 }
 
+// BVHLevelLocator::LevelLocate / LevelLocateExclVol (VecGeom/navigation/BVHLevelLocator.h:44-77 over
+// BVH::LevelLocate, VecGeom/base/BVH.h:219-344): the first daughter of logical volume lv[i] (BVH order)
+// whose shape contains the point, skipping the placed volume excl[i]; -1 when none does (or lv[i] is not
+// a logical volume).
+struct LevelArgs {
+  const int32_t *lv, *excl;
+  const double *px, *py, *pz;
+  int32_t *out;
+  double *dx, *dy, *dz; // the point in the daughter's frame (optional)
+  size_t n;
+};
+
+template <unsigned TM>
+__global__ __launch_bounds__(kNavBlock) void k_level_locate(GeomView g, LevelArgs a)
+{
+  VGMI_NAV_STACK;
+  const size_t i = size_t(blockIdx.x) * kNavBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int lv = a.lv[i];
+  int pv       = -1;
+  V3 dlp(0., 0., 0.);
+  const V3 lp(a.px[i], a.py[i], a.pz[i]);
+  if (lv >= 0 && lv < g.nlogical && g.lvs[lv].nNodes > 0) {
+    NavVol M;
+    M.nodeBase = g.lvs[lv].nodeBase;
+    M.slotBase = g.lvs[lv].dOffset;
+    if (!level_locate<TM>(g, M, a.excl ? a.excl[i] : -1, lp, pv, dlp, stk)) pv = -1;
+  }
+  a.out[i] = pv;
+  if (a.dx) {
+    if (pv < 0) dlp = lp;
+    a.dx[i] = dlp.x;
+    a.dy[i] = dlp.y;
+    a.dz[i] = dlp.z;
+  }
+}
+
 struct StepArgs {
   const double *px, *py, *pz, *dx, *dy, *dz, *limit;
   const uint32_t *nav_in, *last_in;
@@ -385,6 +422,21 @@ vgmi_status vgmi_nav_locate(const vgmi_geom *geom, int precision, const double *
   if (!px || !py || !pz || !nav_out) return set_error(VGMI_ERR_INVALID, "vgmi_nav_locate: null pointer");
   VGMI_NAV_LAUNCH(k_locate, geom, precision, n, stream, geom->view, px, py, pz, nav_out, n);
   return launched("vgmi_nav_locate");
+}
+
+vgmi_status vgmi_nav_level_locate(const vgmi_geom *geom, int precision, const int32_t *logical,
+                                  const int32_t *excl_placed, const double *px, const double *py, const double *pz,
+                                  int32_t *daughter_out, double *dpx, double *dpy, double *dpz, size_t n, void *stream)
+{
+  if (vgmi_status s = check_geom(geom, precision, "vgmi_nav_level_locate")) return s;
+  if (n == 0) return VGMI_OK;
+  if (!logical || !px || !py || !pz || !daughter_out)
+    return set_error(VGMI_ERR_INVALID, "vgmi_nav_level_locate: null pointer");
+  if ((dpx || dpy || dpz) && !(dpx && dpy && dpz))
+    return set_error(VGMI_ERR_INVALID, "vgmi_nav_level_locate: give all three daughter-frame arrays or none");
+  LevelArgs a{logical, excl_placed, px, py, pz, daughter_out, dpx, dpy, dpz, n};
+  VGMI_NAV_LAUNCH(k_level_locate, geom, precision, n, stream, geom->view, a);
+  return launched("vgmi_nav_level_locate");
 }
 
 vgmi_status vgmi_nav_step(const vgmi_geom *geom, int precision, const double *px, const double *py, const double *pz,

@@ -12,7 +12,7 @@ import torch
 
 from ._lib import check, lib
 from .shapes import FP64, _stream
-from .soa import device_array, same_length
+from .soa import SOA3D, device_array, same_length
 
 _U32 = (torch.int32, torch.uint32) if hasattr(torch, "uint32") else (torch.int32,)
 
@@ -41,6 +41,9 @@ def _declare():
         "vgmi_nav_safety": [vp, ci] + [vp] * 5 + [sz, vp],
         "vgmi_nav_step_safety": [vp, ci] + [vp] * 15 + [sz, vp],
         "vgmi_nav_propagate": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
+        "vgmi_nav_level_locate": [vp, ci] + [vp] * 9 + [sz, vp],
+        "vgmi_placed_query": [vp, ci, ci, ci] + [vp] * 8 + [sz, vp],
+        "vgmi_nav_propagate_noalloc": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
         "vgmi_geom_set_regroup": [vp, ci],
         "vgmi_geom_set_lanes_per_wave": [vp, ci],
     }
@@ -127,6 +130,46 @@ class Geometry:
         device_array(out, _U32, n, "out")
         check(lib().vgmi_nav_locate(self._h, precision, *points.ptrs(), out.data_ptr(), n, _stream()),
               "vgmi_nav_locate")
+        return out
+
+    def LevelLocate(self, logical, points, excl=None, precision=FP64):
+        """VLevelLocator::LevelLocate / LevelLocateExclVol (VecGeom/navigation/BVHLevelLocator.h:44-77) per
+        point: logical (int32 tensor or int) is the logical volume the point (in its frame) is in, excl
+        (int32 tensor, int or None) the placed volume to skip. Returns (daughter placed ids, -1 for none;
+        the points in the daughters' frames as an SOA3D)."""
+        n = len(points)
+        dev = points.x.device
+        if not torch.is_tensor(logical):
+            logical = torch.full((n,), int(logical), dtype=torch.int32, device=dev)
+        device_array(logical, torch.int32, n, "logical")
+        if excl is not None and not torch.is_tensor(excl):
+            excl = torch.full((n,), int(excl), dtype=torch.int32, device=dev)
+        if excl is not None:
+            device_array(excl, torch.int32, n, "excl")
+        out = torch.empty(n, dtype=torch.int32, device=dev)
+        dl = SOA3D.empty(n, device=dev)
+        check(lib().vgmi_nav_level_locate(self._h, precision, logical.data_ptr(),
+                                          excl.data_ptr() if excl is not None else None, *points.ptrs(),
+                                          out.data_ptr(), *dl.ptrs(), n, _stream()), "vgmi_nav_level_locate")
+        return out, dl
+
+    def PlacedQuery(self, placed_id, op, points, dirs=None, step_max=None, precision=FP64):
+        """VPlacedVolume SOA query of placement placed_id (the world: len(placements)); op is a method name
+        of OPS or "PlacedDistanceToOut". Contains, Inside, DistanceToIn, SafetyToIn and PlacedDistanceToOut
+        take points in the frame the volume is placed in; DistanceToOut and SafetyToOut in its own frame."""
+        from .shapes import OPS
+        code = 6 if op == "PlacedDistanceToOut" else OPS[op]
+        n = len(points)
+        same_length(points, dirs)
+        if code in (2, 3, 6) and dirs is None:
+            raise ValueError("%s needs directions" % op)
+        out = torch.empty(n, dtype={0: torch.uint8, 1: torch.int32}.get(code, torch.float64), device=points.x.device)
+        if step_max is not None:
+            device_array(step_max, torch.float64, n, "step_max")
+        d = dirs.ptrs() if dirs is not None else (None, None, None)
+        check(lib().vgmi_placed_query(self._h, int(placed_id), code, precision, *points.ptrs(), *d,
+                                      step_max.data_ptr() if step_max is not None else None, out.data_ptr(), n,
+                                      _stream()), "vgmi_placed_query(%s)" % op)
         return out
 
     def ComputeStep(self, points, dirs, step_limit, nav_in, last_in=None, relocate=True, precision=FP64):
