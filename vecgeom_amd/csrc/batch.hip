@@ -1,8 +1,9 @@
 // One-launch batched queries over many shapes with uneven point counts.
 //
 // The points of shape s occupy [offset_s, offset_s + count_s) of the SOA arrays.
-// The range is cut into tiles of `tile` points; tiles are sorted by the cost class
-// of their shape type (most expensive first, longest-processing-time order) and
+// The range is cut into tiles of `tile` points, one tile per method group of the
+// shape type (method_groups); tiles are sorted by the per-point cost of their group
+// (most expensive first, longest-processing-time order) and
 // consumed by a persistent grid through one atomic work counter. A tile belongs to
 // exactly one shape, so every wavefront of a workgroup evaluates one shape type:
 // the type switch is wave-uniform, the shape record is read through scalar loads,
@@ -19,9 +20,28 @@ namespace vgmi {
 
 struct Tile {
   int32_t shape; // index into the batch's record table
-  int32_t pad_;
+  int32_t mask;  // methods this tile evaluates (kM* bits)
   int64_t begin, end;
 };
+
+// Method bits of a tile. A shape whose methods are long serial chains per point (polycone,
+// polyhedron, tessellated) has each point range cut into several tiles, one per method group, so
+// that different wavefronts run the groups side by side: the range then takes as long as its
+// slowest group instead of the sum of all six.
+constexpr int kMContains = 1, kMInside = 2, kMDistIn = 4, kMDistOut = 8, kMSafeIn = 16, kMSafeOut = 32;
+constexpr int kMAll      = 63;
+constexpr int kMPoint    = kMContains | kMInside | kMSafeIn | kMSafeOut;
+
+VGMI_DEV BatchOut mask_outputs(BatchOut o, int m)
+{
+  if (!(m & kMContains)) o.contains = nullptr;
+  if (!(m & kMInside)) o.inside = nullptr;
+  if (!(m & kMDistIn)) o.din = nullptr;
+  if (!(m & kMDistOut)) o.dout = nullptr;
+  if (!(m & kMSafeIn)) o.sin = nullptr;
+  if (!(m & kMSafeOut)) o.sout = nullptr;
+  return o;
+}
 
 template <class Data>
 VGMI_DEV void run_tile(Data const &s, int64_t b, int64_t e, BatchOut const &o, int nthreads)
@@ -65,8 +85,11 @@ VGMI_DEV void tess_tile(TessellatedData const &s, int si, Tile const &tile, Batc
 
 __global__ void k_batch_reset(uint32_t *counter) { *counter = 0u; }
 
+// At least two wavefronts per SIMD: the register budget then counts accumulation registers too
+// (without it the compiler may take 256 vector registers plus one accumulation register, which
+// leaves room for a single wavefront).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__ recs, const Tile *__restrict__ tiles,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_batch(const ShapeRecord *__restrict__ recs, const Tile *__restrict__ tiles,
                                                  int ntiles, uint32_t *counter, BatchOut o, int ldsDoubles)
 {
   extern __shared__ __align__(16) double lds[];
@@ -84,15 +107,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__
     if (t >= ntiles) break; // every wave of every block reaches this exit
     const Tile tile        = tiles[t];
     const int si           = __builtin_amdgcn_readfirstlane(tile.shape);
+    const BatchOut mo      = mask_outputs(o, __builtin_amdgcn_readfirstlane(tile.mask));
+    if (!mo.contains && !mo.inside && !mo.din && !mo.dout && !mo.sin && !mo.sout) continue;
     ShapeRecord const &rec = recs[si];
     if (rec.type == kTessellated) {
-      tess_tile<BLOCK>(rec.u.tess, si, tile, o, lds, ldsDoubles, &s_staged, &s_queue);
+      tess_tile<BLOCK>(rec.u.tess, si, tile, mo, lds, ldsDoubles, &s_staged, &s_queue);
       continue;
     }
     switch (rec.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    run_tile<T>(rec.u.M, tile.begin, tile.end, o, BLOCK);                                                           \
+    run_tile<T>(rec.u.M, tile.begin, tile.end, mo, BLOCK);                                                          \
     break;
       VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)
 #undef X
@@ -102,25 +127,45 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const ShapeRecord *__restrict__
   }
 }
 
-// Relative cost class of one query per shape type (sort key for the tile queue).
-static int cost_class(int type)
+// Method groups of a shape type and the relative cost of one point of each (sort key of the tile
+// queue: the most expensive groups start first, longest-processing-time order). The costs are the
+// per-point times of the groups measured on one MI355X, in units of a box point.
+struct MethodGroup {
+  int mask, cost;
+};
+
+static int method_groups(int type, MethodGroup g[3])
 {
   switch (type) {
   case kBox:
   case kOrb:
+    g[0] = {kMAll, 1};
     return 1;
   case kTrapezoid:
   case kTube:
-    return 2;
+    g[0] = {kMAll, 2};
+    return 1;
   case kCone:
   case kHype:
-    return 4;
+  case kSphere:
+    g[0] = {kMAll, 4};
+    return 1;
+  case kTorus2:
+    g[0] = {kMAll, 8};
+    return 1;
   case kPolycone:
   case kPolyhedron:
-  case kSphere:
-    return 8;
+    g[0] = {kMDistIn, 12};
+    g[1] = {kMDistOut, 10};
+    g[2] = {kMPoint, 10};
+    return 3;
+  case kTessellated:
+    g[0] = {kMPoint, 64};
+    g[1] = {kMDistIn | kMDistOut, 24};
+    return 2;
   default:
-    return 16;
+    g[0] = {kMAll, 16};
+    return 1;
   }
 }
 
@@ -157,12 +202,22 @@ static int default_tile(vgmi_batch const *b)
 static vgmi_status batch_build_tiles(vgmi_batch *b)
 {
   std::vector<Tile> tiles;
-  for (size_t s = 0; s < b->recs.size(); ++s)
-    for (int64_t beg = b->offsets[s]; beg < b->offsets[s] + b->counts[s]; beg += b->tile)
-      tiles.push_back(Tile{int32_t(s), 0, beg, std::min(beg + b->tile, b->offsets[s] + b->counts[s])});
-  std::stable_sort(tiles.begin(), tiles.end(), [&](Tile const &x, Tile const &y) {
-    return cost_class(b->recs[x.shape].type) > cost_class(b->recs[y.shape].type);
-  });
+  std::vector<int> cost;
+  for (size_t s = 0; s < b->recs.size(); ++s) {
+    MethodGroup g[3];
+    const int ng = method_groups(b->recs[s].type, g);
+    for (int k = 0; k < ng; ++k)
+      for (int64_t beg = b->offsets[s]; beg < b->offsets[s] + b->counts[s]; beg += b->tile) {
+        tiles.push_back(Tile{int32_t(s), g[k].mask, beg, std::min(beg + b->tile, b->offsets[s] + b->counts[s])});
+        cost.push_back(g[k].cost);
+      }
+  }
+  std::vector<int> order(tiles.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = int(i);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  std::vector<Tile> sorted(tiles.size());
+  for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
+  tiles.swap(sorted);
   if (b->d_tiles) (void)hipFree(b->d_tiles);
   b->d_tiles = nullptr;
   b->ntiles  = int(tiles.size());

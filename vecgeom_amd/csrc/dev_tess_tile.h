@@ -45,7 +45,7 @@ constexpr int kTessLdsMax = kLdsPerCU / 2 - kStaticLds;
 
 // The tessellated solid shares one test-ray walk, one safety walk and one walk along d across its
 // six methods; the walks read nodes, planes and candidate boxes through `w` (the pool or the LDS
-// image). With all six outputs, a lane whose point is deep (tess::deep_sq) queues its safety search
+// image). With Contains, Inside and both safeties, a lane whose point is deep (tess::deep_sq) queues its safety search
 // and the workgroup's wavefronts run the queued searches together afterwards, each over all facets:
 // a deep point's walk would reach most of the tree and hold its wavefront far longer than the
 // others. Called by every thread of the workgroup; q->n is zero on entry.
@@ -53,18 +53,21 @@ template <class W>
 VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int64_t e, BatchOut const &o,
                             int nthreads, TessQueue *q)
 {
-  const bool all = o.contains && o.inside && o.sin && o.sout && o.din && o.dout;
-  if (all) {
-    const double deep = tess::deep_sq(s);
+  const bool fused = o.contains && o.inside && o.sin && o.sout;
+  if (fused) {
+    const bool withDist = o.din && o.dout;
+    const double deep   = tess::deep_sq(s);
     for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
       const Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
                            __builtin_nontemporal_load(o.pz + i));
-      const Vec3<double> d(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
-                           __builtin_nontemporal_load(o.dz + i));
+      Vec3<double> d(0., 0., 1.);
+      if (withDist)
+        d = Vec3<double>(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
+                         __builtin_nontemporal_load(o.dz + i));
       bool c, toIn;
       int in;
       double si, so, di, dou, seed, lim;
-      if (tess::query_all_or_defer(s, w, p, d, c, in, si, so, di, dou, deep, seed, lim, toIn)) {
+      if (tess::query_all_or_defer(s, w, p, d, c, in, si, so, di, dou, deep, seed, lim, toIn, withDist)) {
         const int k = atomicAdd(&q->n, 1);
         if (k < kTessQueue) {
           q->slot[k] = int(i - b) | (toIn ? 1 << 30 : 0);
@@ -81,8 +84,10 @@ VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int
       }
       __builtin_nontemporal_store(uint8_t(c), o.contains + i);
       __builtin_nontemporal_store(int32_t(in), o.inside + i);
-      __builtin_nontemporal_store(di, o.din + i);
-      __builtin_nontemporal_store(dou, o.dout + i);
+      if (withDist) {
+        __builtin_nontemporal_store(di, o.din + i);
+        __builtin_nontemporal_store(dou, o.dout + i);
+      }
     }
     __syncthreads();
     const int nq = vmin(q->n, kTessQueue);
@@ -99,16 +104,18 @@ VGMI_DEV void run_tile_tess(TessellatedData const &s, W const &w, int64_t b, int
         o.sout[i] = toIn ? -1. : vsqrt(sq);
       }
     }
-    return;
+    if (withDist || (!o.din && !o.dout)) return;
   }
-  // a partial output set: the methods one by one
+  // the methods one by one (after the fused walk: the one distance it left out)
   for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
     const Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
                          __builtin_nontemporal_load(o.pz + i));
-    if (o.contains) __builtin_nontemporal_store(uint8_t(tess::contains64(s, w, p)), o.contains + i);
-    if (o.inside) __builtin_nontemporal_store(int32_t(tess::inside64(s, w, p)), o.inside + i);
-    if (o.sin) __builtin_nontemporal_store(tess::safety_in64(s, w, p), o.sin + i);
-    if (o.sout) __builtin_nontemporal_store(tess::safety_out64(s, w, p), o.sout + i);
+    if (!fused) {
+      if (o.contains) __builtin_nontemporal_store(uint8_t(tess::contains64(s, w, p)), o.contains + i);
+      if (o.inside) __builtin_nontemporal_store(int32_t(tess::inside64(s, w, p)), o.inside + i);
+      if (o.sin) __builtin_nontemporal_store(tess::safety_in64(s, w, p), o.sin + i);
+      if (o.sout) __builtin_nontemporal_store(tess::safety_out64(s, w, p), o.sout + i);
+    }
     if (o.din && o.dout) {
       const Vec3<double> d(o.dx[i], o.dy[i], o.dz[i]);
       double din, dout;

@@ -155,19 +155,29 @@ VGMI_DEV RayResult facet_hits(TessellatedData const &s, W const &w, V3d const &p
   // a node entered beyond the step limit holds no candidate box (every facet box inside it is
   // entered later; the margin covers the float rounding of the candidate test)
   const double stepBound = maxstep < FLT_MAX ? double(maxstep) * (1. + 1e-5) + 1e-5 : kInfLength;
+  // The walk alternates two phases: every lane first moves over skipped subtrees and inner nodes
+  // until it stands on a leaf it must open (or has finished), then the lanes holding a leaf test
+  // its facets together; a lane reaching a leaf early waits for the others instead of making the
+  // whole wavefront run the facet tests once per lane.
   int j = 0;
   while (j < s.nnodes) {
-    const NodeRec nd   = w.node(j);
     const double bound =
         vmin(stepBound, mode == kHitsIn ? r.dIn : (mode == kHitsOut ? r.dOut : vmax(r.dIn, r.dOut)));
-    if (!node_reached(nd.lo, nd.hi, p, inv, bound)) {
-      j = nd.escape;
-      continue;
+    NodeRec nd  = w.node(j);
+    bool atLeaf = false;
+    for (;;) {
+      if (!node_reached(nd.lo, nd.hi, p, inv, bound))
+        j = nd.escape;
+      else if (nd.count == 0)
+        ++j;
+      else {
+        atLeaf = true;
+        break;
+      }
+      if (j >= s.nnodes) break;
+      nd = w.node(j);
     }
-    if (nd.count == 0) {
-      ++j;
-      continue;
-    }
+    if (!atLeaf) break;
     for (int k = nd.first; k < nd.first + nd.count; ++k) {
       // The candidate box, side and triangle tests are pure predicates, taken here plane first.
       // A hit farther than the current one of its kind cannot replace it (equal distances
@@ -377,16 +387,24 @@ VGMI_DEV double safety_sq_threaded(TessellatedData const &s, W const &w, V3d con
 {
   double best = kInfLength;
   int j       = 0;
+  // two phases per round, as in facet_hits: advance to the next leaf to open, then open it
   while (j < s.nnodes) {
-    const NodeRec nd = w.node(j);
-    if (node_dist2(nd.lo, nd.hi, p) > vmin(best, bound)) {
-      j = nd.escape;
-      continue;
+    const double lim = vmin(best, bound);
+    NodeRec nd       = w.node(j);
+    bool atLeaf      = false;
+    for (;;) {
+      if (node_dist2(nd.lo, nd.hi, p) > lim)
+        j = nd.escape;
+      else if (nd.count == 0)
+        ++j;
+      else {
+        atLeaf = true;
+        break;
+      }
+      if (j >= s.nnodes) break;
+      nd = w.node(j);
     }
-    if (nd.count == 0) {
-      ++j;
-      continue;
-    }
+    if (!atLeaf) break;
     leaf_safety_sq(s, w, nd, p, toIn, bound, best);
     j = nd.escape;
   }
@@ -436,11 +454,12 @@ VGMI_DEV double safety_sq_wave(TessellatedData const &s, W const &w, V3d const &
 
 // Squared seeded minimum beyond which a point counts as deep inside or far from the solid: the
 // walk's boxes bound facet distances poorly there, so such a point's search is left to a wavefront
-// (a quarter of the solid's smallest extent).
+// (two fifths of the solid's smallest extent; lower thresholds send points to the wavefront
+// search that the per-lane walk finishes sooner).
 VGMI_DEV double deep_sq(TessellatedData const &s)
 {
   const double e =
-      0.25 * vmin(s.maxExt[0] - s.minExt[0], vmin(s.maxExt[1] - s.minExt[1], s.maxExt[2] - s.minExt[2]));
+      0.4 * vmin(s.maxExt[0] - s.minExt[0], vmin(s.maxExt[1] - s.minExt[1], s.maxExt[2] - s.minExt[2]));
   return e * e;
 }
 
@@ -464,11 +483,12 @@ VGMI_DEV double safety_bound(TessellatedData const &s, V3d const &p, double tHit
 
 // query_all, except that a point whose seeded safety minimum exceeds `deep` returns true with
 // sin and sout unset and its search left to the caller: the result is then vmin(seed, the smallest
-// per-facet squared safety below lim) on the side toIn (safety_sq_wave computes it).
+// per-facet squared safety below lim) on the side toIn (safety_sq_wave computes it). Without
+// withDist the walk along d is left out (din and dout unset).
 template <class W>
 VGMI_DEV bool query_all_or_defer(TessellatedData const &s, W const &w, V3d const &p, V3d const &d, bool &contains,
                                  int &inside, double &sin, double &sout, double &din, double &dout, double deep,
-                                 double &seed, double &lim, bool &toIn)
+                                 double &seed, double &lim, bool &toIn, bool withDist = true)
 {
   const RayResult r = test_ray(s, w, p);
   const V3d t       = at3(s.testDir);
@@ -491,7 +511,7 @@ VGMI_DEV bool query_all_or_defer(TessellatedData const &s, W const &w, V3d const
       contains ? safety_bound(s, p, r.dOut) : safety_bound(s, p, r.sIn >= 0 ? r.dIn : kInfLength);
   seed = safety_sq_seed(s, w, p, toIn, bound);
   lim  = vmin(seed, bound);
-  dist_both64(s, w, p, d, din, dout);
+  if (withDist) dist_both64(s, w, p, d, din, dout);
   if (lim > deep) return true;
   const double sq = vmin(seed, safety_sq_threaded(s, w, p, toIn, lim));
   sin             = contains ? -1. : vsqrt(sq);

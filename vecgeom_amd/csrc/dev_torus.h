@@ -204,9 +204,11 @@ VGMI_DEV int Inside(TorusData const &t, Vec3<R> const &p)
   return r;
 }
 
-// TorusImplementation2.h:367-493
+// TorusImplementation2.h:367-493. Out of line: the quartic solve with the math library's cbrt,
+// pow, cos and atan2 needs far more registers than any other shape kernel, and inlined at its
+// four call sites it would set the register budget of every kernel that also holds the torus.
 template <class R>
-VGMI_DEV R ToBoundary(TorusData const &t, Vec3<R> const &pt, Vec3<R> const &dir, double radiusD, bool out)
+__device__ __noinline__ R ToBoundary(TorusData const &t, Vec3<R> const &pt, Vec3<R> const &dir, double radiusD, bool out)
 {
   const R radius = R(radiusD);
   const R rtor2  = R(t.rtor2);
