@@ -13,12 +13,22 @@ is one replay of the captured HIP graph evaluating all six methods
 Also reported in the same JSON line:
   box_tube: Box (10,20,30) and full-phi Tube (5,10,15), 2^20 points each, six methods;
   nav:      calorimeter (23k touchables), 2^20 tracks x 50 ComputeStepAndPropagatedState
-            steps (10 mm limit), fused transport loop; nav-steps/s.
+            steps (10 mm limit), fused transport loop; nav-steps/s; LevelLocate and
+            LocateGlobalPoint of the same 2^20 world points (points/s).
   serve:    the continuous-batching server on the calorimeter: 16 producer threads with
             Poisson arrivals (rates swept), 2^U(10,17.6) tracks per request, U{1..100}
             steps per track; p50/p99 latency and nav-steps/s per rate.
 Each throughput figure carries a roofline entry (HIP events on the launch stream) and a
 CPU baseline (the oracle restatement of the scalar backend, 1 thread, bounded sample).
+Algorithmic bytes: 56 B per directed query, 32 B per safety, 28 B per Inside, 25 B per
+Contains, 85 B per point of the six-method batch, 88 B per nav-step (p, d, limit and state
+in, step and state out), 56 B per LevelLocate (point and volume in; daughter and its local
+point out), 28 B per LocateGlobalPoint. The VALU-bound kernels (batched mix, transport loop)
+also carry a `valu` entry from the committed rocprofv3 counters of the same launch shape:
+issued VALU cycles over the SIMDs' cycles, and FP64 throughput over the vector peak.
+
+The batched launch runs at a fixed launch shape (--tuning, default the autotuner's choice on
+one MI355X, the shape the committed profiles were taken with); --autotune picks it afresh.
 """
 import argparse
 import json
@@ -30,6 +40,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK = 78.6e12  # FLOP/s, MI355X
+NUM_SIMDS = 256 * 4
+VALU_CYCLES_PER_INST = 4     # a wave64 instruction occupies a 16-lane SIMD for four cycles
+DEFAULT_TUNING = "256,256,4"  # tile, block, blocks per CU of the batched launch
+NAV_BYTES_PER_STEP = 88
+LEVEL_LOCATE_BYTES = 56
+LOCATE_BYTES = 28
 METHODS = ["Contains", "Inside", "DistanceToIn", "DistanceToOut", "SafetyToIn", "SafetyToOut"]
 # algorithmic bytes per query (float64 SOA in, result out)
 BYTES = {"Contains": 25, "Inside": 28, "DistanceToIn": 56, "DistanceToOut": 56, "SafetyToIn": 32,
@@ -57,9 +74,9 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-check", action="store_true",
                     help="only run a plain 1 GiB device copy 5 times (calibrates HBM byte counters) and exit")
-    ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tuning", default=None,
-                    help="tile,block,blocks_per_cu for the batched launch (skips the autotuner)")
+    ap.add_argument("--autotune", action="store_true", help="pick the batched launch shape with the autotuner")
+    ap.add_argument("--tuning", default=DEFAULT_TUNING,
+                    help="tile,block,blocks_per_cu for the batched launch (default %(default)s)")
     return ap.parse_args()
 
 
@@ -93,10 +110,32 @@ def measured_traffic(kernel, prefix=None):
         name = kernel if kernel in kernels or prefix is None else next(
             (n for n in kernels if n.startswith(prefix)), kernel)
         k = kernels.get(name)
-        return None if k is None else {"bytes": k["traffic_bytes"], "workload": k["workload"], "kernel": name,
-                                       "source": "profiles/pmc_hbm_traffic.md"}
+        if k is None:
+            return None
+        out = {"bytes": k["traffic_bytes"], "workload": k["workload"], "kernel": name,
+               "source": "profiles/pmc_hbm_traffic.md"}
+        if "nav_steps" in k:
+            out["nav_steps"] = k["nav_steps"]
+        return out
     except (OSError, ValueError, KeyError):
         return None
+
+
+def valu_entry(kernel):
+    """VALU issue fraction and FP64 throughput of `kernel` from the committed rocprofv3 counters
+    (profiles/counters.json: SQ_INSTS_VALU, SQ_INSTS_VALU_FLOPS_FP64, GRBM_GUI_ACTIVE summed over the
+    8 XCDs, and the average dispatch time, per dispatch), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "counters.json")) as f:
+            c = json.load(f)["kernels"][kernel]
+    except (OSError, ValueError, KeyError):
+        return None
+    cycles = c["GRBM_GUI_ACTIVE"] / 8.0
+    # SQ_INSTS_VALU_FLOPS_FP64 counts per wavefront instruction: x64 lanes is an upper bound
+    flops = 64 * c["SQ_INSTS_VALU_FLOPS_FP64"] / (c["dispatch_ns"] * 1e-9)
+    return {"issue_frac": round(c["SQ_INSTS_VALU"] * VALU_CYCLES_PER_INST / (NUM_SIMDS * cycles), 4),
+            "fp64_tflops_upper": round(flops / 1e12, 2), "fp64_frac_upper": round(flops / FP64_VECTOR_PEAK, 4),
+            "kernel": kernel, "workload": c["workload"], "source": c["source"]}
 
 
 def event_time_us(fn, reps):
@@ -137,13 +176,13 @@ def run_mix(args):
     P, D = SOA3D.from_array(p), SOA3D.from_array(d)
     batch = ShapeBatch(gpu, counts)
     outs = batch.alloc_outputs()
-    tuning = None
-    if args.tuning:
+    if args.autotune:
+        tuning, _ = autotune(batch, P, D, outs)
+        tuning = dict(tuning, source="autotuner")
+    else:
         tile, block, bpc = (int(x) for x in args.tuning.split(","))
         batch.set_tuning(tile, block, bpc)
         tuning = {"tile": tile, "block": block, "blocks_per_cu": bpc, "source": "--tuning"}
-    elif not args.no_tune:
-        tuning, _ = autotune(batch, P, D, outs)
     batch.capture(P, D, outs)
     for _ in range(args.warmup):
         batch.replay()
@@ -159,11 +198,12 @@ def run_mix(args):
                       "not_yet_in_mix": [k for k in ("box", "tube", "cone", "polycone", "polyhedron", "trapezoid",
                                                       "sphere", "torus2", "orb", "tessellated")
                                          if k not in {s for s, _ in shapes}],
-                      "tuning": tuning or batch.info()},
+                      "tuning": tuning},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("vgmi::k_batch<%d>" % batch.info()["block"], "vgmi::k_batch<"),
                         "kernel": "k_batch",
-                        "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)}}
+                        "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)},
+           "valu": valu_entry("vgmi::k_batch<%d>" % batch.info()["block"])}
     if not args.no_cpu:
         from oracle.pyoracle import OracleShape
         orc = [OracleShape(k, prm) for k, prm in shapes]
@@ -174,8 +214,9 @@ def run_mix(args):
                 off = 0
                 for o, c in zip(orc, counts):
                     m = min(c, sample)
-                    ps = np.ascontiguousarray(p[:, off:off + m])
-                    dsm = np.ascontiguousarray(d[:, off:off + m])
+                    idx = off + (np.arange(m, dtype=np.int64) * c) // m  # evenly spaced over both halves
+                    ps = np.ascontiguousarray(p[:, idx])
+                    dsm = np.ascontiguousarray(d[:, idx])
                     t0 = time.perf_counter()
                     for meth in METHODS:
                         o.query(meth, ps, dsm if meth.startswith("Distance") else None, threads=threads)
@@ -188,13 +229,14 @@ def run_mix(args):
         done, used = mix_rate(sample, 1, args.cpu_seconds)
         model, ncpu = cpu_info()
         res["cpu_baseline"] = {"value": round(done / used, 1), "unit": "queries/s", "cores": 1, "kind": "port",
-                               "sample": "%d queries: first %d points of every shape of the same mix x 6 methods, "
-                                         "repeated for ~%.0f s; host %s, %d logical CPUs"
+                               "sample": "%d queries: %d points evenly spaced over every shape's range of the same "
+                                         "mix (uniform and near-surface halves alike) x 6 methods, repeated for "
+                                         "~%.0f s; host %s, %d logical CPUs"
                                          % (done, sample, args.cpu_seconds, model, ncpu)}
         cores = host_cores()
         done, used = mix_rate(1 << 15, cores, args.cpu_seconds / 2)
         res["cpu_baseline"]["all_cores"] = {"value": round(done / used, 1), "cores": cores,
-                                            "sample": "%d queries, first %d points of every shape, %d threads"
+                                            "sample": "%d queries, %d evenly spaced points of every shape, %d threads"
                                                       % (done, 1 << 15, cores)}
     return res
 
@@ -310,8 +352,8 @@ def run_nav(args):
 
     from vecgeom_amd.shapes import FP64, MIXED
     s = torch.cuda.current_stream()
-    bytes_launch = 88 * n  # per track: read p, d, state (56 B), write p, state (32 B)
     res = {"tracks": n, "nsteps": nsteps, "step_limit_mm": limit}
+    kname = {"fp64": "vgmi::nav::k_propagate<295u>", "mixed": "vgmi::nav::k_propagate<2147483943u>"}
     for name, prec in (("fp64", FP64), ("mixed", MIXED)):
         for _ in range(max(1, args.warmup)):
             reset()
@@ -329,22 +371,44 @@ def run_nav(args):
             torch.cuda.synchronize()
             total_us += a.elapsed_time(b) * 1e3
         avg_us = total_us / reps
-        gbs = bytes_launch / (avg_us * 1e-6) / 1e9
+        bytes_run = NAV_BYTES_PER_STEP * steps_per_run
+        gbs = bytes_run / (avg_us * 1e-6) / 1e9
+        traffic = measured_traffic(kname[name])
+        if traffic is not None and traffic.get("nav_steps"):
+            # HBM bytes per nav-step of the profiled dispatches over the 88 algorithmic ones
+            traffic["over_algorithmic"] = round(traffic["bytes"] / (NAV_BYTES_PER_STEP * traffic["nav_steps"]), 2)
         res[name] = {"nav_steps_per_s": round(steps_per_run / (avg_us * 1e-6), 1), "avg_launch_us": round(avg_us, 2),
                      "steps_per_launch": steps_per_run,
                      "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(gbs / HBM_PEAK_GBS, 5),
-                                  "traffic": measured_traffic("vgmi::nav::k_propagate<%s>" % (
-                                      "295u" if name == "fp64" else "2147483943u")),
-                                  "kernel": "k_propagate",
-                                  "bytes_per_launch": bytes_launch,
-                                  "note": "latency/divergence bound: track state stays in registers for all steps"}}
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                                  "kernel": "k_propagate (the transport loop: regrouping sorts and %d-step "
+                                            "dispatches)" % 10,
+                                  "bytes_per_launch": bytes_run,
+                                  "note": "88 B per nav-step; the fused loop keeps the track state in registers "
+                                          "between steps, so its HBM traffic is geometry and spills"},
+                     "valu": valu_entry(kname[name])}
         if name == "mixed":
             res[name]["final_state_equal_fp64"] = bool(torch.equal(nav, nav_fp64) and torch.equal(P.x, px_fp64))
         else:
             nav_fp64, px_fp64 = nav.clone(), P.x.clone()
     res["nav_steps_per_s"] = res["fp64"]["nav_steps_per_s"]
     res["mixed_over_fp64"] = round(res["mixed"]["nav_steps_per_s"] / res["fp64"]["nav_steps_per_s"], 3)
+    # one LevelLocate per track in the world volume (the world's placement is the identity, so the
+    # global points are the world's local points), and the full descent of LocateGlobalPoint
+    world_lv = torch.full((n,), g.world, dtype=torch.int32, device="cuda")
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        for label, fn, nbytes in (("level_locate", lambda: G.LevelLocate(world_lv, P0, precision=prec),
+                                   LEVEL_LOCATE_BYTES),
+                                  ("locate_global_point", lambda: G.LocateGlobalPoint(P0, out=nav, precision=prec),
+                                   LOCATE_BYTES)):
+            for _ in range(max(1, args.warmup)):
+                fn()
+            us = event_time_us(fn, max(5, args.steps // 2))
+            gbs = nbytes * n / (us * 1e-6) / 1e9
+            res[label + "_" + name] = {"points_per_s": round(n / (us * 1e-6), 1), "avg_launch_us": round(us, 2),
+                                       "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+                                                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                                                    "bytes_per_launch": nbytes * n}}
     if not args.no_cpu:
         from oracle.pyoracle import OracleGeometry
         O = OracleGeometry(g)
@@ -407,6 +471,12 @@ def run_serve(args):
                                                   seed=int(rate))
                 sm = summarize(rec, refused, wall)
                 sm["rate_hz"] = rate
+                if sm.get("nav_steps_per_s"):
+                    # at the offered load, not at saturation: the rate is set by the arrivals
+                    gbs = NAV_BYTES_PER_STEP * sm["nav_steps_per_s"] / 1e9
+                    sm["roofline"] = {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": round(gbs / HBM_PEAK_GBS, 5), "basis": "88 B per nav-step at the "
+                                      "offered load"}
                 sweep.append(sm)
             res["sweep"] = sweep
         out[name] = res
