@@ -49,7 +49,9 @@ enum vgmi_shape_type {
   VGMI_TORUS2 = 7,      /* rmin rmax rtor sphi dphi                volumes/UnplacedTorus2.h */
   VGMI_ORB = 8,         /* r                                       volumes/UnplacedOrb.h */
   VGMI_TESSELLATED = 9, /* ntri then 9 doubles per triangle        volumes/UnplacedTessellated.h */
-  VGMI_HYPE = 10        /* rmin rmax stIn stOut dz                 volumes/UnplacedHype.h */
+  VGMI_HYPE = 10,       /* rmin rmax stIn stOut dz                 volumes/UnplacedHype.h */
+  VGMI_TRD = 11,        /* x1 x2 y1 y2 z (or x1 x2 y z, or x y z)  volumes/UnplacedTrd.h:51-80 */
+  VGMI_PARABOLOID = 12  /* rlo rhi dz (rhi > rlo, as G4Paraboloid)  volumes/UnplacedParaboloid.h:60 */
 };
 
 /* Query methods of VUnplacedVolume / VPlacedVolume (volumes/UnplacedVolume.h:88-223). */

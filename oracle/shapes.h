@@ -188,6 +188,50 @@ public:
   bool Hyperbolic(V3 const &p, V3 const &d, bool toIn, bool innerSurf, double &dist) const;
 };
 
+// ---------------------------------------------------------------- Trd (UniversalTrd)
+// VecGeom/volumes/TrdStruct.h:18-121 (cached members), VecGeom/volumes/kernel/TrdImplementation.h:25-518
+// with TrdTypes::UniversalTrd (VecGeom/volumes/kernel/shapetypes/TrdTypes.h:46-76): the y faces are
+// taken as varying whenever dy1 != dy2, and the Inside/Contains y test always uses the cross product.
+class Trd : public Shape {
+public:
+  double dx1, dx2, dy1, dy2, dz;
+  double x2minusx1, y2minusy1, halfx1plusx2, halfy1plusy2, fx, fy, secxz, secyz, calfx, calfy, tolx, toly;
+  Trd(double x1, double x2, double y1, double y2, double z);
+  bool varyingY() const { return dy1 != dy2; }
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override
+  {
+    const double mx = dx1 > dx2 ? dx1 : dx2, my = dy1 > dy2 ? dy1 : dy2;
+    lo = V3(-mx, -my, -dz), hi = V3(mx, my, dz);
+  }
+  void InsideFlags(V3 const &p, bool surface, bool &cin, bool &cout) const;
+  bool Face(bool forY, bool mirrored, bool toIn, V3 const &p, V3 const &d, double &dist) const;
+  double Safety(V3 const &p, bool inside) const;
+};
+
+// ---------------------------------------------------------------- Paraboloid
+// VecGeom/volumes/ParaboloidStruct.h:17-100 (cached members), VecGeom/volumes/kernel/ParaboloidImplementation.h:70-420
+class Paraboloid : public Shape {
+public:
+  double rlo, rhi, dz;
+  double dx, dy, a, invA, a2, b, b2, invB, k1, k2, rlo2, rhi2;
+  Paraboloid(double rlo, double rhi, double dz);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  void Extent(V3 &lo, V3 &hi) const override { lo = V3(-dx, -dy, -dz), hi = V3(dx, dy, dz); }
+  void Flags(V3 const &p, bool forInside, bool &cin, bool &cout) const;
+  bool OnZPlane(bool top, V3 const &p) const;
+};
+
 // ---------------------------------------------------------------- Cone (UniversalCone)
 // VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
 // ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820

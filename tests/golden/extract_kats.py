@@ -705,6 +705,8 @@ SHAPES = {
     "torus2": ("TestTorus2.cpp", r"Torus_t", (5,), r"UnplacedTorus2"),
     "tessellated": ("TestTessellated.cpp", r"SimpleTessellated", "tessellated", None),
     "hype": ("TestHype.cpp", r"Hype_t", (5,), r"S?UnplacedHype\w*"),
+    "trd": ("TestTrd.cpp", r"Trd_t", (3, 4, 5), r"UnplacedTrd"),
+    "paraboloid": ("TestParaboloid.cpp", r"Paraboloid_t", (3,), r"UnplacedParaboloid"),
 }
 
 

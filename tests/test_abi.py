@@ -52,7 +52,9 @@ def test_extent_matches_oracle():
              ("sphere", [0.0, 10.0, math.radians(20), math.radians(270), math.radians(30), math.radians(100)]),
              ("sphere", [45.0, 50.0, -math.pi / 4, 0.5 * math.pi, math.pi / 4, 0.5 * math.pi]),
              ("sphere", [0.0, 50.0, 0.0, 2 * math.pi, 0.75 * math.pi, 0.25 * math.pi]),
-             ("sphere", [6.0, 8.0, math.pi / 6, math.pi / 3, 0.0, math.pi])]
+             ("sphere", [6.0, 8.0, math.pi / 6, math.pi / 3, 0.0, math.pi]),
+             ("trd", [10.0, 30.0, 20.0, 40.0, 40.0]), ("trd", [30.0, 10.0, 25.0, 40.0]), ("trd", [5.0, 6.0, 7.0]),
+             ("paraboloid", [6.0, 9.0, 10.0]), ("paraboloid", [0.0, 8.0, 10.0])]
     # shapes with a variable-length block need a device for creation: extent from the host record
     cases = [c for c in cases if c[0] != "polyhedron"]
     for kind, params in cases:

@@ -65,6 +65,13 @@ CASES = [
     ("hype", [10.0, 20.0, math.pi / 4, math.pi / 4, 50.0]),
     ("hype", [0.0, 20.0, 0.0, math.pi / 3, 50.0]),
     ("hype", [10.0, 20.0, 0.0, math.pi / 4, 50.0]),
+    ("trd", [10.0, 30.0, 20.0, 40.0, 40.0]),
+    ("trd", [5.0, 12.0, 9.0, 9.0, 15.0]),
+    ("trd", [20.0, 20.0, 30.0, 30.0, 40.0]),
+    ("trd", [0.15, 0.15, 24.707, 24.707, 22.7]),
+    ("paraboloid", [6.0, 9.0, 10.0]),
+    ("paraboloid", [0.0, 8.0, 10.0]),
+    ("paraboloid", [1.0, 12.0, 8.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing then stops at a neighbouring
@@ -111,7 +118,7 @@ def test_query_all_matches_single_methods(case):
 
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
-                                   "torus2", "tessellated", "hype"])
+                                   "torus2", "tessellated", "hype", "trd", "paraboloid"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

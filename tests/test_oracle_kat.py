@@ -9,7 +9,7 @@ from oracle.pyoracle import OracleShape
 from tests import kat_util
 
 SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated",
-          "hype"]
+          "hype", "trd", "paraboloid"]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

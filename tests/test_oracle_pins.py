@@ -205,6 +205,27 @@ def margin_convex(corners):
     return f
 
 
+def trd_corners(params):
+    """The eight corners of a Trd (dx1 dx2 dy1 dy2 dz): half-widths dx1, dy1 at -dz and dx2, dy2 at +dz."""
+    dx1, dx2, dy1, dy2, dz = params
+    return np.array([(xs * dx, ys * dy, zs * dz) for zs, dx, dy in ((-1, dx1, dy1), (1, dx2, dy2))
+                     for ys in (-1, 1) for xs in (-1, 1)])
+
+
+def margin_paraboloid(params, p):
+    """|z| <= dz and rho <= r(z) = sqrt(k1 z + k2), the circle radii rlo at -dz and rhi at +dz on a
+    paraboloid of revolution. h = r(clamp(z)) - rho is L-Lipschitz with L = sqrt(1 + r'^2) and
+    r' = k1 / (2 r) largest where r is smallest on [-dz, dz], so h / L is 1-Lipschitz; both terms
+    vanish on the surface."""
+    rlo, rhi, dz = params
+    k1 = (rhi * rhi - rlo * rlo) / (2 * dz)
+    k2 = (rhi * rhi + rlo * rlo) / 2
+    lip = math.sqrt(1 + (k1 / (2 * min(rlo, rhi))) ** 2)
+    zc = np.clip(p[2], -dz, dz)
+    h = np.sqrt(k1 * zc + k2) - _rho(p)
+    return np.minimum(dz - np.abs(p[2]), h / lip)
+
+
 # ------------------------------------------------------------------------------- shapes under test
 TWO_PI = 2 * math.pi
 CASES = [
@@ -252,6 +273,12 @@ def _hull_case():
 
 
 CASES.append(_hull_case())
+for _t in ([10.0, 30.0, 20.0, 40.0, 40.0], [5.0, 12.0, 9.0, 9.0, 15.0], [20.0, 20.0, 30.0, 30.0, 40.0]):
+    CASES.append(("trd", _t, margin_convex(trd_corners(_t))))
+CASES.append(("paraboloid", [6.0, 9.0, 10.0], margin_paraboloid))
+# the reference's root selection takes the quadratic's leading coefficient as positive, i.e. rhi > rlo
+# (the G4Paraboloid convention R2 > R1); rlo > rhi is not pinned
+CASES.append(("paraboloid", [1.0, 12.0, 8.0], margin_paraboloid))
 IDS = ["%s-%d" % (c[0], i) for i, c in enumerate(CASES)]
 
 

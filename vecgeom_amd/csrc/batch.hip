@@ -143,6 +143,8 @@ static int method_groups(int type, MethodGroup g[3])
     return 1;
   case kTrapezoid:
   case kTube:
+  case kTrd:
+  case kParaboloid:
     g[0] = {kMAll, 2};
     return 1;
   case kCone:

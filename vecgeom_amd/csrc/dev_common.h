@@ -25,6 +25,7 @@ constexpr double kConeTolerance = 1e-7;
 constexpr double kMixedBand      = 1e-3;
 constexpr double kMixedPlaneBand = 1e-3;
 constexpr double kInfLength     = DBL_MAX;
+constexpr double kTiny          = 1e-30; // base/Global.h kTiny: keeps a denominator off zero
 constexpr double kPi            = 3.14159265358979323846;
 constexpr double kTwoPi         = 2. * kPi;
 

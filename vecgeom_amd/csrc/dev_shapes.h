@@ -5,12 +5,14 @@
 #include "dev_cone.h"
 #include "dev_hype.h"
 #include "dev_orb.h"
+#include "dev_paraboloid.h"
 #include "dev_polycone.h"
 #include "dev_polyhedron.h"
 #include "dev_sphere.h"
 #include "dev_tessellated.h"
 #include "dev_torus.h"
 #include "dev_trapezoid.h"
+#include "dev_trd.h"
 #include "dev_tube.h"
 
 // (enum value, record type, union member, kernel namespace)
@@ -28,7 +30,9 @@
   X(kSphere, SphereData, sph, sph)                                                                                   \
   X(kPolyhedron, PolyhedronData, phed, phed)                                                                         \
   X(kTorus2, TorusData, tor, tor)                                                                                    \
-  X(kHype, HypeData, hype, hype)
+  X(kHype, HypeData, hype, hype)                                                                                     \
+  X(kTrd, TrdData, trd, trd)                                                                                         \
+  X(kParaboloid, ParaboloidData, para, para)
 
 namespace vgmi {
 

@@ -22,7 +22,9 @@ enum ShapeType : int32_t {
   kOrb = 8,
   kTessellated = 9,
   kHype = 10,
-  kNumShapeTypes = 11
+  kTrd = 11,
+  kParaboloid = 12,
+  kNumShapeTypes = 13
 };
 
 enum QueryOp : int32_t {
@@ -145,6 +147,22 @@ struct HypeData {
   int32_t pad_;
 };
 
+// Trd (VecGeom/volumes/TrdStruct.h:18-121): half-lengths and the cached differences, half sums,
+// slopes, cosines and lateral tolerances of CalculateCached (the y tolerance from the x slope, as
+// there). varyingY: dy1 != dy2 (TrdTypes::checkVaryingY for UniversalTrd).
+struct TrdData {
+  double dx1, dx2, dy1, dy2, dz;
+  double x2minusx1, y2minusy1, halfx1plusx2, halfy1plusy2, fx, fy, calfx, calfy, tolx, toly;
+  int32_t varyingY;
+  int32_t pad_;
+};
+
+// Paraboloid (VecGeom/volumes/ParaboloidStruct.h:17-70): rho^2 = k1 z + k2 between z = -dz (radius rlo)
+// and z = +dz (radius rhi), i.e. z = a rho^2 + b; cached coefficients and squares.
+struct ParaboloidData {
+  double rlo, rhi, dz, dx, a, invA, b, k1, k2, rlo2, rhi2;
+};
+
 // Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
 // and a threaded BVH over them live in the variable-length pool:
 //   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
@@ -197,6 +215,8 @@ struct alignas(16) ShapeRecord {
     TorusData tor;
     TessellatedData tess;
     HypeData hype;
+    TrdData trd;
+    ParaboloidData para;
     double raw[64];
   } u;
 };

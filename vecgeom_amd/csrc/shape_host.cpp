@@ -277,6 +277,62 @@ int MakeHype(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
+// Trd: the UnplacedTrd constructors (VecGeom/volumes/UnplacedTrd.h:51-80) and TrdStruct::CalculateCached
+// (VecGeom/volumes/TrdStruct.h:102-121; the y tolerance is computed from the x slope, as there).
+int MakeTrd(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  double x1, x2, y1, y2, z;
+  if (n == 3)
+    x1 = x2 = a[0], y1 = y2 = a[1], z = a[2];
+  else if (n == 4)
+    x1 = a[0], x2 = a[1], y1 = y2 = a[2], z = a[3];
+  else if (n == 5)
+    x1 = a[0], x2 = a[1], y1 = a[2], y2 = a[3], z = a[4];
+  else
+    return err = "trd takes x1 x2 y1 y2 z, x1 x2 y z or x y z", -1;
+  if (!(x1 >= 0 && x2 >= 0 && y1 >= 0 && y2 >= 0 && z > 0 && (x1 > 0 || x2 > 0) && (y1 > 0 || y2 > 0)))
+    return err = "trd half-lengths must be non-negative (not both zero at the two ends) and z > 0", -1;
+  r          = ShapeRecord();
+  r.type     = kTrd;
+  TrdData &t = r.u.trd;
+  t.dx1 = x1, t.dx2 = x2, t.dy1 = y1, t.dy2 = y2, t.dz = z;
+  t.x2minusx1    = t.dx2 - t.dx1;
+  t.y2minusy1    = t.dy2 - t.dy1;
+  t.halfx1plusx2 = 0.5 * (t.dx1 + t.dx2);
+  t.halfy1plusy2 = 0.5 * (t.dy1 + t.dy2);
+  t.fx           = 0.5 * (t.dx1 - t.dx2) / t.dz;
+  t.fy           = 0.5 * (t.dy1 - t.dy2) / t.dz;
+  t.calfx        = 1. / std::sqrt(1.0 + t.fx * t.fx);
+  t.calfy        = 1. / std::sqrt(1.0 + t.fy * t.fy);
+  t.tolx         = 1e-9 * std::sqrt(t.x2minusx1 * t.x2minusx1 + 4 * t.dz * t.dz);
+  t.toly         = 1e-9 * std::sqrt(t.x2minusx1 * t.x2minusx1 + 4 * t.dz * t.dz);
+  t.varyingY     = t.dy1 != t.dy2 ? 1 : 0;
+  return 0;
+}
+
+// Paraboloid: UnplacedParaboloid(rlo, rhi, dz) (VecGeom/volumes/UnplacedParaboloid.h:60) and
+// ParaboloidStruct::CalculateCached (VecGeom/volumes/ParaboloidStruct.h:48-70).
+int MakeParaboloid(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n != 3) return err = "paraboloid takes rlo rhi dz", -1;
+  if (!(a[0] >= 0 && a[1] >= 0 && a[2] > 0 && a[0] != a[1]))
+    return err = "paraboloid needs rlo >= 0, rhi >= 0, rlo != rhi and dz > 0", -1;
+  r                 = ShapeRecord();
+  r.type            = kParaboloid;
+  ParaboloidData &q = r.u.para;
+  q.rlo = a[0], q.rhi = a[1], q.dz = a[2];
+  q.rlo2          = q.rlo * q.rlo;
+  q.rhi2          = q.rhi * q.rhi;
+  const double dd = 1. / (q.rhi2 - q.rlo2);
+  q.a             = 2. * q.dz * dd;
+  q.b             = -q.dz * (q.rlo2 + q.rhi2) * dd;
+  q.k1            = (q.rhi2 - q.rlo2) * 0.5 / q.dz;
+  q.k2            = (q.rhi2 + q.rlo2) * 0.5;
+  q.invA          = 1 / q.a;
+  q.dx            = q.rhi < q.rlo ? q.rlo : q.rhi;
+  return 0;
+}
+
 // Cone: VecGeom/volumes/ConeStruct.h:415-426 (constructor), :309-346 (angle normalisation: the
 // wedge keeps the raw angles, the phi-plane vectors use the normalised ones) and :76-140 (cache).
 int MakeCone(double const *a, int n, ShapeRecord &r, std::string &err)
@@ -852,6 +908,10 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeTessellated(params, nparams, r, extra, err);
   case kHype:
     return MakeHype(params, nparams, r, err);
+  case kTrd:
+    return MakeTrd(params, nparams, r, err);
+  case kParaboloid:
+    return MakeParaboloid(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -916,6 +976,19 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     const double e = r.u.hype.endOuterRadius;
     lo[0] = lo[1] = -e, hi[0] = hi[1] = e;
     lo[2] = -r.u.hype.dz, hi[2] = r.u.hype.dz;
+    return;
+  }
+  case kTrd: {
+    // VecGeom/volumes/UnplacedTrd.h:218-222
+    TrdData const &t = r.u.trd;
+    const double mx = t.dx1 < t.dx2 ? t.dx2 : t.dx1, my = t.dy1 < t.dy2 ? t.dy2 : t.dy1;
+    lo[0] = -mx, lo[1] = -my, lo[2] = -t.dz, hi[0] = mx, hi[1] = my, hi[2] = t.dz;
+    return;
+  }
+  case kParaboloid: {
+    // source/UnplacedParaboloid.cpp:68-76
+    ParaboloidData const &q = r.u.para;
+    lo[0] = lo[1] = -q.dx, hi[0] = hi[1] = q.dx, lo[2] = -q.dz, hi[2] = q.dz;
     return;
   }
   default:
