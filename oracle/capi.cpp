@@ -8,7 +8,8 @@
 //                 also the reduced forms of 3, 4, 5, 6 parameters and 8 corners (24 doubles, the
 //                 TrapCorners order of source/UnplacedTrapezoid.cpp:447-465)
 //  10 hype        rmin rmax stIn stOut dz                    (volumes/UnplacedHype.h:56)
-//  11 trd         x1 x2 y1 y2 z; also x1 x2 y z and x y z     (volumes/UnplacedTrd.h:51-80)
+//  11 trd         x1 x2 y1 y2 z; also x1 x2 y z and x y z     (volumes/UnplacedTrd.h:51-80); a sixth
+//                 value 1 selects the Trd1 kind (y1 == y2) of GeoManager::MakeInstance
 //  12 paraboloid  rlo rhi dz                                 (volumes/UnplacedParaboloid.h:60)
 //   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
 //                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
@@ -73,7 +74,8 @@ Shape *MakeShape(int type, const double *a, int n)
   case 11:
     if (n == 3) return new Trd(a[0], a[0], a[1], a[1], a[2]);
     if (n == 4) return new Trd(a[0], a[1], a[2], a[2], a[3]);
-    return n == 5 ? new Trd(a[0], a[1], a[2], a[3], a[4]) : nullptr;
+    if (n == 6 && a[5] == 1.0 && a[2] == a[3]) return new Trd(a[0], a[1], a[2], a[3], a[4], true);
+    return n == 5 || (n == 6 && a[5] == 0.0) ? new Trd(a[0], a[1], a[2], a[3], a[4]) : nullptr;
   case 12:
     return n >= 3 ? new Paraboloid(a[0], a[1], a[2]) : nullptr;
   case 3: {

@@ -196,7 +196,8 @@ class Trd : public Shape {
 public:
   double dx1, dx2, dy1, dy2, dz;
   double x2minusx1, y2minusy1, halfx1plusx2, halfy1plusy2, fx, fy, secxz, secyz, calfx, calfy, tolx, toly;
-  Trd(double x1, double x2, double y1, double y2, double z);
+  bool plainY = false; // TrdTypes::Trd1 (HasVaryingY kNo): Inside/Contains test |y| against dy1
+  Trd(double x1, double x2, double y1, double y2, double z, bool trd1 = false);
   bool varyingY() const { return dy1 != dy2; }
   bool Contains(V3 const &p) const override;
   int Inside(V3 const &p) const override;

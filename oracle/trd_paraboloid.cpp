@@ -8,7 +8,8 @@ namespace vgo {
 
 // ============================================================== Trd (UniversalTrd)
 // TrdStruct.h:102-121 CalculateCached. The y tolerance is computed from the x slope, as there.
-Trd::Trd(double x1, double x2, double y1, double y2, double z) : dx1(x1), dx2(x2), dy1(y1), dy2(y2), dz(z)
+Trd::Trd(double x1, double x2, double y1, double y2, double z, bool trd1)
+    : dx1(x1), dx2(x2), dy1(y1), dy2(y2), dz(z), plainY(trd1)
 {
   x2minusx1    = dx2 - dx1;
   y2minusy1    = dy2 - dy1;
@@ -37,6 +38,11 @@ void Trd::InsideFlags(V3 const &p, bool surface, bool &cin, bool &cout) const
     cin  = cin && cross > tolx;
   } else {
     cout = cout || cross < 0;
+  }
+  if (plainY) { // TrdImplementation.h:201-205 (HasVaryingY kNo)
+    cout = cout || std::fabs(p.y) > (surface ? dy1 + kHalfTolerance : dy1);
+    if (surface) cin = cin && std::fabs(p.y) < dy1 - kHalfTolerance;
+    return;
   }
   cross = y2minusy1 * pz - 2.0 * dz * (std::fabs(p.y) - dy1);
   if (surface) {

@@ -67,6 +67,7 @@ CASES = [
     ("hype", [10.0, 20.0, 0.0, math.pi / 4, 50.0]),
     ("trd", [10.0, 30.0, 20.0, 40.0, 40.0]),
     ("trd", [5.0, 12.0, 9.0, 9.0, 15.0]),
+    ("trd", [5.0, 12.0, 9.0, 9.0, 15.0, 1.0]),
     ("trd", [20.0, 20.0, 30.0, 30.0, 40.0]),
     ("trd", [0.15, 0.15, 24.707, 24.707, 22.7]),
     ("paraboloid", [6.0, 9.0, 10.0]),

@@ -273,8 +273,9 @@ def _hull_case():
 
 
 CASES.append(_hull_case())
-for _t in ([10.0, 30.0, 20.0, 40.0, 40.0], [5.0, 12.0, 9.0, 9.0, 15.0], [20.0, 20.0, 30.0, 30.0, 40.0]):
-    CASES.append(("trd", _t, margin_convex(trd_corners(_t))))
+for _t in ([10.0, 30.0, 20.0, 40.0, 40.0], [5.0, 12.0, 9.0, 9.0, 15.0], [20.0, 20.0, 30.0, 30.0, 40.0],
+           [5.0, 12.0, 9.0, 9.0, 15.0, 1.0]):  # the last: the Trd1 kind
+    CASES.append(("trd", _t, margin_convex(trd_corners(_t[:5]))))
 CASES.append(("paraboloid", [6.0, 9.0, 10.0], margin_paraboloid))
 # the reference's root selection takes the quadratic's leading coefficient as positive, i.e. rhi > rlo
 # (the G4Paraboloid convention R2 > R1); rlo > rhi is not pinned

@@ -93,10 +93,23 @@ VGMI_DEV void top_matrix(GeomView const &g, uint32_t nav, Mat &m)
 
 // ---- shape dispatch on a device shape record. Every function below is templated on TM,
 // the bit mask of shape types the kernel compiles in (bit E for ShapeType E).
+// The distances take the step limit the reference's navigator hands the shapes: the mother's
+// DistanceToOut gets the step limit (navigation/VNavigator.h:246), a daughter's DistanceToIn the
+// current step (base/BVH.h:115). Only the tessellated solid's result depends on it.
 template <unsigned TM>
-VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_in<TM>(s, p, d); }
+VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d, double stepMax)
+{
+  if constexpr ((TM >> kTessellated) & 1u)
+    if (s.type == kTessellated) return tess::dist_in_step64(s.u.tess, tess::pool_walk(s.u.tess), p, d, stepMax);
+  return rec_dist_in<TM>(s, p, d);
+}
 template <unsigned TM>
-VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d) { return rec_dist_out<TM>(s, p, d); }
+VGMI_DEV double dist_out(ShapeRecord const &s, V3 const &p, V3 const &d, double stepMax)
+{
+  if constexpr ((TM >> kTessellated) & 1u)
+    if (s.type == kTessellated) return tess::dist_out_step64(s.u.tess, tess::pool_walk(s.u.tess), p, d, stepMax);
+  return rec_dist_out<TM>(s, p, d);
+}
 template <unsigned TM>
 VGMI_DEV bool contains(ShapeRecord const &s, V3 const &p) { return rec_contains<TM>(s, p); }
 template <unsigned TM>
@@ -287,7 +300,7 @@ VGMI_DEV void check_daughters(GeomView const &g, NavVol const &M, V3 const &lp, 
           const double lbs     = lower_bound_in(S, loc);
           lb                   = vmax(lb, lbs);
           if (!(lbs > capm)) {
-            const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
+            const double dist = dist_in<TM>(S, loc, rotate(P.r, ld), step);
             if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
           }
         }
@@ -383,7 +396,7 @@ VGMI_DEV void check_daughters_mixed(GeomView const &g, NavVol const &M, V3 const
           const double lbs     = lower_bound_in(S, loc);
           lb                   = vmax(lb, lbs);
           if (!(lbs > capm)) {
-            const double dist = dist_in<TM>(S, loc, rotate(P.r, ld));
+            const double dist = dist_in<TM>(S, loc, rotate(P.r, ld), step);
             if (dist < step && !(dist <= 0.0 && pv == last)) step = dist, hit = pv;
           }
           bnd = vmin(bnd, lb);
@@ -696,7 +709,7 @@ VGMI_DEV double compute_step(GeomView const &g, V3 const &gp, V3 const &gd, doub
   const int pvol    = top(g, in.nav);
   NavVol const &M   = g.vols[pvol];
   if (safety) *safety = calc_safety ? compute_safety_local<TM>(g, lp, M, stk) : 0.;
-  double step       = dist_out<TM>(g.shapes[M.shape], lp, ld);
+  double step       = dist_out<TM>(g.shapes[M.shape], lp, ld, limit);
   if (step < 0.) step = 0.;
   int hit = -1;
   if (M.ndaughters == 0) {

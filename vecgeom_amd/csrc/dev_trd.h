@@ -18,12 +18,23 @@ VGMI_DEV void flags(TrdData const &t, Vec3<R> const &p, bool surface, bool &cin,
   cout        = vabs(p.z) > (surface ? R(t.dz) + hT : R(t.dz));
   cin         = surface && vabs(p.z) < R(t.dz) - hT;
   const R cx  = R(t.x2minusx1) * pz - R(2.0) * R(t.dz) * (vabs(p.x) - R(t.dx1));
-  const R cy  = R(t.y2minusy1) * pz - R(2.0) * R(t.dz) * (vabs(p.y) - R(t.dy1));
   if (surface) {
-    cout = cout || cx < -R(t.tolx) || cy < -R(t.toly);
-    cin  = cin && cx > R(t.tolx) && cy > R(t.toly);
+    cout = cout || cx < -R(t.tolx);
+    cin  = cin && cx > R(t.tolx);
   } else {
-    cout = cout || cx < R(0) || cy < R(0);
+    cout = cout || cx < R(0);
+  }
+  if (t.plainY) { // TrdTypes::Trd1 (HasVaryingY kNo)
+    cout = cout || vabs(p.y) > (surface ? R(t.dy1) + hT : R(t.dy1));
+    if (surface) cin = cin && vabs(p.y) < R(t.dy1) - hT;
+    return;
+  }
+  const R cy = R(t.y2minusy1) * pz - R(2.0) * R(t.dz) * (vabs(p.y) - R(t.dy1));
+  if (surface) {
+    cout = cout || cy < -R(t.toly);
+    cin  = cin && cy > R(t.toly);
+  } else {
+    cout = cout || cy < R(0);
   }
 }
 

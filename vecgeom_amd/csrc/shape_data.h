@@ -149,12 +149,14 @@ struct HypeData {
 
 // Trd (VecGeom/volumes/TrdStruct.h:18-121): half-lengths and the cached differences, half sums,
 // slopes, cosines and lateral tolerances of CalculateCached (the y tolerance from the x slope, as
-// there). varyingY: dy1 != dy2 (TrdTypes::checkVaryingY for UniversalTrd).
+// there). varyingY: dy1 != dy2 (TrdTypes::checkVaryingY for UniversalTrd). plainY: the TrdTypes::Trd1
+// specialisation GeoManager::MakeInstance picks when dy1 == dy2 (source/UnplacedTrd.cpp:60-63), whose
+// Inside/Contains test |y| against dy1 instead of the cross product.
 struct TrdData {
   double dx1, dx2, dy1, dy2, dz;
   double x2minusx1, y2minusy1, halfx1plusx2, halfy1plusy2, fx, fy, calfx, calfy, tolx, toly;
   int32_t varyingY;
-  int32_t pad_;
+  int32_t plainY;
 };
 
 // Paraboloid (VecGeom/volumes/ParaboloidStruct.h:17-70): rho^2 = k1 z + k2 between z = -dz (radius rlo)

@@ -277,7 +277,8 @@ int MakeHype(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
-// Trd: the UnplacedTrd constructors (VecGeom/volumes/UnplacedTrd.h:51-80) and TrdStruct::CalculateCached
+// Trd: the UnplacedTrd constructors (VecGeom/volumes/UnplacedTrd.h:51-80), with an optional sixth value 1
+// for the Trd1 kind of GeoManager::MakeInstance (source/UnplacedTrd.cpp:60-63), and TrdStruct::CalculateCached
 // (VecGeom/volumes/TrdStruct.h:102-121; the y tolerance is computed from the x slope, as there).
 int MakeTrd(double const *a, int n, ShapeRecord &r, std::string &err)
 {
@@ -286,10 +287,13 @@ int MakeTrd(double const *a, int n, ShapeRecord &r, std::string &err)
     x1 = x2 = a[0], y1 = y2 = a[1], z = a[2];
   else if (n == 4)
     x1 = a[0], x2 = a[1], y1 = y2 = a[2], z = a[3];
-  else if (n == 5)
+  else if (n == 5 || n == 6)
     x1 = a[0], x2 = a[1], y1 = a[2], y2 = a[3], z = a[4];
   else
-    return err = "trd takes x1 x2 y1 y2 z, x1 x2 y z or x y z", -1;
+    return err = "trd takes x1 x2 y1 y2 z, x1 x2 y z or x y z (and a sixth value 1 for the Trd1 kind)", -1;
+  const bool plainY = n == 6 && a[5] == 1.0;
+  if (n == 6 && !(a[5] == 0.0 || (plainY && y1 == y2)))
+    return err = "trd kind must be 0 (UniversalTrd) or 1 (Trd1, which needs y1 == y2)", -1;
   if (!(x1 >= 0 && x2 >= 0 && y1 >= 0 && y2 >= 0 && z > 0 && (x1 > 0 || x2 > 0) && (y1 > 0 || y2 > 0)))
     return err = "trd half-lengths must be non-negative (not both zero at the two ends) and z > 0", -1;
   r          = ShapeRecord();
@@ -307,6 +311,7 @@ int MakeTrd(double const *a, int n, ShapeRecord &r, std::string &err)
   t.tolx         = 1e-9 * std::sqrt(t.x2minusx1 * t.x2minusx1 + 4 * t.dz * t.dz);
   t.toly         = 1e-9 * std::sqrt(t.x2minusx1 * t.x2minusx1 + 4 * t.dz * t.dz);
   t.varyingY     = t.dy1 != t.dy2 ? 1 : 0;
+  t.plainY       = plainY ? 1 : 0;
   return 0;
 }
 
