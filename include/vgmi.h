@@ -111,6 +111,18 @@ VGMI_API vgmi_status vgmi_shape_query_all(const vgmi_shape *shape, int precision
                                  uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,
                                  double *safety_in, double *safety_out, size_t n, void *stream);
 
+/*
+ * Surface normal per point: VUnplacedVolume::Normal (VecGeom/volumes/UnplacedVolume.h:252), the
+ * shape's own Normal (e.g. BoxImplementation::NormalKernel, BoxImplementation.h:172-191;
+ * UnplacedTube::Normal, source/UnplacedTube.cpp:181-186). Writes the normal to n{x,y,z}[i] and the
+ * validity flag (1: the point is on the surface as the shape's Normal decides it) to valid[i].
+ * Where the reference leaves its normal argument unassigned (tube and hype points near none of the
+ * surfaces their kernels test, torus points on no surface), n{x,y,z}[i] keep the caller's values.
+ * FP64 only.
+ */
+VGMI_API vgmi_status vgmi_shape_normal(const vgmi_shape *shape, const double *px, const double *py, const double *pz,
+                                       double *nx, double *ny, double *nz, uint8_t *valid, size_t n, void *stream);
+
 /* ------------------------------------------------------------------------
  * One-launch batches over many shapes with uneven point counts.
  * No single reference counterpart: it batches the per-shape SOA calls above

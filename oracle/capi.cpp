@@ -188,4 +188,18 @@ int vgo_shape_query(void *s, int op, const double *px, const double *py, const d
   return 0;
 }
 
+// Normal per point; n{x,y,z} hold the caller's vectors on entry (the reference leaves its normal
+// argument unassigned in some branches) and the normals on return.
+int vgo_shape_normal(void *s, const double *px, const double *py, const double *pz, double *nx, double *ny,
+                     double *nz, uint8_t *valid, int64_t n)
+{
+  if (!s) return -1;
+  for (int64_t i = 0; i < n; ++i) {
+    V3 nv(nx[i], ny[i], nz[i]);
+    valid[i] = ((Shape *)s)->Normal(V3(px[i], py[i], pz[i]), nv);
+    nx[i] = nv.x, ny[i] = nv.y, nz[i] = nv.z;
+  }
+  return 0;
+}
+
 } // extern "C"

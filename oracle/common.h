@@ -65,5 +65,8 @@ inline double NonZero(double x) { return x + std::copysign(1.0e-30, x); }
 inline double NonZeroAbs(double x) { return std::fabs(x) + 1.0e-30; }
 // VecCore math::Sign is CopySign(1, x): -0.0 maps to -1.
 inline double Sign(double x) { return std::copysign(1.0, x); }
+// MakePlusTolerantSquare<true> / MakeMinusTolerantSquare<true> (volumes/kernel/GenericKernels.h:65-82)
+inline double PlusTolSq(double x) { return x * (x + 2.0 * kTolerance); }
+inline double MinusTolSq(double x) { return std::fabs(x * (x - 2.0 * kTolerance)); }
 
 } // namespace vgo

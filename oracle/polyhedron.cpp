@@ -354,27 +354,38 @@ double Polyhedron::DistanceToOutZSegment(int i, double zMin, double zMax, V3 con
 }
 
 // PolyhedronImplementation.h:411-557
-double Polyhedron::SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bool ptInside) const
+double Polyhedron::SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bool ptInside, int *iSurf) const
 {
+  // iSurf (when given) receives the surface kind of the minimum as the reference's Normal uses it:
+  // 0 / 1 the phi planes, 2 the inner, 3 the outer quadrilaterals (left unchanged if none is below kInfLength)
+  auto mark = [iSurf](int k) {
+    if (iSurf) *iSurf = k;
+  };
   ZSeg const &S       = segs[i];
   const bool inCutout = phiIndex < 0;
   double sq           = kInfLength;
   if (hasPhiCutout && S.phi.size() == 2) {
     if (ptInside || inCutout) {
+      mark(0);
       sq               = S.phi.DistanceSquared(0, p);
       const double saf = S.phi.DistanceSquared(1, p);
-      if (saf < sq) sq = saf;
+      if (saf < sq) {
+        sq = saf;
+        mark(1);
+      }
       if (inCutout) {
         if (hasLargePhiCutout && S.hasInner()) {
           if (S.inner.size() > 0) {
             double si = S.inner.DistanceSquared(0, p);
             if (si < sq) {
+              mark(2);
               phiIndex = 0;
               sq       = si;
             }
             if (S.inner.size() > 1) {
               si = S.inner.DistanceSquared(S.inner.size() - 1, p);
               if (si < sq) {
+                mark(2);
                 phiIndex = S.inner.size() - 1;
                 sq       = si;
               }
@@ -390,12 +401,14 @@ double Polyhedron::SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bo
     if (S.outer.size() > 0) {
       so = S.outer.DistanceSquared(0, p);
       if (so < sq) {
+        mark(3);
         phiIndex = 0;
         sq       = so;
       }
       if (S.outer.size() > 1) {
         so = S.outer.DistanceSquared(S.outer.size() - 1, p);
         if (so < sq) {
+          mark(3);
           phiIndex = S.outer.size() - 1;
           sq       = so;
         }
@@ -406,12 +419,14 @@ double Polyhedron::SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bo
       if (S.inner.size() > 0) {
         si = S.inner.DistanceSquared(0, p);
         if (si < sq) {
+          mark(2);
           phiIndex = 0;
           sq       = si;
         }
         if (S.inner.size() > 1) {
           si = S.inner.DistanceSquared(S.inner.size() - 1, p);
           if (si < sq) {
+            mark(2);
             phiIndex = S.inner.size() - 1;
             sq       = si;
           }
@@ -426,8 +441,14 @@ double Polyhedron::SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bo
   if (S.hasInner()) {
     if (S.inner.size() > 0) si = S.inner.DistanceSquared(phiIndex, p);
   }
-  if (si < sq) sq = si;
-  if (so < sq) sq = so;
+  if (si < sq) {
+    mark(2);
+    sq = si;
+  }
+  if (so < sq) {
+    mark(3);
+    sq = so;
+  }
   return sq;
 }
 
@@ -454,8 +475,9 @@ void Polyhedron::DistanceToEndcaps(bool inside, V3 const &p, V3 const &d, double
 }
 
 // PolyhedronImplementation.h:616-653
-void Polyhedron::SafetyToEndcapsSquared(V3 const &p, double &dsq) const
+void Polyhedron::SafetyToEndcapsSquared(V3 const &p, double &dsq, int *iz) const
 {
+  if (iz) *iz = 0;
   const double firstD = zs[0] - p[2];
   const double lastD  = zs[NSeg()] - p[2];
   const bool isFirst  = std::fabs(firstD) < std::fabs(lastD);
@@ -467,6 +489,7 @@ void Polyhedron::SafetyToEndcapsSquared(V3 const &p, double &dsq) const
   if (!S.outer.Contains(x)) return;
   if (S.hasInner() && S.inner.Contains(x)) return;
   if (hasPhiCutout && InPhiCutoutWedge(S, x)) return;
+  if (iz) *iz = isFirst ? -1 : 1;
   dsq = dt2;
 }
 

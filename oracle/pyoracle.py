@@ -31,6 +31,8 @@ def lib():
         L.vgo_shape_extent.argtypes = [vp, dp, dp]
         L.vgo_shape_query.restype = ctypes.c_int
         L.vgo_shape_query.argtypes = [vp, ctypes.c_int] + [vp] * 7 + [vp, i64, ctypes.c_int]
+        L.vgo_shape_normal.restype = ctypes.c_int
+        L.vgo_shape_normal.argtypes = [vp] + [vp] * 7 + [i64]
         _LIB = L
     return _LIB
 
@@ -83,6 +85,22 @@ class OracleShape:
         if rc != 0:
             raise RuntimeError("vgo_shape_query failed: %d" % rc)
         return out
+
+    def normal(self, p, init=None):
+        """Normal per point of p (3, n). init (3, n): the caller's vectors, kept where the reference
+        leaves its normal argument unassigned (default NaN). Returns (normals (3, n), valid uint8 (n,))."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        nv = np.full((3, n), np.nan) if init is None else np.array(init, dtype=np.float64, copy=True)
+        nv = np.ascontiguousarray(nv)
+        valid = np.zeros(n, np.uint8)
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        nx, ny, nz = (np.ascontiguousarray(nv[i]) for i in range(3))
+        rc = lib().vgo_shape_normal(self._h, _dptr(px), _dptr(py), _dptr(pz), _dptr(nx), _dptr(ny), _dptr(nz),
+                                    _dptr(valid), n)
+        if rc != 0:
+            raise RuntimeError("vgo_shape_normal failed: %d" % rc)
+        return np.stack([nx, ny, nz]), valid
 
 
 def _np_ptr(a):

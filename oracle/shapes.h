@@ -21,6 +21,9 @@ public:
   virtual double SafetyToIn(V3 const &p) const = 0;
   virtual double SafetyToOut(V3 const &p) const = 0;
   virtual void Extent(V3 &lo, V3 &hi) const = 0;
+  // VUnplacedVolume::Normal (VecGeom/volumes/UnplacedVolume.h:252), restated in normal.cpp: n is the
+  // caller's vector (left as it was where the reference leaves it unassigned); returns valid
+  virtual bool Normal(V3 const &p, V3 &n) const = 0;
 };
 
 // Phi wedge: VecGeom/volumes/Wedge_Evolution.h, ctor source/Wedge_Evolution.cpp:17-128
@@ -79,6 +82,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override { lo = -h; hi = h; }
 };
 
@@ -101,6 +105,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override;
   // internals shared with other shapes
   void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
@@ -135,6 +140,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override;
   void ContainsInside(V3 const &p, bool forInside, bool &cin, bool &cout) const;
 };
@@ -153,6 +159,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override { lo = V3(-r, -r, -r), hi = V3(r, r, r); }
 };
 
@@ -173,6 +180,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override
   {
     lo = V3(-endOuterRadius, -endOuterRadius, -dz), hi = V3(endOuterRadius, endOuterRadius, dz);
@@ -205,6 +213,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override
   {
     const double mx = dx1 > dx2 ? dx1 : dx2, my = dy1 > dy2 ? dy1 : dy2;
@@ -228,6 +237,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override { lo = V3(-dx, -dy, -dz), hi = V3(dx, dy, dz); }
   void Flags(V3 const &p, bool forInside, bool &cin, bool &cout) const;
   bool OnZPlane(bool top, V3 const &p) const;
@@ -255,6 +265,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override;
   // helpers shared with the polycone
   double RadiusAt(bool inner, double z) const;
@@ -291,6 +302,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override;
 };
 } // namespace vgo
@@ -317,6 +329,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override;
 };
 } // namespace vgo
@@ -373,9 +386,9 @@ public:
   int FindPhiSegment(V3 const &p) const;
   double DistanceToInZSegment(int i, V3 const &p, V3 const &d) const;
   double DistanceToOutZSegment(int i, double zMin, double zMax, V3 const &p, V3 const &d) const;
-  double SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bool ptInside) const;
+  double SafetyToZSegmentSquared(int i, int &phiIndex, V3 const &p, bool ptInside, int *iSurf = nullptr) const;
   void DistanceToEndcaps(bool inside, V3 const &p, V3 const &d, double &distance) const;
-  void SafetyToEndcapsSquared(V3 const &p, double &dsq) const;
+  void SafetyToEndcapsSquared(V3 const &p, double &dsq, int *iz = nullptr) const;
   bool InPhiCutoutWedge(ZSeg const &s, V3 const &p) const;
   int InsideSegBorder(V3 const &p, int zIndex) const;
   int InsideSegPhi(V3 const &p, int zIndex, int phiIndex) const;
@@ -387,6 +400,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override { return Safety(p, false); }
   double SafetyToOut(V3 const &p) const override { return Safety(p, true); }
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override;
 };
 } // namespace vgo
@@ -411,6 +425,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override { bounding.Extent(lo, hi); }
 };
 // TorusImplementation2.h:14-190
@@ -449,6 +464,7 @@ public:
   double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
   double SafetyToIn(V3 const &p) const override;
   double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
   void Extent(V3 &lo, V3 &hi) const override
   {
     lo = minExt;

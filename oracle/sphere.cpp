@@ -10,8 +10,6 @@
 namespace vgo {
 
 namespace {
-inline double PlusTolSq(double x) { return x * (x + 2.0 * kTolerance); }            // GenericKernels.h:65-70
-inline double MinusTolSq(double x) { return std::fabs(x * (x - 2.0 * kTolerance)); } // GenericKernels.h:76-82
 inline double Min(double a, double b) { return std::min(a, b); }
 inline double Max(double a, double b) { return std::max(a, b); }
 } // namespace

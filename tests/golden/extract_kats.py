@@ -14,7 +14,10 @@ Expectation kinds: "approx" (ApproxEqual, test/unit_tests/ApproxEqual.h),
 "negative" (assert(Dist < 0)), "nonpositive" (assert(Dist <= 0),
 TestWrongSidePoint), "absdiff" (assert(fabs(s->M(..) - v) < tol)), "bool"
 (assert(s->Contains(p)) / assert(!s->Contains(p))), "extent" (s.Extent(lo, hi) then
-assert(ApproxEqual(lo, Vec_t(..))), method "Extent").
+assert(ApproxEqual(lo, Vec_t(..))), method "Extent"); for method "Normal" (valid = s.Normal(p, normal)):
+"normal" (ApproxEqual(normal, Vec_t(..))), "normal_valid" (assert([!]valid)), "normal_mag" /
+"normal_mag2" (ApproxEqual(normal.Mag[2](), v)). A Normal point written p0 + Dist * v, with Dist a stored
+distance query, is recorded as "chain": {"base": p0, "dir": v, "dist": the query} ("p" null).
 
 Run in a container that holds the reference tree:
     python tests/golden/extract_kats.py /root/reference
@@ -345,7 +348,7 @@ def skip_reason(body, shapes, pending, unknown_results):
             return why
         return "comparison on a query result not recognised"
     if re.search(r"\b(?:normal|Normal|valid|norm\w*)\b", body):
-        return "normal vector or validity flag (Normal() is not on the path)"
+        return "Normal() of a shape or point the extractor does not reconstruct (loops, unknown shapes)"
     return "not a query assert (capacity, surface area, getters, bounding box)"
 
 
@@ -353,6 +356,7 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
     text = open(path).read()
     vecs, scalars, shapes, pending, arrays, extents, logicals = {}, {}, {}, {}, {}, {}, {}
     unknown_results = {}
+    normals, valids = {}, {}  # Normal() output vector / validity variables -> (shape, p, chain, line)
     vec_arrays = set()
     global CURRENT_VECS
     CURRENT_VECS = vecs
@@ -373,6 +377,137 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
             else:
                 vals.append(eval_scalar(a, scalars))
         return vals
+
+    def normal_point(expr):
+        """The point argument of a Normal call: (p, None), or (None, chain) for p0 + Dist * v where Dist
+        holds a stored distance query (the test derives the surface point from the query's result)."""
+        try:
+            return eval_vec(expr, vecs, scalars, vec_type), None
+        except (Unsupported, IndexError, ValueError):
+            pass
+        e = _strip_parens(expr.strip())
+        m = (re.fullmatch(r"(\w+)\s*([-+])\s*(\w+)\s*\*\s*(\w+)", e))
+        if m:
+            a, sign, x, y = m.groups()
+            dvar, vvar = (x, y) if x in pending else (y, x)
+            if a in vecs and vvar in vecs and dvar in pending and pending[dvar][1] in ("DistanceToIn",
+                                                                                        "DistanceToOut"):
+                sh2, meth2, p2, d2, _ = pending[dvar]
+                # p0 - Dist * v is p0 + Dist * (-v) in floating point as well
+                v = [-c for c in vecs[vvar]] if sign == "-" else list(vecs[vvar])
+                return None, {"base": list(vecs[a]), "dir": v,
+                              "dist": {"shape": shapes[sh2], "method": meth2, "p": p2, "d": d2}}
+        raise Unsupported(expr)
+
+    def normal_assert_term(t):
+        """One term of an assert on a Normal() result: an expectation dict with its call, or None."""
+        t = _strip_parens(t.strip())
+        mv = re.fullmatch(r"(!?)\s*(\w+)(?:\s*==\s*(true|false))?", t)
+        if mv and mv.group(2) in valids:
+            val = (mv.group(3) or "true") == "true"
+            return valids[mv.group(2)], {"kind": "normal_valid", "value": val != bool(mv.group(1))}
+        mn = re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,\s*(.+)\)", t, re.S)
+        if mn and mn.group(1) in normals:
+            return normals[mn.group(1)], {"kind": "normal", "value": eval_vec(mn.group(2), vecs, scalars,
+                                                                               vec_type)}
+        mm = re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*\.\s*(Mag2?)\s*\(\s*\)\s*,\s*(.+)\)", t, re.S)
+        if mm and mm.group(1) in normals:
+            kind = "normal_mag2" if mm.group(2) == "Mag2" else "normal_mag"
+            return normals[mm.group(1)], {"kind": kind, "value": eval_scalar(mm.group(3), scalars)}
+        mc = re.fullmatch(r"(\w+)\s*(?:\.|->)\s*Normal\s*\((.*)\)", t, re.S)
+        if mc and mc.group(1) in shapes:
+            a = split_args(mc.group(2))
+            pt, chain = normal_point(a[0])
+            call = (mc.group(1), pt, chain, ln)
+            normals[a[1].strip()] = call
+            vecs.pop(a[1].strip(), None)
+            return call, {"kind": "normal_valid", "value": True}
+        return None
+
+    def normal_records(body):
+        """Records of an assert that tests a Normal() result, alone or in a conjunction with distance
+        terms (assert(ApproxEqual(Dist, 20) && ApproxEqual(normal, vx))). None when the body names no
+        Normal() result."""
+        names = set(re.findall(r"\b(\w+)\b", body))
+        if not (names & (set(normals) | set(valids))) and not re.search(r"(?:\.|->)\s*Normal\s*\(", body):
+            return None
+        if "||" in body:
+            return []
+        out = []
+        for t in body.split("&&"):
+            try:
+                r = normal_assert_term(t)
+            except (Unsupported, IndexError, ValueError, ZeroDivisionError):
+                return []
+            if r is not None:
+                (sh, pt, chain, line), exp = r
+                out.append({"line": "%s:%d" % (fname, ln), "shape": shapes[sh], "method": "Normal", "p": pt,
+                            "d": None, "chain": chain, "expect": exp})
+                continue
+            rec, _ = query_assert(t.strip())
+            if rec is None:
+                return []
+            sh, meth, p, d, line, exp = rec
+            out.append({"line": "%s:%d" % (fname, line), "shape": shapes[sh], "method": meth, "p": p, "d": d,
+                        "expect": exp})
+        return out
+
+    def query_assert(body):
+        """One assert body on the shape queries: (record tuple or None, reason or None)."""
+        rec, why = None, None
+        try:
+            rec = direct_query_assert(body, shapes, vecs, scalars, vec_type, ln)
+        except (Unsupported, IndexError, ValueError, ZeroDivisionError):
+            rec, why = None, "argument or expected value not evaluable"
+        if rec is None and why is None:
+            if "&&" in body and "||" not in body:
+                # a conjunction: every scalar ApproxEqual(var, value) term on a stored result is
+                # a requirement of its own (the other terms, e.g. on normals, are skipped)
+                terms = [t.strip() for t in body.split("&&")]
+                body = next((t for t in terms
+                             if re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,.*\)", t)
+                             and re.match(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)", t).group(1) in pending), body)
+            alts = re.findall(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,\s*((?:[^()]|\([^()]*(?:\([^()]*\))?[^()]*\))+)\)", body)
+            if alts and all(a[0] == alts[0][0] for a in alts) and alts[0][0] in pending and "&&" not in body:
+                var = alts[0][0]
+                try:
+                    vals = [eval_scalar(a[1], scalars) for a in alts]
+                    exp = {"kind": "approx", "value": vals[0]} if len(vals) == 1 else {"kind": "anyof", "values": vals}
+                    sh, meth, p, d, _ = pending[var]
+                    rec = (sh, meth, p, d, ln, exp)
+                except Unsupported:
+                    rec = None
+            else:
+                mm = re.fullmatch(r"(\w+)\s*==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", body)
+                if mm and mm.group(1) in pending and pending[mm.group(1)][1] == "Inside":
+                    sh, meth, p, d, _ = pending[mm.group(1)]
+                    rec = (sh, meth, p, d, ln, {"kind": "inside", "value": mm.group(2)})
+                mm = re.fullmatch(r"(\w+)\s*(<|<=)\s*0(?:\.0*)?", body)
+                if mm and mm.group(1) in pending:
+                    sh, meth, p, d, _ = pending[mm.group(1)]
+                    rec = (sh, meth, p, d, ln, {"kind": "negative" if mm.group(2) == "<" else "nonpositive"})
+                # TestWrongSidePoint(Dist, msg) is Dist <= 0 (test/unit_tests/TestSphere.cpp:24-29)
+                mm = re.fullmatch(r"TestWrongSidePoint\s*\(\s*(\w+)\s*,.*\)", body)
+                if mm and mm.group(1) in pending:
+                    sh, meth, p, d, _ = pending[mm.group(1)]
+                    rec = (sh, meth, p, d, ln, {"kind": "nonpositive"})
+                mm = re.fullmatch(r"(\w+)\s*(>|>=)\s*0(?:\.0*)?", body)
+                if mm and mm.group(1) in pending:
+                    sh, meth, p, d, _ = pending[mm.group(1)]
+                    rec = (sh, meth, p, d, ln, {"kind": "positive" if mm.group(2) == ">" else "nonnegative"})
+                # d == value (exact), d >= kInfLength (a miss)
+                mm = re.fullmatch(r"(\w+)\s*(==|>=)\s*(.+)", body)
+                if rec is None and mm and mm.group(1) in pending and pending[mm.group(1)][1] != "Inside":
+                    try:
+                        v = eval_scalar(mm.group(3), scalars)
+                        sh, meth, p, d, _ = pending[mm.group(1)]
+                        if mm.group(2) == "==":
+                            rec = (sh, meth, p, d, ln, {"kind": "exact", "value": v})
+                        elif v == DBL_MAX:
+                            rec = (sh, meth, p, d, ln, {"kind": "exact", "value": v})
+                    except Unsupported:
+                        why = "argument or expected value not evaluable"
+        return rec, why
 
     for ln, st in statements(text):
         # drop block braces and function headers joined in front of a statement
@@ -522,6 +657,26 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
                     name = (d1 or d2).group(1) if (d1 or d2) else None
                     vecs.pop(name, None)
             continue
+        # Normal calls: [bool] valid = s.Normal(p, normal) or s.Normal(p, normal)
+        m = re.match(r"(?:bool\s+)?(?:(\w+)\s*=\s*)?(\w+)\s*(?:\.|->)\s*Normal\s*\((.*)\)$", st)
+        if m:
+            vvar, sh, args = m.groups()
+            a = split_args(args)
+            nvar = a[1].strip() if len(a) == 2 else None
+            for v in (vvar, nvar):
+                if v:
+                    normals.pop(v, None), valids.pop(v, None), pending.pop(v, None), vecs.pop(v, None)
+                    scalars.pop(v, None)
+            if sh in shapes and nvar:
+                try:
+                    pt, chain = normal_point(a[0])
+                    call = (sh, pt, chain, ln)
+                    normals[nvar] = call
+                    if vvar:
+                        valids[vvar] = call
+                except Unsupported:
+                    pass
+            continue
         # queries stored into a variable
         m = re.match(r"(?:Precision\s+|double\s+|auto\s+|vecgeom::EnumInside\s+|EnumInside\s+|Inside_t\s+)?(\w+)\s*=\s*"
                      r"(\w+)\s*(?:\.|->)\s*(%s)\s*\((.*)\)$" % METHODS, st)
@@ -574,59 +729,18 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
                         stats["skipped"].setdefault(
                             "not a query assert (capacity, surface area, getters, bounding box)", []).append(ln)
                 continue
-            rec, why = None, None
-            try:
-                rec = direct_query_assert(body, shapes, vecs, scalars, vec_type, ln)
-            except (Unsupported, IndexError, ValueError, ZeroDivisionError):
-                rec, why = None, "argument or expected value not evaluable"
-            if rec is None and why is None:
-                if "&&" in body and "||" not in body:
-                    # a conjunction: every scalar ApproxEqual(var, value) term on a stored result is
-                    # a requirement of its own (the other terms, e.g. on normals, are skipped)
-                    terms = [t.strip() for t in body.split("&&")]
-                    body = next((t for t in terms
-                                 if re.fullmatch(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,.*\)", t)
-                                 and re.match(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)", t).group(1) in pending), body)
-                alts = re.findall(r"ApproxEqual(?:<\w+>)?\s*\(\s*(\w+)\s*,\s*((?:[^()]|\([^()]*(?:\([^()]*\))?[^()]*\))+)\)", body)
-                if alts and all(a[0] == alts[0][0] for a in alts) and alts[0][0] in pending and "&&" not in body:
-                    var = alts[0][0]
-                    try:
-                        vals = [eval_scalar(a[1], scalars) for a in alts]
-                        exp = {"kind": "approx", "value": vals[0]} if len(vals) == 1 else {"kind": "anyof", "values": vals}
-                        sh, meth, p, d, _ = pending[var]
-                        rec = (sh, meth, p, d, ln, exp)
-                    except Unsupported:
-                        rec = None
-                else:
-                    mm = re.fullmatch(r"(\w+)\s*==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", body)
-                    if mm and mm.group(1) in pending and pending[mm.group(1)][1] == "Inside":
-                        sh, meth, p, d, _ = pending[mm.group(1)]
-                        rec = (sh, meth, p, d, ln, {"kind": "inside", "value": mm.group(2)})
-                    mm = re.fullmatch(r"(\w+)\s*(<|<=)\s*0(?:\.0*)?", body)
-                    if mm and mm.group(1) in pending:
-                        sh, meth, p, d, _ = pending[mm.group(1)]
-                        rec = (sh, meth, p, d, ln, {"kind": "negative" if mm.group(2) == "<" else "nonpositive"})
-                    # TestWrongSidePoint(Dist, msg) is Dist <= 0 (test/unit_tests/TestSphere.cpp:24-29)
-                    mm = re.fullmatch(r"TestWrongSidePoint\s*\(\s*(\w+)\s*,.*\)", body)
-                    if mm and mm.group(1) in pending:
-                        sh, meth, p, d, _ = pending[mm.group(1)]
-                        rec = (sh, meth, p, d, ln, {"kind": "nonpositive"})
-                    mm = re.fullmatch(r"(\w+)\s*(>|>=)\s*0(?:\.0*)?", body)
-                    if mm and mm.group(1) in pending:
-                        sh, meth, p, d, _ = pending[mm.group(1)]
-                        rec = (sh, meth, p, d, ln, {"kind": "positive" if mm.group(2) == ">" else "nonnegative"})
-                    # d == value (exact), d >= kInfLength (a miss)
-                    mm = re.fullmatch(r"(\w+)\s*(==|>=)\s*(.+)", body)
-                    if rec is None and mm and mm.group(1) in pending and pending[mm.group(1)][1] != "Inside":
-                        try:
-                            v = eval_scalar(mm.group(3), scalars)
-                            sh, meth, p, d, _ = pending[mm.group(1)]
-                            if mm.group(2) == "==":
-                                rec = (sh, meth, p, d, ln, {"kind": "exact", "value": v})
-                            elif v == DBL_MAX:
-                                rec = (sh, meth, p, d, ln, {"kind": "exact", "value": v})
-                        except Unsupported:
-                            why = "argument or expected value not evaluable"
+            nrecs = normal_records(body)
+            if nrecs is not None:
+                records.extend(nrecs)
+                if stats is not None:
+                    stats["asserts"] += 1
+                    if nrecs:
+                        stats["extracted"] += 1
+                    else:
+                        stats["skipped"].setdefault("Normal() at a point the extractor cannot evaluate (sampled surface points)",
+                                                    []).append(ln)
+                continue
+            rec, why = query_assert(body)
             if rec:
                 sh, meth, p, d, line, exp = rec
                 records.append({"line": "%s:%d" % (fname, line), "shape": shapes[sh], "method": meth,
@@ -644,6 +758,8 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
         if m and m.group(1) not in shapes:
             name, expr = m.groups()
             pending.pop(name, None)
+            normals.pop(name, None)
+            valids.pop(name, None)
             try:
                 scalars[name] = eval_scalar(expr, scalars)
                 vecs.pop(name, None)
@@ -652,6 +768,19 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
                 scalars.pop(name, None)
             try:
                 vecs[name] = eval_vec(expr, vecs, scalars, vec_type)
+            except Unsupported:
+                vecs.pop(name, None)
+            continue
+        # v.Set(x, y, z) / v.Set(a) (Vector3D::Set, base/Vector3D.h)
+        m = re.fullmatch(r"(\w+)\s*\.\s*Set\s*\((.*)\)", st)
+        if m and m.group(1) not in shapes:
+            name = m.group(1)
+            pending.pop(name, None), normals.pop(name, None), valids.pop(name, None)
+            try:
+                a = [eval_scalar(x, scalars) for x in split_args(m.group(2))]
+                if len(a) not in (1, 3):
+                    raise Unsupported(st)
+                vecs[name] = a * 3 if len(a) == 1 else a
             except Unsupported:
                 vecs.pop(name, None)
             continue

@@ -28,6 +28,33 @@ def load(name):
         return json.load(f)["records"]
 
 
+def normal_point(rec, distance):
+    """The point of a Normal record: rec["p"], or base + Dist * dir for a chained record, with Dist from
+    distance(shape_params, method, p, d) (the reference test feeds its own query result back)."""
+    c = rec.get("chain")
+    if c is None:
+        return rec["p"]
+    q = c["dist"]
+    dist = float(distance(q["shape"], q["method"], q["p"], q["d"]))
+    return [b + dist * v for b, v in zip(c["base"], c["dir"])]
+
+
+def check_normal(rec, normal, valid):
+    """ApproxEqual on each component (test/unit_tests/ApproxEqual.h:22-26), the validity flag, or the
+    length / squared length of the normal."""
+    e = rec["expect"]
+    if e["kind"] == "normal":
+        return all(approx_equal(float(a), float(b)) for a, b in zip(normal, e["value"]))
+    if e["kind"] == "normal_valid":
+        return bool(valid) == e["value"]
+    m2 = sum(float(a) * float(a) for a in normal)
+    if e["kind"] == "normal_mag":
+        return approx_equal(m2 ** 0.5, e["value"])
+    if e["kind"] == "normal_mag2":
+        return approx_equal(m2, e["value"])
+    raise ValueError(e)
+
+
 def check(rec, got):
     e = rec["expect"]
     if e["kind"] == "approx":

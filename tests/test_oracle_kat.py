@@ -3,6 +3,7 @@
 The vectors in tests/golden/kat_*.json are extracted from the reference unit
 tests (test/unit_tests/Test<Shape>.cpp); each record cites its source line.
 """
+import numpy as np
 import pytest
 
 from oracle.pyoracle import OracleShape
@@ -10,6 +11,12 @@ from tests import kat_util
 
 SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated",
           "hype", "trd", "paraboloid"]
+
+
+def _oracle_distance(kind):
+    def dist(params, method, p, d):
+        return OracleShape(kind, params).query(method, kat_util.soa([p]), kat_util.soa([d]))[0]
+    return dist
 
 
 @pytest.mark.parametrize("shape", SHAPES)
@@ -23,6 +30,12 @@ def test_oracle_matches_reference_kats(shape):
             got = s.extent()
             if not kat_util.check(r, got):
                 failures.append((r["line"], "Extent", r["expect"], got))
+            continue
+        if r["method"] == "Normal":
+            pt = kat_util.normal_point(r, _oracle_distance(shape))
+            nv, valid = s.normal(kat_util.soa([pt]), init=np.zeros((3, 1)))
+            if not kat_util.check_normal(r, nv[:, 0], valid[0]):
+                failures.append((r["line"], "Normal", r["expect"], list(nv[:, 0]), int(valid[0])))
             continue
         p = kat_util.soa([r["p"]])
         d = kat_util.soa([r["d"]]) if r["d"] is not None else None

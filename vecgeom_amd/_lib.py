@@ -45,6 +45,7 @@ def _declare(L):
         "vgmi_shape_extent": [vp, dp, dp],
         "vgmi_shape_query": [vp, i, i] + [vp] * 8 + [sz, vp],
         "vgmi_shape_query_all": [vp, i] + [vp] * 12 + [sz, vp],
+        "vgmi_shape_normal": [vp] + [vp] * 7 + [sz, vp],
     }
     for name, args in sigs.items():
         if hasattr(L, name):

@@ -288,26 +288,38 @@ VGMI_DEV R DistanceToOutZSegment(PolyhedronData const &P, int i, double zMin, do
 }
 
 template <class R>
-VGMI_DEV R SafetyToZSegmentSquared(PolyhedronData const &P, int i, int &phiIndex, Vec3<R> const &p, bool ptInside)
+VGMI_DEV R SafetyToZSegmentSquared(PolyhedronData const &P, int i, int &phiIndex, Vec3<R> const &p, bool ptInside,
+                                   int *iSurf = nullptr)
 {
+  // iSurf (when given) receives the surface kind of the minimum, as ScalarNormalKernel uses it
+  // (PolyhedronImplementation.h:448-557): 0 / 1 the phi planes, 2 the inner, 3 the outer quadrilaterals
+  auto mark = [iSurf](int k) {
+    if (iSurf) *iSurf = k;
+  };
   const QSet outer = qset(P, i, 0), phi = qset(P, i, 1), inner = qset(P, i, 2);
   const bool inCutout = phiIndex < 0;
   R sq                = InfLen<R>();
   if (P.hasPhiCutout && phi.n == 2) {
     if (ptInside || inCutout) {
+      mark(0);
       sq          = QDistanceSquared(phi, 0, p);
       const R saf = QDistanceSquared(phi, 1, p);
-      if (saf < sq) sq = saf;
+      if (saf < sq) {
+        sq = saf;
+        mark(1);
+      }
       if (inCutout) {
         if (P.hasLargePhiCutout && inner.n > 0) {
           R si = QDistanceSquared(inner, 0, p);
           if (si < sq) {
+            mark(2);
             phiIndex = 0;
             sq       = si;
           }
           if (inner.n > 1) {
             si = QDistanceSquared(inner, inner.n - 1, p);
             if (si < sq) {
+              mark(2);
               phiIndex = inner.n - 1;
               sq       = si;
             }
@@ -321,12 +333,14 @@ VGMI_DEV R SafetyToZSegmentSquared(PolyhedronData const &P, int i, int &phiIndex
     if (outer.n > 0) {
       R so = QDistanceSquared(outer, 0, p);
       if (so < sq) {
+        mark(3);
         phiIndex = 0;
         sq       = so;
       }
       if (outer.n > 1) {
         so = QDistanceSquared(outer, outer.n - 1, p);
         if (so < sq) {
+          mark(3);
           phiIndex = outer.n - 1;
           sq       = so;
         }
@@ -335,12 +349,14 @@ VGMI_DEV R SafetyToZSegmentSquared(PolyhedronData const &P, int i, int &phiIndex
     if (inner.n > 0) {
       R si = QDistanceSquared(inner, 0, p);
       if (si < sq) {
+        mark(2);
         phiIndex = 0;
         sq       = si;
       }
       if (inner.n > 1) {
         si = QDistanceSquared(inner, inner.n - 1, p);
         if (si < sq) {
+          mark(2);
           phiIndex = inner.n - 1;
           sq       = si;
         }
@@ -352,8 +368,14 @@ VGMI_DEV R SafetyToZSegmentSquared(PolyhedronData const &P, int i, int &phiIndex
   if (outer.n > 0) so = QDistanceSquared(outer, phiIndex, p);
   R si = InfLen<R>();
   if (inner.n > 0) si = QDistanceSquared(inner, phiIndex, p);
-  if (si < sq) sq = si;
-  if (so < sq) sq = so;
+  if (si < sq) {
+    mark(2);
+    sq = si;
+  }
+  if (so < sq) {
+    mark(3);
+    sq = so;
+  }
   return sq;
 }
 
@@ -374,8 +396,9 @@ VGMI_DEV void DistanceToEndcaps(PolyhedronData const &P, bool inside, Vec3<R> co
 }
 
 template <class R>
-VGMI_DEV void SafetyToEndcapsSquared(PolyhedronData const &P, Vec3<R> const &p, R &dsq)
+VGMI_DEV void SafetyToEndcapsSquared(PolyhedronData const &P, Vec3<R> const &p, R &dsq, int *iz = nullptr)
 {
+  if (iz) *iz = 0;
   const R firstD     = R(zp(P, 0)) - p.z;
   const R lastD      = R(zp(P, P.nseg)) - p.z;
   const bool isFirst = vabs(firstD) < vabs(lastD);
@@ -388,6 +411,7 @@ VGMI_DEV void SafetyToEndcapsSquared(PolyhedronData const &P, Vec3<R> const &p, 
   const QSet in = qset(P, seg, 2);
   if (in.n > 0 && QContains(in, x)) return;
   if (P.hasPhiCutout && InPhiCutoutWedge(P, seg, x)) return;
+  if (iz) *iz = isFirst ? -1 : 1;
   dsq = dt2;
 }
 

@@ -411,6 +411,37 @@ VGMI_DEV double safety_sq_threaded(TessellatedData const &s, W const &w, V3d con
   return best;
 }
 
+// SafetySq<false> together with the facet that gives it, for TessellatedImplementation::NormalKernel
+// (TessellatedImplementation.h:162-172): the walk of safety_sq_threaded over the pool, keeping among
+// equal squared safeties the lowest facet index. Returns the facet's slot in the pool (-1 when the
+// solid has no facet).
+VGMI_DEV int nearest_facet_slot(TessellatedData const &s, V3d const &p, bool toIn)
+{
+  const PoolWalk w = pool_walk(s);
+  double best      = kInfLength;
+  int bestSlot = -1, bestIdx = 0x7fffffff;
+  int j = 0;
+  while (j < s.nnodes) {
+    const NodeRec nd = w.node(j);
+    if (node_dist2(nd.lo, nd.hi, p) > best) {
+      j = nd.escape;
+      continue;
+    }
+    if (nd.count == 0) {
+      ++j;
+      continue;
+    }
+    for (int k = nd.first; k < nd.first + nd.count; ++k) {
+      double sq = kInfLength;
+      facet_safety_sq(w.plane(k), facet(s, k), p, toIn, best, sq);
+      const int idx = int(facet(s, k)[25]);
+      if (sq < best || (sq == best && sq < kInfLength && idx < bestIdx)) best = sq, bestSlot = k, bestIdx = idx;
+    }
+    j = nd.escape;
+  }
+  return bestSlot;
+}
+
 // The facets of the leaf reached by always descending into the nearer child give a first minimum
 // (an upper bound of the result) that the full search prunes against.
 template <class W>

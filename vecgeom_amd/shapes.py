@@ -88,6 +88,21 @@ class UnplacedShape:
     def SafetyToOut(self, points, out=None, precision=FP64):
         return self._query("SafetyToOut", points, out=out, precision=precision)
 
+    def Normal(self, points, out=None, valid=None):
+        """Surface normals (VUnplacedVolume::Normal, VecGeom/volumes/UnplacedVolume.h:252): returns
+        (normals as an SOA3D, valid uint8 tensor). Where the reference leaves the normal unassigned,
+        the vector of `out` is kept (a fresh `out` starts at zero)."""
+        n, dev = len(points), points.x.device
+        if out is None:
+            out = SOA3D.zeros(n, dev)
+        same_length(points, out)
+        if valid is None:
+            valid = torch.empty(n, dtype=torch.uint8, device=dev)
+        device_array(valid, torch.uint8, n, "valid")
+        check(lib().vgmi_shape_normal(self._h, *points.ptrs(), *out.ptrs(), valid.data_ptr(), n, _stream()),
+              "vgmi_shape_normal")
+        return out, valid
+
     def QueryAll(self, points, dirs, precision=FP64):
         """All six methods in one pass; returns a dict of device tensors."""
         n, dev = len(points), points.x.device

@@ -44,6 +44,10 @@ class SOA3D:
     def empty(cls, n, device="cuda"):
         return cls(*(torch.empty(n, dtype=torch.float64, device=device) for _ in range(3)))
 
+    @classmethod
+    def zeros(cls, n, device="cuda"):
+        return cls(*(torch.zeros(n, dtype=torch.float64, device=device) for _ in range(3)))
+
     def size(self):
         return self.x.numel()
 
