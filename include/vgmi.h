@@ -255,6 +255,16 @@ VGMI_API vgmi_status vgmi_placed_query(const vgmi_geom *geom, int placed_id, int
                                        size_t n, void *stream);
 
 /*
+ * Surface normal of a placed volume: VPlacedVolume::Normal (source/PlacedVolume.cpp:126-137). Points
+ * are in the frame the volume is placed in; the normal is returned in that frame (the unplaced
+ * Normal of the local point, started from a zero vector, rotated back). placed_id as in
+ * vgmi_placed_query. valid[i] as in vgmi_shape_normal. FP64 only.
+ */
+VGMI_API vgmi_status vgmi_placed_normal(const vgmi_geom *geom, int placed_id, const double *px, const double *py,
+                                        const double *pz, double *nx, double *ny, double *nz, uint8_t *valid,
+                                        size_t n, void *stream);
+
+/*
  * One navigation step per track: VNavigatorHelper<BVHNavigator>::ComputeStepAndPropagatedState
  * (relocate = 1) or ComputeStep (relocate = 0) (navigation/VNavigator.h:436-540).
  * Inputs are global points/directions, per-track step limits and in-states;

@@ -42,6 +42,22 @@ V3 Transform::TransformDirection(V3 const &m) const
   return l;
 }
 
+// Transformation3D.h:808-823 InverseTransformKernel<true> (InverseTransformDirection, :858-864)
+V3 Transform::InverseTransformDirection(V3 const &l) const
+{
+  V3 m;
+  m.x = l.x * r[0];
+  m.x += l.y * r[1];
+  m.x += l.z * r[2];
+  m.y = l.x * r[3];
+  m.y += l.y * r[4];
+  m.y += l.z * r[5];
+  m.z = l.x * r[6];
+  m.z += l.y * r[7];
+  m.z += l.z * r[8];
+  return m;
+}
+
 // Transformation3D.h:800-830 InverseTransformKernel<false>
 V3 Transform::InverseTransform(V3 const &l) const
 {

@@ -26,6 +26,7 @@ struct Transform {
   V3 Transform3(V3 const &m) const;        // master -> local point
   V3 TransformDirection(V3 const &m) const; // master -> local direction
   V3 InverseTransform(V3 const &l) const;  // local -> master point
+  V3 InverseTransformDirection(V3 const &l) const; // local -> master direction
   void MultiplyFromRight(Transform const &rhs);
 };
 

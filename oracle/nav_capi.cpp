@@ -240,4 +240,23 @@ int vgo_placed_query(void *gp, int pvol, int op, const double *px, const double 
   return 0;
 }
 
+// VPlacedVolume::Normal (source/PlacedVolume.cpp:126-137): the point to the volume's frame, the
+// unplaced Normal starting from a zero vector, the normal back to the mother's frame.
+int vgo_placed_normal(void *gp, int pvol, const double *px, const double *py, const double *pz, double *nx,
+                      double *ny, double *nz, uint8_t *valid, int64_t n)
+{
+  Geometry *g = (Geometry *)gp;
+  if (pvol < 0 || pvol >= int(g->pvols.size())) return -1;
+  Transform const &tr = g->pvols[pvol].tr;
+  Shape const &s      = g->ShapeOf(pvol);
+  for (int64_t i = 0; i < n; ++i) {
+    const V3 lp = tr.Transform3(V3(px[i], py[i], pz[i]));
+    V3 ln;
+    valid[i]    = s.Normal(lp, ln);
+    const V3 mn = tr.InverseTransformDirection(ln);
+    nx[i] = mn.x, ny[i] = mn.y, nz[i] = mn.z;
+  }
+  return 0;
+}
+
 } // extern "C"

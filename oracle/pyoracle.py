@@ -133,6 +133,8 @@ class OracleGeometry:
             L.vgo_nav_level_locate.argtypes = [vp] + [vp] * 9 + [i64, ci]
             L.vgo_placed_query.restype = ci
             L.vgo_placed_query.argtypes = [vp, ci, ci] + [vp] * 8 + [i64, ci]
+            L.vgo_placed_normal.restype = ci
+            L.vgo_placed_normal.argtypes = [vp, ci] + [vp] * 7 + [i64]
             L.vgo_nav_propagate.restype = i64
             L.vgo_nav_propagate.argtypes = [vp] + [vp] * 6 + [ctypes.c_double, vp, vp, i64, ci, ci]
             L._geom_declared = True
@@ -240,6 +242,18 @@ class OracleGeometry:
         if rc != 0:
             raise ValueError("vgo_placed_query rejected placed volume %d / op %d" % (pvol, code))
         return out
+
+    def placed_normal(self, pvol, p):
+        """VPlacedVolume::Normal of placement pvol at points p (3, n) of the mother's frame: (normals, valid)."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = p.shape[1]
+        px, py, pz = (np.ascontiguousarray(p[i]) for i in range(3))
+        nx, ny, nz = (np.empty(n) for _ in range(3))
+        valid = np.empty(n, np.uint8)
+        if lib().vgo_placed_normal(self._h, int(pvol), _np_ptr(px), _np_ptr(py), _np_ptr(pz), _np_ptr(nx),
+                                   _np_ptr(ny), _np_ptr(nz), _np_ptr(valid), n) != 0:
+            raise ValueError("vgo_placed_normal rejected placed volume %d" % pvol)
+        return np.stack([nx, ny, nz]), valid
 
     def propagate(self, p, d, limit, nav, last, nsteps, threads=1):
         """In-place transport loop; returns (p, nav, last, steps_taken)."""

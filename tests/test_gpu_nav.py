@@ -226,6 +226,36 @@ def test_placed_queries(geo, prec):
         assert (ins_loc == ins_mas).mean() > 0.999, pid  # equal up to the rounding of the transform
 
 
+def test_placed_normals(geo):
+    """VPlacedVolume::Normal (source/PlacedVolume.cpp:126-137) against the oracle's, bit for bit, at points
+    near and on the placed volume's surface given in its mother's frame; and the normal is the rotated
+    normal of the unplaced shape at the local point (pins the frame handling)."""
+    from oracle.pyoracle import OracleShape, SHAPE_TYPES
+    g, G, O = geo
+    names = {v: k for k, v in SHAPE_TYPES.items()}
+    for pid in _placed_cases(g):
+        loc, dl, mas, _ = _placed_points(g, pid, 1 << 13, seed=70 + pid % 89)
+        lv = g.world if pid == len(g.placements) else g.placements[pid][1]
+        st, params = g.shapes[g.logicals[lv]]
+        un = OracleShape(names[st], params)
+        inside = un.query("Contains", loc).astype(bool)
+        t_out = un.query("DistanceToOut", loc[:, inside], dl[:, inside])
+        ok = t_out < 1e290
+        exits = loc[:, inside][:, ok] + t_out[ok] * dl[:, inside][:, ok]
+        t = np.array(geometry.IDENTITY if pid == len(g.placements) else g.placements[pid][2])
+        R = t[3:].reshape(3, 3)
+        loc_all = np.concatenate([loc, exits], axis=1)
+        mas_all = np.ascontiguousarray(np.concatenate([mas, t[:3, None] + R @ exits], axis=1))
+        print("placed volume", pid, "points", mas_all.shape[1], flush=True)
+        got_n, got_v = G.PlacedNormal(pid, SOA3D.from_array(mas_all))
+        got_n, got_v = got_n.cpu(), got_v.cpu().numpy()
+        ref_n, ref_v = O.placed_normal(pid, mas_all)
+        assert np.array_equal(got_n, ref_n) and np.array_equal(got_v, ref_v), pid
+        ln, lvld = un.normal(loc_all, init=np.zeros_like(loc_all))
+        close = np.all(np.abs(R @ ln - ref_n) < 1e-9, axis=0) & (lvld == ref_v)
+        assert close.mean() > 0.999, (pid, close.mean())
+
+
 @PRECISIONS
 def test_level_locate(geo, prec):
     """BVHLevelLocator::LevelLocate and LevelLocateExclVol per point against the oracle: the daughter

@@ -11,6 +11,7 @@
 // FP64 and runs the mixed-precision logic of a single-shape query on the local point.
 #include "dev_mixed.h"
 #include "dev_nav.h"
+#include "dev_normal.h"
 #include "geom_internal.h"
 #include "vgmi_internal.h"
 
@@ -100,6 +101,38 @@ static hipError_t launch_placed(Data const &s, int op, Placement const &T, Place
   return hipGetLastError();
 }
 
+struct PlacedNormalArgs {
+  const double *px, *py, *pz;
+  double *nx, *ny, *nz;
+  uint8_t *valid;
+  size_t n;
+};
+
+// VPlacedVolume::Normal (source/PlacedVolume.cpp:126-137): Transform, the unplaced Normal into a zero
+// vector, InverseTransformDirection (Transformation3D.h:808-823 with transform_direction).
+template <class Data>
+__global__ __launch_bounds__(kPlacedBlock) void k_placed_normal(Data s, Placement T, PlacedNormalArgs a)
+{
+  const size_t stride = size_t(gridDim.x) * kPlacedBlock;
+  for (size_t i = size_t(blockIdx.x) * kPlacedBlock + threadIdx.x; i < a.n; i += stride) {
+    const Vec3<double> lp = nav::to_local(T.t, T.r, Vec3<double>(a.px[i], a.py[i], a.pz[i]));
+    Vec3<double> ln(0., 0., 0.);
+    bool set         = false;
+    const bool valid = nrm::Normal(s, lp, ln, set);
+    double x = ln.x * T.r[0], y = ln.x * T.r[3], z = ln.x * T.r[6];
+    x += ln.y * T.r[1];
+    x += ln.z * T.r[2];
+    y += ln.y * T.r[4];
+    y += ln.z * T.r[5];
+    z += ln.y * T.r[7];
+    z += ln.z * T.r[8];
+    a.nx[i]    = x;
+    a.ny[i]    = y;
+    a.nz[i]    = z;
+    a.valid[i] = uint8_t(valid);
+  }
+}
+
 } // namespace vgmi
 
 using namespace vgmi;
@@ -143,6 +176,43 @@ vgmi_status vgmi_placed_query(const vgmi_geom *geom, int placed_id, int op, int 
     return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_placed_query: shape type %d has no kernel", sr.type);
   }
   if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_placed_query launch: %s", hipGetErrorString(e));
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_placed_normal(const vgmi_geom *geom, int placed_id, const double *px, const double *py,
+                               const double *pz, double *nx, double *ny, double *nz, uint8_t *valid, size_t n,
+                               void *stream)
+{
+  if (!geom) return set_error(VGMI_ERR_INVALID, "vgmi_placed_normal: null geometry");
+  if (!geom->device_block) return set_error(VGMI_ERR_INVALID, "vgmi_placed_normal: geometry was built host-only");
+  const int npv = int(geom->host.pvols.size());
+  if (placed_id < 0 || placed_id >= npv)
+    return set_error(VGMI_ERR_INVALID, "vgmi_placed_normal: placed volume %d not in [0, %d)", placed_id, npv);
+  if (n == 0) return VGMI_OK;
+  if (!px || !py || !pz || !nx || !ny || !nz || !valid)
+    return set_error(VGMI_ERR_INVALID, "vgmi_placed_normal: null point/normal/valid pointer");
+  PVolRec const &pv     = geom->host.pvols[placed_id];
+  ShapeRecord const &sr = geom->devShapes[geom->host.lvs[pv.lv].shape];
+  Placement T;
+  for (int k = 0; k < 3; ++k) T.t[k] = pv.t[k];
+  for (int k = 0; k < 9; ++k) T.r[k] = pv.r[k];
+  const PlacedNormalArgs a{px, py, pz, nx, ny, nz, valid, n};
+  size_t blocks = (n + kPlacedBlock - 1) / kPlacedBlock;
+  blocks        = blocks < size_t(256) * 64 ? blocks : size_t(256) * 64;
+  const dim3 g(static_cast<unsigned>(blocks)), b(kPlacedBlock);
+  hipStream_t st = (hipStream_t)stream;
+  switch (sr.type) {
+#define X(E, TT, M, NS)                                                                                              \
+  case E:                                                                                                            \
+    hipLaunchKernelGGL((k_placed_normal<TT>), g, b, 0, st, sr.u.M, T, a);                                             \
+    break;
+    VGMI_FOR_EACH_SHAPE(X)
+#undef X
+  default:
+    return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_placed_normal: shape type %d has no kernel", sr.type);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_placed_normal launch: %s", hipGetErrorString(e));
   return VGMI_OK;
 }
 

@@ -43,6 +43,7 @@ def _declare():
         "vgmi_nav_propagate": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
         "vgmi_nav_level_locate": [vp, ci] + [vp] * 9 + [sz, vp],
         "vgmi_placed_query": [vp, ci, ci, ci] + [vp] * 8 + [sz, vp],
+        "vgmi_placed_normal": [vp, ci] + [vp] * 7 + [sz, vp],
         "vgmi_nav_propagate_noalloc": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
         "vgmi_geom_set_regroup": [vp, ci],
         "vgmi_geom_set_lanes_per_wave": [vp, ci],
@@ -171,6 +172,16 @@ class Geometry:
                                       step_max.data_ptr() if step_max is not None else None, out.data_ptr(), n,
                                       _stream()), "vgmi_placed_query(%s)" % op)
         return out
+
+    def PlacedNormal(self, placed_id, points):
+        """VPlacedVolume::Normal of placement placed_id at points of the frame it is placed in: returns
+        (normals in that frame as an SOA3D, valid uint8 tensor)."""
+        n, dev = len(points), points.x.device
+        out = SOA3D.empty(n, device=dev)
+        valid = torch.empty(n, dtype=torch.uint8, device=dev)
+        check(lib().vgmi_placed_normal(self._h, int(placed_id), *points.ptrs(), *out.ptrs(), valid.data_ptr(), n,
+                                       _stream()), "vgmi_placed_normal")
+        return out, valid
 
     def ComputeStep(self, points, dirs, step_limit, nav_in, last_in=None, relocate=True, precision=FP64):
         """Returns (step, nav_out, last_out, boundary) device tensors."""
