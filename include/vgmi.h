@@ -296,6 +296,40 @@ VGMI_API vgmi_status vgmi_nav_step_safety(const vgmi_geom *geom, int precision, 
  * widened FP32 copies as conservative filters in front of the same FP64 decisions (results are the
  * FP64 results bit for bit). */
 
+/* ---- NavStateIndex <-> path interop (callers that keep NavStatePath-style states: one placed
+ * volume per level, VecGeom/navigation/NavStatePath.h:104-400). Per-level arrays are level-major:
+ * entry k of state i is at [k * n + i]. No precision argument: these walk the NavIndex table and
+ * (global_to_local) apply its FP64 cached matrices. */
+#define VGMI_PATH_INDICES 0 /* daughter indices below the world (NavStateIndex::GetPathAsListOfIndices) */
+#define VGMI_PATH_PLACED 1  /* placed-volume ids, world first (NavStatePath::ValueAt per level) */
+/* levels[i] = GetCurrentLevel() (NavStateIndex.h:315-317; 0 for the outside state); pvols[k] = ValueAt(k)
+ * (NavStateIndex.h:341-349); indices[k] = GetPathAsListOfIndices (source/NavStateIndex.cpp:40-53: 0 at
+ * level 0, then each level's position in its mother's daughter list); -1 beyond the state's levels.
+ * pvols and indices may be NULL. max_levels must be at least the geometry's depth (VGMI_ERR_CAPACITY). */
+VGMI_API vgmi_status vgmi_nav_to_path(const vgmi_geom *geom, const uint32_t *nav, size_t n, int max_levels,
+                                      int32_t *levels, int32_t *pvols, int32_t *indices, void *stream);
+/* The NavIndex of a path: kind VGMI_PATH_INDICES is NavStateIndex::ResetPathFromListOfIndices
+ * (source/NavStateIndex.cpp:77-88: the world, then daughter path[k] of each level; the first entry is
+ * not read); kind VGMI_PATH_PLACED pushes placed volumes level by level (NavStateIndex::Push,
+ * NavStateIndex.h:282-287), the first being the world. levels[i] entries of path are read (at most
+ * max_levels). A path the geometry does not hold (an index beyond a volume's daughters, a placed volume
+ * that is not a daughter of the previous level's volume, a first placed volume other than the world)
+ * and levels[i] == 0 give the outside state 0; the reference does not check these. */
+VGMI_API vgmi_status vgmi_nav_from_path(const vgmi_geom *geom, int kind, const int32_t *levels, const int32_t *path,
+                                        size_t n, int max_levels, uint32_t *nav_out, void *stream);
+/* NavStateIndex::Distance (source/NavStateIndex.cpp:150-166): moves up from nav_a[i] to the deepest
+ * level both paths share, plus moves down to nav_b[i]. The outside state counts as having no level
+ * (the reference reads an undefined level for it). */
+VGMI_API vgmi_status vgmi_nav_distance(const vgmi_geom *geom, const uint32_t *nav_a, const uint32_t *nav_b,
+                                       size_t n, int32_t *out, void *stream);
+/* NavStateIndex::GlobalToLocal(point, tolevel) (NavStateIndex.h:262-269, 368-377): the global point in
+ * the frame of the state's ancestor at level tolevel (tolevel < 0: of the state's top volume), through
+ * that ancestor's cached matrix. A level deeper than the state's own reaches the outside state (identity), as
+ * the reference's GetNavIndexImpl does. */
+VGMI_API vgmi_status vgmi_nav_global_to_local(const vgmi_geom *geom, const uint32_t *nav, int tolevel,
+                                              const double *px, const double *py, const double *pz, double *lx,
+                                              double *ly, double *lz, size_t n, void *stream);
+
 /* Isotropic safety: VSafetyEstimatorHelper<BVHSafetyEstimator>::ComputeSafety
  * (navigation/VSafetyEstimator.h:98-107, navigation/BVHSafetyEstimator.h:38-46). */
 VGMI_API vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,
@@ -362,7 +396,8 @@ VGMI_API vgmi_status vgmi_geom_set_lanes_per_wave(vgmi_geom *geom, int lanes);
  * buffers) are in flight at once. Admission: a request with more than
  * lane_tracks tracks, or one that would put more than queue_tracks tracks in
  * the queue, is refused with VGMI_ERR_CAPACITY; creation fails with
- * VGMI_ERR_CAPACITY when the lanes would take more than half of the free HBM.
+ * VGMI_ERR_CAPACITY when the lanes would take more than half of the free HBM;
+ * vgmi_server_set_hbm_reserve adds a live free-HBM check to every submission and batch.
  * precision is VGMI_FP64 or VGMI_MIXED (the navigation kernels' mixed mode: FP32 box filters,
  * results identical to FP64).
  * ------------------------------------------------------------------------ */
@@ -379,6 +414,18 @@ VGMI_API vgmi_status vgmi_server_submit(vgmi_server *server, const double *px, c
  * Each ticket is waited for exactly once. Returns VGMI_ERR_CAPACITY for a request the server was
  * destroyed before launching. */
 VGMI_API vgmi_status vgmi_server_wait(vgmi_server *server, uint64_t ticket, double *latency_us);
+/* Non-blocking completion test: *done = 1 once the request has finished (vgmi_server_wait then
+ * returns at once), else 0. The ticket stays valid until it is waited for. */
+VGMI_API vgmi_status vgmi_server_poll(vgmi_server *server, uint64_t ticket, int *done);
+/* Cancels a request that is still queued: it finishes at once and vgmi_server_wait returns
+ * VGMI_ERR_CAPACITY for it (the ticket must still be waited for). A request already on the GPU
+ * or done is not cancelled (VGMI_ERR_UNSUPPORTED). */
+VGMI_API vgmi_status vgmi_server_cancel(vgmi_server *server, uint64_t ticket);
+/* Admission by free HBM: while the runtime reports less than `bytes` of free device memory
+ * (hipMemGetInfo, queried at each submission and before each batch is launched), submissions are
+ * refused with VGMI_ERR_CAPACITY, and queued requests whose batch comes up are finished with
+ * VGMI_ERR_CAPACITY instead of being launched. 0 (the default) turns the check off. */
+VGMI_API vgmi_status vgmi_server_set_hbm_reserve(vgmi_server *server, int64_t bytes);
 /* stats[0..3] = launches, tracks done, steps done, requests refused */
 VGMI_API vgmi_status vgmi_server_stats(vgmi_server *server, int64_t stats[4]);
 /* Launches whose tracks need at most 100 steps replay HIP graphs captured at creation (one per

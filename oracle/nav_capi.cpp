@@ -259,4 +259,127 @@ int vgo_placed_normal(void *gp, int pvol, const double *px, const double *py, co
   return 0;
 }
 
+// ---- NavStateIndex <-> path interop. Level-major arrays: entry k of state i at [k * n + i].
+
+// NavStateIndex::GetLevel / ValueAt / GetPathAsListOfIndices (VecGeom/navigation/NavStateIndex.h:194-211,
+// 341-349; source/NavStateIndex.cpp:40-53): levels[i] = GetCurrentLevel (0 for the outside state),
+// pvols[k] = the placed volume at level k, indices[k] = its position in its mother's daughter list
+// (0 at level 0); -1 beyond the state's levels. Returns -1 when a state has more than maxl levels.
+int vgo_nav_to_path(void *gp, const uint32_t *nav, int64_t n, int maxl, int32_t *levels, int32_t *pvols,
+                    int32_t *indices)
+{
+  Geometry *g = (Geometry *)gp;
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t s   = nav[i];
+    const int L  = g->Level(s);
+    if (L + 1 > maxl) return -1;
+    levels[i] = L + 1;
+    for (int k = L + 1; k < maxl; ++k) {
+      if (pvols) pvols[k * n + i] = -1;
+      if (indices) indices[k * n + i] = -1;
+    }
+    for (int k = L; k >= 0; --k) {
+      const int pv = g->Top(s);
+      if (pvols) pvols[k * n + i] = pv;
+      if (indices) indices[k * n + i] = k > 0 ? g->pvols[pv].childId : 0;
+      s = g->Pop(s);
+    }
+  }
+  return 0;
+}
+
+// kind 0: NavStateIndex::ResetPathFromListOfIndices (source/NavStateIndex.cpp:77-88): the world, then
+// daughter `indices[k]` of the volume reached at each further level (the first entry is not read).
+// kind 1: a NavStatePath's placed volumes, world first, pushed level by level (NavStateIndex::Push,
+// NavStateIndex.h:282-287). A path the geometry does not hold (a daughter index beyond the volume's
+// daughters, a placed volume that is not a daughter of the previous level's volume, a first entry
+// other than the world) gives the outside state 0; so does levels[i] == 0.
+int vgo_nav_from_path(void *gp, int kind, const int32_t *levels, const int32_t *path, int64_t n, int maxl,
+                      uint32_t *nav)
+{
+  Geometry *g = (Geometry *)gp;
+  const int np = int(g->pvols.size());
+  for (int64_t i = 0; i < n; ++i) {
+    const int L = levels[i];
+    uint32_t s  = 0;
+    bool ok     = L >= 1 && L <= maxl;
+    for (int k = 0; ok && k < L; ++k) {
+      const int v = path[k * n + i];
+      if (k == 0) {
+        ok = kind == 0 || v == g->worldPvol;
+        s  = 1;
+        continue;
+      }
+      int cid = v;
+      if (kind == 1) {
+        if (v < 0 || v >= np) {
+          ok = false;
+          break;
+        }
+        cid = g->pvols[v].childId;
+      }
+      if (cid < 0 || cid >= g->NDaughters(s)) {
+        ok = false;
+        break;
+      }
+      const uint32_t c = g->NavInd(s + 3 + uint32_t(cid));
+      if (kind == 1 && g->Top(c) != v) {
+        ok = false;
+        break;
+      }
+      s = c;
+    }
+    nav[i] = ok ? s : 0u;
+  }
+  return 0;
+}
+
+// NavStateIndex::Distance (source/NavStateIndex.cpp:150-166): up moves from a to the deepest common
+// level plus down moves to b. The outside state counts as having no level (-1).
+int vgo_nav_distance(void *gp, const uint32_t *a, const uint32_t *b, int64_t n, int32_t *out)
+{
+  Geometry *g = (Geometry *)gp;
+  auto at = [&](uint32_t s, int level) { // NavStateIndex::At (NavStateIndex.h:331-337)
+    int up = g->Level(s) - level;
+    while (s && up--) s = g->Pop(s);
+    return s > 0 ? g->Top(s) : -1;
+  };
+  for (int64_t i = 0; i < n; ++i) {
+    const int la = g->Level(a[i]), lb = g->Level(b[i]);
+    const int maxlevel = std::min(la, lb);
+    int common = -1;
+    for (int k = 0; k < maxlevel + 1; ++k) {
+      if (at(a[i], k) == at(b[i], k))
+        common = k;
+      else
+        break;
+    }
+    out[i] = (la - common) + (lb - common);
+  }
+  return 0;
+}
+
+// NavStateIndex::GlobalToLocal(point, tolevel) (NavStateIndex.h:262-269, 368-377): the point in the
+// frame of the state's ancestor at `tolevel` (tolevel < 0: the top volume), through the cached matrix
+// of that ancestor. An ancestor level beyond the state's own level reaches the outside state, whose
+// matrix is the identity (the reference's GetNavIndexImpl walks up past the world then).
+int vgo_nav_global_to_local(void *gp, const uint32_t *nav, int tolevel, const double *px, const double *py,
+                            const double *pz, double *lx, double *ly, double *lz, int64_t n)
+{
+  Geometry *g = (Geometry *)gp;
+  for (int64_t i = 0; i < n; ++i) {
+    uint32_t s = nav[i];
+    if (tolevel >= 0) {
+      int up = g->Level(s) - tolevel;
+      if (up < 0) s = 0;
+      while (s && up-- > 0) s = g->Pop(s);
+    }
+    Transform m;
+    g->TopMatrix(s, m);
+    const V3 l = m.Transform3(V3(px[i], py[i], pz[i]));
+    lx[i] = l.x, ly[i] = l.y, lz[i] = l.z;
+  }
+  return 0;
+}
+
 } // extern "C"

@@ -135,6 +135,14 @@ class OracleGeometry:
             L.vgo_placed_query.argtypes = [vp, ci, ci] + [vp] * 8 + [i64, ci]
             L.vgo_placed_normal.restype = ci
             L.vgo_placed_normal.argtypes = [vp, ci] + [vp] * 7 + [i64]
+            L.vgo_nav_to_path.restype = ci
+            L.vgo_nav_to_path.argtypes = [vp, vp, i64, ci, vp, vp, vp]
+            L.vgo_nav_from_path.restype = ci
+            L.vgo_nav_from_path.argtypes = [vp, ci, vp, vp, i64, ci, vp]
+            L.vgo_nav_distance.restype = ci
+            L.vgo_nav_distance.argtypes = [vp, vp, vp, i64, vp]
+            L.vgo_nav_global_to_local.restype = ci
+            L.vgo_nav_global_to_local.argtypes = [vp, vp, ci] + [vp] * 6 + [i64]
             L.vgo_nav_propagate.restype = i64
             L.vgo_nav_propagate.argtypes = [vp] + [vp] * 6 + [ctypes.c_double, vp, vp, i64, ci, ci]
             L._geom_declared = True
@@ -254,6 +262,44 @@ class OracleGeometry:
                                    _np_ptr(ny), _np_ptr(nz), _np_ptr(valid), n) != 0:
             raise ValueError("vgo_placed_normal rejected placed volume %d" % pvol)
         return np.stack([nx, ny, nz]), valid
+
+    def nav_to_path(self, nav, max_levels):
+        """(levels, placed volumes (max_levels, n), daughter indices (max_levels, n)) per state."""
+        nav = np.ascontiguousarray(nav, dtype=np.uint32)
+        n = nav.shape[0]
+        levels = np.empty(n, np.int32)
+        pv = np.empty((max_levels, n), np.int32)
+        ix = np.empty((max_levels, n), np.int32)
+        if lib().vgo_nav_to_path(self._h, _np_ptr(nav), n, int(max_levels), _np_ptr(levels), _np_ptr(pv),
+                                 _np_ptr(ix)) != 0:
+            raise ValueError("a state has more than max_levels levels")
+        return levels, pv, ix
+
+    def nav_from_path(self, levels, path, kind):
+        """NavIndex per path; kind 0: daughter indices, 1: placed volumes (world first)."""
+        levels = np.ascontiguousarray(levels, dtype=np.int32)
+        path = np.ascontiguousarray(path, dtype=np.int32)
+        n = levels.shape[0]
+        out = np.empty(n, np.uint32)
+        lib().vgo_nav_from_path(self._h, int(kind), _np_ptr(levels), _np_ptr(path), n, path.shape[0], _np_ptr(out))
+        return out
+
+    def nav_distance(self, a, b):
+        a = np.ascontiguousarray(a, dtype=np.uint32)
+        b = np.ascontiguousarray(b, dtype=np.uint32)
+        out = np.empty(a.shape[0], np.int32)
+        lib().vgo_nav_distance(self._h, _np_ptr(a), _np_ptr(b), a.shape[0], _np_ptr(out))
+        return out
+
+    def global_to_local(self, nav, p, tolevel=-1):
+        nav = np.ascontiguousarray(nav, dtype=np.uint32)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = nav.shape[0]
+        comps = [np.ascontiguousarray(p[i]) for i in range(3)]
+        out = [np.empty(n) for _ in range(3)]
+        lib().vgo_nav_global_to_local(self._h, _np_ptr(nav), int(tolevel), *[_np_ptr(c) for c in comps],
+                                      *[_np_ptr(c) for c in out], n)
+        return np.stack(out)
 
     def propagate(self, p, d, limit, nav, last, nsteps, threads=1):
         """In-place transport loop; returns (p, nav, last, steps_taken)."""

@@ -100,6 +100,10 @@ def test_server_rejects_bad_arguments_without_gpu():
     t = C.c_uint64()
     assert L.vgmi_server_submit(None, *([None] * 7), 1, *([None] * 5), C.byref(t)) == -1
     assert L.vgmi_server_wait(None, 1, None) == -1
+    done = C.c_int()
+    assert L.vgmi_server_poll(None, 1, C.byref(done)) == -1
+    assert L.vgmi_server_cancel(None, 1) == -1
+    assert L.vgmi_server_set_hbm_reserve(None, 0) == -1
     assert L.vgmi_server_destroy(None) == 0
 
 

@@ -7,6 +7,7 @@ expected bit-identical; NavStateIndex values and step counts exact); the mixed-p
 server returns the FP64 server's results.
 """
 import threading
+import time
 
 import numpy as np
 import pytest
@@ -210,3 +211,59 @@ def test_destroy_with_queued_requests_and_a_blocked_waiter(geo):
     t.join(timeout=60)
     assert not t.is_alive()
     assert outcome and (outcome[0] == "done" or "never launched" in outcome[0]), outcome
+
+
+def test_poll_and_cancel(geo):
+    """vgmi_server_poll never blocks and turns true once a request is done; a request still in
+    the queue can be cancelled (its wait then reports the cancellation), one that is done cannot;
+    the other requests' results are unaffected."""
+    g, G = geo
+    reqs = _requests(g, [30000] * 5, seed=41)
+    with Server(G, lane_tracks=1 << 15, lanes=1) as S:
+        tickets = [S.submit(*r) for r in reqs]
+        cancelled = []
+        for t in tickets[::-1]:
+            try:
+                S.cancel(t)
+                cancelled.append(t)
+            except VgmiError as e:
+                assert "on the GPU" in str(e) or "already done" in str(e), str(e)
+        assert cancelled, "with one lane the last requests are still queued"
+        for t in tickets:
+            if t in cancelled:
+                assert S.poll(t)
+                with pytest.raises(VgmiError, match="cancelled"):
+                    S.wait(t)
+            else:
+                S.wait(t)
+        kept = [(r, t) for r, t in zip(reqs, tickets) if t not in cancelled]
+        assert kept
+        for r, t in kept:
+            alone = S.wait(S.submit(*r))
+            assert np.array_equal(alone.nav, t.nav) and np.array_equal(alone.p_out, t.p_out)
+        t = S.submit(*reqs[0])
+        while not S.poll(t):
+            time.sleep(1e-4)
+        S.wait(t)
+        with pytest.raises(VgmiError, match="unknown ticket"):
+            S.poll(t)
+
+
+def test_admission_by_free_hbm(geo):
+    """With a free-HBM reserve above what the GPU has, submissions are refused; a reserve the GPU
+    meets admits them again and the results are those of an unrestricted server."""
+    g, G = geo
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    (p, d, nsteps), = _requests(g, [4096], seed=43)
+    with Server(G, lane_tracks=1 << 15, lanes=2) as S:
+        ref = S.wait(S.submit(p, d, nsteps))
+        S.set_hbm_reserve(total + (1 << 30))
+        before = S.stats()["refused"]
+        with pytest.raises(VgmiError, match="free HBM"):
+            S.submit(p, d, nsteps)
+        assert S.stats()["refused"] == before + 1
+        S.set_hbm_reserve(1 << 20)
+        r = S.wait(S.submit(p, d, nsteps))
+        assert np.array_equal(r.nav, ref.nav) and np.array_equal(r.p_out, ref.p_out)
+        S.set_hbm_reserve(0)

@@ -145,11 +145,26 @@ __global__ __launch_bounds__(kNavBlock) __attribute__((amdgpu_waves_per_eu(kLoop
 {
   VGMI_NAV_STACK;
   const unsigned lane = threadIdx.x & 63u;
-  if (lane >= unsigned(a.lanes)) return;
-  size_t i = ((size_t(blockIdx.x) * kNavBlock + threadIdx.x) >> 6) * unsigned(a.lanes) + lane;
-  const size_t n = a.dn ? size_t(*a.dn) : a.n;
+  const size_t wave   = (size_t(blockIdx.x) * kNavBlock + threadIdx.x) >> 6;
+  const size_t n      = a.dn ? size_t(*a.dn) : a.n;
+  unsigned lanes      = unsigned(a.lanes);
+  size_t first        = 0;
   if (a.perm) {
-    i += *a.first;
+    // Regrouped chunk: only the tracks still running are in perm[first, n). They are spread over
+    // every launched wavefront, fewest per wavefront (a power of two) that covers them, so the
+    // tail of a loop whose tracks finish early runs with few tracks per wavefront. The grid holds
+    // ceil(n / a.lanes) wavefronts, so a.lanes always covers them.
+    first               = *a.first;
+    const size_t active = n > first ? n - first : 0;
+    const size_t waves  = (size_t(gridDim.x) * kNavBlock) >> 6;
+    unsigned l          = 1;
+    while (l < lanes && size_t(l) * waves < active) l <<= 1;
+    lanes = l;
+  }
+  if (lane >= lanes) return;
+  size_t i = wave * lanes + lane;
+  if (a.perm) {
+    i += first;
     if (i >= n) return;
     i = a.perm[i];
   } else if (i >= n) {
@@ -290,6 +305,118 @@ __global__ __launch_bounds__(kNavBlock) void k_locate_init(GeomView g, const dou
   nav[i]  = locate_global<TM>(g, p, stk);
   last[i] = 0u;
   taken[i] = 0;
+}
+
+// ---- NavStateIndex <-> path interop (one lane per state; level-major arrays, entry k of state i
+// at [k * n + i], so every level's entries are written coalesced).
+
+// NavStateIndex::GetCurrentLevel / ValueAt / GetPathAsListOfIndices
+// (VecGeom/navigation/NavStateIndex.h:194-211, 341-349; source/NavStateIndex.cpp:40-53).
+__global__ __launch_bounds__(256) void k_nav_to_path(GeomView g, const uint32_t *__restrict__ nav, size_t n,
+                                                     int maxl, int32_t *levels, int32_t *pvols, int32_t *indices)
+{
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s  = nav[i];
+  const int L = s ? int(g.navind[s + 2] & 0xffu) : -1;
+  levels[i]   = L + 1;
+  for (int k = L + 1; k < maxl; ++k) {
+    if (pvols) pvols[size_t(k) * n + i] = -1;
+    if (indices) indices[size_t(k) * n + i] = -1;
+  }
+  for (int k = L; k >= 0; --k) {
+    const int pv = int(g.navind[s + 1]);
+    if (pvols) pvols[size_t(k) * n + i] = pv;
+    if (indices) indices[size_t(k) * n + i] = k > 0 ? g.vols[pv].childId : 0;
+    s = g.navind[s];
+  }
+}
+
+// kind 0: ResetPathFromListOfIndices (source/NavStateIndex.cpp:77-88), daughter indices below the
+// world (the first entry is not read); kind 1: placed volumes world first (NavStatePath levels pushed
+// one by one, NavStateIndex.h:282-287). A path the geometry does not hold gives state 0.
+__global__ __launch_bounds__(256) void k_nav_from_path(GeomView g, int kind, const int32_t *__restrict__ levels,
+                                                       const int32_t *__restrict__ path, size_t n, int maxl,
+                                                       int npvol, uint32_t *nav)
+{
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int L = levels[i];
+  uint32_t s  = 0;
+  bool ok     = L >= 1 && L <= maxl;
+  for (int k = 0; ok && k < L; ++k) {
+    const int v = path[size_t(k) * n + i];
+    if (k == 0) {
+      ok = kind == 0 || v == g.worldPvol;
+      s  = 1;
+      continue;
+    }
+    int cid = v;
+    if (kind == 1) {
+      if (v < 0 || v >= npvol) {
+        ok = false;
+        break;
+      }
+      cid = g.vols[v].childId;
+    }
+    if (cid < 0 || uint32_t(cid) >= (g.navind[s + 2] >> 16)) {
+      ok = false;
+      break;
+    }
+    const uint32_t c = g.navind[s + 3 + uint32_t(cid)];
+    if (kind == 1 && int(g.navind[c + 1]) != v) {
+      ok = false;
+      break;
+    }
+    s = c;
+  }
+  nav[i] = ok ? s : 0u;
+}
+
+// NavStateIndex::Distance (source/NavStateIndex.cpp:150-166). The reference compares the placed
+// volumes of both paths from the top and stops at the first difference; equal placed volumes at
+// every level down to k is the same as equal states at level k, so both states are lifted to the
+// shallower level and popped together until they meet.
+__global__ __launch_bounds__(256) void k_nav_distance(GeomView g, const uint32_t *__restrict__ a,
+                                                      const uint32_t *__restrict__ b, size_t n, int32_t *out)
+{
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t sa = a[i], sb = b[i];
+  const int la = sa ? int(g.navind[sa + 2] & 0xffu) : -1;
+  const int lb = sb ? int(g.navind[sb + 2] & 0xffu) : -1;
+  int m        = la < lb ? la : lb;
+  for (int k = la; k > m; --k) sa = g.navind[sa];
+  for (int k = lb; k > m; --k) sb = g.navind[sb];
+  while (m >= 0 && sa != sb) {
+    sa = g.navind[sa];
+    sb = g.navind[sb];
+    --m;
+  }
+  out[i] = (la - m) + (lb - m);
+}
+
+// NavStateIndex::GlobalToLocal(point[, tolevel]) (NavStateIndex.h:262-269, 368-377).
+__global__ __launch_bounds__(256) void k_nav_global_to_local(GeomView g, const uint32_t *__restrict__ nav,
+                                                             int tolevel, const double *__restrict__ px,
+                                                             const double *__restrict__ py,
+                                                             const double *__restrict__ pz, double *lx, double *ly,
+                                                             double *lz, size_t n)
+{
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t s = nav[i];
+  if (tolevel >= 0 && s) {
+    int up = int(g.navind[s + 2] & 0xffu) - tolevel;
+    if (up < 0) s = 0;
+    for (; s && up > 0; --up) s = g.navind[s];
+  }
+  Mat m;
+  top_matrix(g, s, m);
+  const V3 l = to_local(m.t, m.r, V3(px[i], py[i], pz[i]));
+  lx[i]      = l.x;
+  ly[i]      = l.y;
+  lz[i]      = l.z;
 }
 
 } // namespace nav
@@ -468,6 +595,59 @@ vgmi_status vgmi_nav_step_safety(const vgmi_geom *geom, int precision, const dou
              1, calc_safety, safety_out};
   VGMI_NAV_LAUNCH(k_step, geom, precision, n, stream, geom->view, a);
   return launched("vgmi_nav_step_safety");
+}
+
+static unsigned grid256(size_t n) { return unsigned((n + 255) / 256); }
+
+vgmi_status vgmi_nav_to_path(const vgmi_geom *geom, const uint32_t *nav, size_t n, int max_levels, int32_t *levels,
+                             int32_t *pvols, int32_t *indices, void *stream)
+{
+  if (vgmi_status s = check_geom(geom, VGMI_FP64, "vgmi_nav_to_path")) return s;
+  if (n == 0) return VGMI_OK;
+  if (!nav || !levels) return set_error(VGMI_ERR_INVALID, "vgmi_nav_to_path: null pointer");
+  if (max_levels < geom->host.maxDepth)
+    return set_error(VGMI_ERR_CAPACITY, "vgmi_nav_to_path: max_levels %d below the geometry's depth %d", max_levels,
+                     geom->host.maxDepth);
+  hipLaunchKernelGGL(k_nav_to_path, dim3(grid256(n)), dim3(256), 0, (hipStream_t)stream, geom->view, nav, n,
+                     max_levels, levels, pvols, indices);
+  return launched("vgmi_nav_to_path");
+}
+
+vgmi_status vgmi_nav_from_path(const vgmi_geom *geom, int kind, const int32_t *levels, const int32_t *path,
+                               size_t n, int max_levels, uint32_t *nav_out, void *stream)
+{
+  if (vgmi_status s = check_geom(geom, VGMI_FP64, "vgmi_nav_from_path")) return s;
+  if (kind != VGMI_PATH_INDICES && kind != VGMI_PATH_PLACED)
+    return set_error(VGMI_ERR_INVALID, "vgmi_nav_from_path: unknown path kind %d", kind);
+  if (n == 0) return VGMI_OK;
+  if (!levels || !path || !nav_out || max_levels < 1) return set_error(VGMI_ERR_INVALID, "vgmi_nav_from_path: null pointer or no levels");
+  hipLaunchKernelGGL(k_nav_from_path, dim3(grid256(n)), dim3(256), 0, (hipStream_t)stream, geom->view, kind, levels,
+                     path, n, max_levels, int(geom->host.pvols.size()), nav_out);
+  return launched("vgmi_nav_from_path");
+}
+
+vgmi_status vgmi_nav_distance(const vgmi_geom *geom, const uint32_t *nav_a, const uint32_t *nav_b, size_t n,
+                              int32_t *out, void *stream)
+{
+  if (vgmi_status s = check_geom(geom, VGMI_FP64, "vgmi_nav_distance")) return s;
+  if (n == 0) return VGMI_OK;
+  if (!nav_a || !nav_b || !out) return set_error(VGMI_ERR_INVALID, "vgmi_nav_distance: null pointer");
+  hipLaunchKernelGGL(k_nav_distance, dim3(grid256(n)), dim3(256), 0, (hipStream_t)stream, geom->view, nav_a, nav_b, n,
+                     out);
+  return launched("vgmi_nav_distance");
+}
+
+vgmi_status vgmi_nav_global_to_local(const vgmi_geom *geom, const uint32_t *nav, int tolevel, const double *px,
+                                     const double *py, const double *pz, double *lx, double *ly, double *lz, size_t n,
+                                     void *stream)
+{
+  if (vgmi_status s = check_geom(geom, VGMI_FP64, "vgmi_nav_global_to_local")) return s;
+  if (n == 0) return VGMI_OK;
+  if (!nav || !px || !py || !pz || !lx || !ly || !lz)
+    return set_error(VGMI_ERR_INVALID, "vgmi_nav_global_to_local: null pointer");
+  hipLaunchKernelGGL(k_nav_global_to_local, dim3(grid256(n)), dim3(256), 0, (hipStream_t)stream, geom->view, nav,
+                     tolevel, px, py, pz, lx, ly, lz, n);
+  return launched("vgmi_nav_global_to_local");
 }
 
 vgmi_status vgmi_nav_safety(const vgmi_geom *geom, int precision, const double *px, const double *py,

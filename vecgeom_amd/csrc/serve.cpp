@@ -8,7 +8,9 @@
 // requests done. Lanes run concurrently, so copies of one launch overlap the kernels of
 // another. Admission: a request larger than a lane is rejected, and the queue holds at
 // most `queue_tracks` tracks (back-pressure); lane sizes are checked against the free HBM
-// reported by the runtime at creation.
+// reported by the runtime at creation, and every submission and every batch against the
+// free HBM reported at that moment (vgmi_server_set_hbm_reserve). Queued requests can be
+// cancelled; completion can be polled without blocking.
 #include "geom_internal.h"
 #include "vgmi_internal.h"
 
@@ -41,7 +43,11 @@ struct Request {
   Clock::time_point tSubmit, tDone;
   int state  = 0; // 0 queued, 1 in flight, 2 done
   int status = 0; // vgmi_status of the launch that carried it
+  int reason = 0; // why a request that never ran was finished (kNot*)
 };
+
+// Reasons a request finishes without running.
+constexpr int kNotDestroyed = 1, kNotCancelled = 2, kNotHbm = 3;
 
 struct Lane {
   hipStream_t stream = nullptr;
@@ -80,11 +86,27 @@ struct vgmi_server {
   uint64_t nextTicket = 1;
   bool stop           = false;
   int waiters         = 0; // threads inside vgmi_server_wait (destroy waits for them to leave)
+  int64_t hbmReserve  = 0; // bytes of HBM that must stay free for a request or batch to be admitted
   std::thread dispatcher, completer;
   int device = 0;
   // statistics
   int64_t launches = 0, tracksDone = 0, stepsDone = 0, rejected = 0, graphLaunches = 0;
 };
+
+namespace {
+// Free HBM on the server's device now (0 when the runtime cannot tell).
+int64_t free_hbm(const vgmi_server *S)
+{
+  int old = -1;
+  size_t freeB = 0, totalB = 0;
+  if (hipGetDevice(&old) != hipSuccess) return 0;
+  const bool other = old != S->device;
+  if (other && hipSetDevice(S->device) != hipSuccess) return 0;
+  const bool ok = hipMemGetInfo(&freeB, &totalB) == hipSuccess;
+  if (other) (void)hipSetDevice(old);
+  return ok ? int64_t(freeB) : 0;
+}
+} // namespace
 
 namespace {
 
@@ -174,6 +196,21 @@ void dispatch_loop(vgmi_server *S)
       L->offset.clear();
       L->ntracks = 0;
       pick_batch(S, *L);
+      // admission of the batch against the free HBM now: a batch that would leave less than the
+      // reserve free is not launched, and its requests finish with VGMI_ERR_CAPACITY
+      if (S->hbmReserve > 0 && free_hbm(S) < S->hbmReserve) {
+        const auto now = Clock::now();
+        for (Request *r : L->batch) {
+          r->state  = 2;
+          r->status = VGMI_ERR_CAPACITY;
+          r->reason = kNotHbm;
+          r->tDone  = now;
+        }
+        S->rejected += int64_t(L->batch.size());
+        L->batch.clear();
+        S->cvDone.notify_all();
+        continue;
+      }
       L->busy   = true;
       L->issued = false;
     }
@@ -389,6 +426,12 @@ vgmi_status vgmi_server_submit(vgmi_server *S, const double *px, const double *p
       delete r;
       return set_error(VGMI_ERR_INVALID, "vgmi_server_submit: the server is being destroyed");
     }
+    if (S->hbmReserve > 0 && free_hbm(S) < S->hbmReserve) {
+      S->rejected++;
+      delete r;
+      return set_error(VGMI_ERR_CAPACITY, "vgmi_server_submit: request of %lld tracks not admitted (free HBM below "
+                       "the reserve of %lld bytes)", (long long)n, (long long)S->hbmReserve);
+    }
     if (n > S->lanes[0].cap || S->queuedTracks + n > S->queueCap) {
       S->rejected++;
       delete r;
@@ -416,15 +459,68 @@ vgmi_status vgmi_server_wait(vgmi_server *S, uint64_t ticket, double *latency_us
   S->cvDone.wait(lk, [&] { return r->state == 2; });
   --S->waiters;
   if (latency_us) *latency_us = std::chrono::duration<double, std::micro>(r->tDone - r->tSubmit).count();
-  const int st = r->status;
+  const int st     = r->status;
+  const int reason = r->reason;
   S->requests.erase(it);
   delete r;
   if (S->stop) S->cvDone.notify_all(); // vgmi_server_destroy waits for the last waiter to leave
   if (st == VGMI_OK) return VGMI_OK;
-  if (st == VGMI_ERR_CAPACITY)
+  if (reason == kNotDestroyed)
     return set_error(st, "vgmi_server_wait: ticket %llu was never launched (the server was destroyed)",
                      (unsigned long long)ticket);
+  if (reason == kNotCancelled)
+    return set_error(st, "vgmi_server_wait: ticket %llu was cancelled before its launch", (unsigned long long)ticket);
+  if (reason == kNotHbm)
+    return set_error(st, "vgmi_server_wait: ticket %llu was not admitted (free HBM below the reserve)",
+                     (unsigned long long)ticket);
   return set_error(st, "vgmi_server_wait: the launch carrying ticket %llu failed", (unsigned long long)ticket);
+}
+
+vgmi_status vgmi_server_poll(vgmi_server *S, uint64_t ticket, int *done)
+{
+  if (!S || !done) return set_error(VGMI_ERR_INVALID, "vgmi_server_poll: null argument");
+  std::lock_guard<std::mutex> lk(S->m);
+  auto it = S->requests.find(ticket);
+  if (it == S->requests.end())
+    return set_error(VGMI_ERR_INVALID, "vgmi_server_poll: unknown ticket %llu", (unsigned long long)ticket);
+  *done = it->second->state == 2 ? 1 : 0;
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_server_cancel(vgmi_server *S, uint64_t ticket)
+{
+  if (!S) return set_error(VGMI_ERR_INVALID, "vgmi_server_cancel: null server");
+  {
+    std::lock_guard<std::mutex> lk(S->m);
+    auto it = S->requests.find(ticket);
+    if (it == S->requests.end())
+      return set_error(VGMI_ERR_INVALID, "vgmi_server_cancel: unknown ticket %llu", (unsigned long long)ticket);
+    Request *r = it->second;
+    if (r->state != 0)
+      return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_server_cancel: ticket %llu is %s", (unsigned long long)ticket,
+                       r->state == 1 ? "on the GPU" : "already done");
+    for (auto q = S->queue.begin(); q != S->queue.end(); ++q)
+      if (*q == r) {
+        S->queue.erase(q);
+        break;
+      }
+    S->queuedTracks -= r->n;
+    r->state  = 2;
+    r->status = VGMI_ERR_CAPACITY;
+    r->reason = kNotCancelled;
+    r->tDone  = Clock::now();
+  }
+  S->cvDone.notify_all();
+  return VGMI_OK;
+}
+
+vgmi_status vgmi_server_set_hbm_reserve(vgmi_server *S, int64_t bytes)
+{
+  if (!S) return set_error(VGMI_ERR_INVALID, "vgmi_server_set_hbm_reserve: null server");
+  if (bytes < 0) return set_error(VGMI_ERR_INVALID, "vgmi_server_set_hbm_reserve: negative reserve");
+  std::lock_guard<std::mutex> lk(S->m);
+  S->hbmReserve = bytes;
+  return VGMI_OK;
 }
 
 vgmi_status vgmi_server_stats(vgmi_server *S, int64_t stats[4])
@@ -466,6 +562,7 @@ vgmi_status vgmi_server_destroy(vgmi_server *S)
       if (kv.second->state != 2) {
         kv.second->state  = 2;
         kv.second->status = VGMI_ERR_CAPACITY;
+        kv.second->reason = kNotDestroyed;
         kv.second->tDone  = now;
       }
     S->queue.clear();

@@ -45,6 +45,10 @@ def _declare():
         "vgmi_placed_query": [vp, ci, ci, ci] + [vp] * 8 + [sz, vp],
         "vgmi_placed_normal": [vp, ci] + [vp] * 7 + [sz, vp],
         "vgmi_nav_propagate_noalloc": [vp, ci] + [vp] * 6 + [ctypes.c_double] + [vp] * 3 + [sz, ci, vp],
+        "vgmi_nav_to_path": [vp, vp, sz, ci, vp, vp, vp, vp],
+        "vgmi_nav_from_path": [vp, ci, vp, vp, sz, ci, vp, vp],
+        "vgmi_nav_distance": [vp, vp, vp, sz, vp, vp],
+        "vgmi_nav_global_to_local": [vp, vp, ci] + [vp] * 6 + [sz, vp],
         "vgmi_geom_set_regroup": [vp, ci],
         "vgmi_geom_set_lanes_per_wave": [vp, ci],
     }
@@ -131,6 +135,58 @@ class Geometry:
         device_array(out, _U32, n, "out")
         check(lib().vgmi_nav_locate(self._h, precision, *points.ptrs(), out.data_ptr(), n, _stream()),
               "vgmi_nav_locate")
+        return out
+
+    # ---- NavStateIndex <-> path interop (NavStatePath-style states: one placed volume per level)
+    PATH_INDICES, PATH_PLACED = 0, 1
+
+    def NavToPath(self, nav, max_levels=None):
+        """Per state: (levels = GetCurrentLevel, placed volumes per level, daughter indices per level), the
+        last two as (max_levels, n) int32 tensors with -1 beyond each state's levels
+        (NavStateIndex::ValueAt, GetPathAsListOfIndices)."""
+        n = len(nav)
+        dev = nav.device
+        device_array(nav, _U32, n, "nav")
+        L = int(max_levels if max_levels is not None else self.info()["max_depth"])
+        levels = torch.empty(n, dtype=torch.int32, device=dev)
+        pvols = torch.empty((L, n), dtype=torch.int32, device=dev)
+        idx = torch.empty((L, n), dtype=torch.int32, device=dev)
+        check(lib().vgmi_nav_to_path(self._h, nav.data_ptr(), n, L, levels.data_ptr(), pvols.data_ptr(),
+                                     idx.data_ptr(), _stream()), "vgmi_nav_to_path")
+        return levels, pvols, idx
+
+    def PathToNav(self, levels, path, kind=PATH_PLACED):
+        """NavIndex per path: path is (max_levels, n) int32, placed volumes world first (kind PATH_PLACED) or
+        daughter indices (PATH_INDICES, NavStateIndex::ResetPathFromListOfIndices). Paths the geometry
+        does not hold give 0."""
+        n = len(levels)
+        device_array(levels, torch.int32, n, "levels")
+        if not (torch.is_tensor(path) and path.dim() == 2 and path.shape[1] == n and path.is_contiguous()
+                and path.dtype == torch.int32 and path.device == levels.device):
+            raise ValueError("path must be a contiguous (max_levels, n) int32 tensor on the levels' device")
+        out = torch.empty(n, dtype=torch.int32, device=levels.device)
+        check(lib().vgmi_nav_from_path(self._h, int(kind), levels.data_ptr(), path.data_ptr(), n, path.shape[0],
+                                       out.data_ptr(), _stream()), "vgmi_nav_from_path")
+        return out
+
+    def Distance(self, nav_a, nav_b):
+        """NavStateIndex::Distance per pair (moves up to the deepest shared level, then down)."""
+        n = len(nav_a)
+        device_array(nav_a, _U32, n, "nav_a")
+        device_array(nav_b, _U32, n, "nav_b")
+        out = torch.empty(n, dtype=torch.int32, device=nav_a.device)
+        check(lib().vgmi_nav_distance(self._h, nav_a.data_ptr(), nav_b.data_ptr(), n, out.data_ptr(), _stream()),
+              "vgmi_nav_distance")
+        return out
+
+    def GlobalToLocal(self, nav, points, tolevel=-1):
+        """NavStateIndex::GlobalToLocal(point[, tolevel]): points in the frame of each state's ancestor at
+        tolevel (-1: the top volume)."""
+        n = len(points)
+        device_array(nav, _U32, n, "nav")
+        out = SOA3D.empty(n, device=points.x.device)
+        check(lib().vgmi_nav_global_to_local(self._h, nav.data_ptr(), int(tolevel), *points.ptrs(), *out.ptrs(), n,
+                                             _stream()), "vgmi_nav_global_to_local")
         return out
 
     def LevelLocate(self, logical, points, excl=None, precision=FP64):

@@ -30,6 +30,9 @@ def _declare():
         "vgmi_server_create": [vp, ci, d, i64, ci, i64, ctypes.POINTER(vp)],
         "vgmi_server_submit": [vp] + [vp] * 7 + [i64] + [vp] * 5 + [ctypes.POINTER(ctypes.c_uint64)],
         "vgmi_server_wait": [vp, ctypes.c_uint64, ctypes.POINTER(d)],
+        "vgmi_server_poll": [vp, ctypes.c_uint64, ctypes.POINTER(ci)],
+        "vgmi_server_cancel": [vp, ctypes.c_uint64],
+        "vgmi_server_set_hbm_reserve": [vp, i64],
         "vgmi_server_stats": [vp, vp],
         "vgmi_server_graph_stats": [vp, vp],
         "vgmi_server_destroy": [vp],
@@ -94,6 +97,22 @@ class Server:
         check(lib().vgmi_server_wait(self._h, r.ticket, ctypes.byref(lat)), "vgmi_server_wait")
         r.latency_us = lat.value
         return r
+
+    def poll(self, r):
+        """True once the request has finished (wait() then returns at once); never blocks."""
+        done = ctypes.c_int()
+        check(lib().vgmi_server_poll(self._h, r.ticket, ctypes.byref(done)), "vgmi_server_poll")
+        return bool(done.value)
+
+    def cancel(self, r):
+        """Cancels a request that is still queued; wait() then raises for it. Raises when the
+        request is already on the GPU or done."""
+        check(lib().vgmi_server_cancel(self._h, r.ticket), "vgmi_server_cancel")
+
+    def set_hbm_reserve(self, nbytes):
+        """Admission by free HBM: submissions and batches are refused while less than `nbytes`
+        of device memory is free (0 turns the check off)."""
+        check(lib().vgmi_server_set_hbm_reserve(self._h, int(nbytes)), "vgmi_server_set_hbm_reserve")
 
     def stats(self):
         s = np.zeros(4, dtype=np.int64)
