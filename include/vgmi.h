@@ -51,7 +51,9 @@ enum vgmi_shape_type {
   VGMI_TESSELLATED = 9, /* ntri then 9 doubles per triangle        volumes/UnplacedTessellated.h */
   VGMI_HYPE = 10,       /* rmin rmax stIn stOut dz                 volumes/UnplacedHype.h */
   VGMI_TRD = 11,        /* x1 x2 y1 y2 z (or x1 x2 y z, or x y z; a sixth value 1: the Trd1 kind)  volumes/UnplacedTrd.h:51-80 */
-  VGMI_PARABOLOID = 12  /* rlo rhi dz (rhi > rlo, as G4Paraboloid)  volumes/UnplacedParaboloid.h:60 */
+  VGMI_PARABOLOID = 12, /* rlo rhi dz (rhi > rlo, as G4Paraboloid)  volumes/UnplacedParaboloid.h:60 */
+  VGMI_PARALLELEPIPED = 13, /* dx dy dz alpha theta phi            volumes/UnplacedParallelepiped.h:65-71 */
+  VGMI_TET = 14         /* the four corners p0 p1 p2 p3 (12 values) volumes/UnplacedTet.h:48-49 */
 };
 
 /* Query methods of VUnplacedVolume / VPlacedVolume (volumes/UnplacedVolume.h:88-223). */
@@ -202,15 +204,19 @@ VGMI_API vgmi_status vgmi_geom_bvh(const vgmi_geom *geom, int logical, int count
  * GDML ingestion: vgdml::Frontend::Load / Parser::Load (persistency/gdml/source/include/Frontend.h:12-21,
  * persistency/gdml/source/src/Middleware.cpp) without Xerces. The file's solids, logical volumes and
  * placements become a vgmi_geom_desc for vgmi_geom_build: logical volumes in document order,
- * placement k from the k-th <physvol>, solids as GeoManager::MakeInstance returns them (a full sphere
+ * placement k from the k-th <physvol>, solids as GeoManager::MakeInstance returns them. In the
+ * reference's default build (VECGEOM_NO_SPECIALIZATION=ON, CMakeLists.txt:155) that is every solid as
+ * written; flags VGMI_GDML_SPECIALIZED applies the substitutions of a build with it OFF (a full sphere
  * as an orb, a cone or hype with straight sides as a tube, a Trd or trapezoid with equal ends as a
- * box, ...). Lengths are scaled by mm_unit per millimetre (the reference's default 0.1: centimetres;
- * 1.0 keeps millimetres). Supported solids: box, orb, tube, cone, polycone, polyhedra, torus, sphere,
- * trd, trap, paraboloid, hype, tessellated; any other solid, a reflected placement or a malformed
- * document fails with VGMI_ERR_INVALID and a message. Materials and auxiliary data are skipped.
+ * box, a trapezoid of equal widths and tilts as a parallelepiped, ...). Lengths are scaled by mm_unit
+ * per millimetre (the reference's default 0.1: centimetres; 1.0 keeps millimetres). Supported solids:
+ * box, orb, tube, cone, polycone, polyhedra, torus, sphere, trd, trap, para, tet, paraboloid, hype,
+ * tessellated; any other solid, a reflected placement or a malformed document fails with
+ * VGMI_ERR_INVALID and a message. Materials and auxiliary data are skipped.
  */
+#define VGMI_GDML_SPECIALIZED 1
 typedef struct vgmi_gdml vgmi_gdml;
-VGMI_API vgmi_status vgmi_gdml_read(const char *path, double mm_unit, vgmi_gdml **out);
+VGMI_API vgmi_status vgmi_gdml_read(const char *path, double mm_unit, int flags, vgmi_gdml **out);
 /* Fills *desc with arrays owned by the reader handle (valid until vgmi_gdml_destroy). */
 VGMI_API vgmi_status vgmi_gdml_desc(const vgmi_gdml *gdml, vgmi_geom_desc *desc);
 /* Name of logical volume (kind 0) or placed volume (kind 1) `index`; the string is owned by the handle. */

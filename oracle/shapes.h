@@ -243,6 +243,47 @@ public:
   bool OnZPlane(bool top, V3 const &p) const;
 };
 
+// ---------------------------------------------------------------- Parallelepiped
+// VecGeom/volumes/ParallelepipedStruct.h:80-124 (cached tangents, normals, safety scale factors),
+// VecGeom/volumes/kernel/ParallelepipedImplementation.h:60-236, source/UnplacedParallelepiped.cpp:79-89
+class Parallelepiped : public Shape {
+public:
+  V3 dim;
+  double tanAlpha, tanThetaSinPhi, tanThetaCosPhi, cosTheta, ctx, cty;
+  V3 normals[3];
+  Parallelepiped(double dx, double dy, double dz, double alpha, double theta, double phi);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  V3 Oblique(V3 p) const;           // ParallelepipedImplementation::Transform
+  V3 SafetyVector(V3 const &lp) const;
+};
+
+// ---------------------------------------------------------------- Tet
+// VecGeom/volumes/TetStruct.h:57-90 (face planes from the corners, outward),
+// VecGeom/volumes/kernel/TetImplementation.h:61-221, source/UnplacedTet.cpp:34-50
+class Tet : public Shape {
+public:
+  V3 vertex[4];
+  V3 n[4];
+  double d[4];
+  Tet(V3 const &p0, V3 const &p1, V3 const &p2, V3 const &p3);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &nrm) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  double MaxPlaneDistance(V3 const &p) const;
+};
+
 // ---------------------------------------------------------------- Cone (UniversalCone)
 // VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
 // ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820

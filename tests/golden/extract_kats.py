@@ -58,6 +58,11 @@ def _norm_expr(s):
 
 
 CURRENT_VECS = {}
+# shapes of the file being read (name -> constructor parameters) and the getters of its shape class
+# that the evaluator may call (method name -> function of the parameters), e.g. a test that builds
+# its points from para.GetTanThetaCosPhi()
+CURRENT_SHAPES = {}
+CURRENT_GETTERS = {}
 _VEC_CTOR = re.compile(r"(?:Vec_t|Vector3D\s*<\s*\w+\s*>|Vec3D_t)\s*\(")
 
 
@@ -133,6 +138,10 @@ def eval_expr(expr, scalars, vecs=None):
                 if any(isvec(v) for v in args):
                     raise Unsupported(expr)
                 return float(FUNCS[n.func.id](*args))
+        if (isinstance(n, ast.Call) and isinstance(n.func, ast.Attribute) and not n.args
+                and isinstance(n.func.value, ast.Name) and n.func.value.id in CURRENT_SHAPES
+                and n.func.attr in CURRENT_GETTERS):
+            return float(CURRENT_GETTERS[n.func.attr](CURRENT_SHAPES[n.func.value.id]))
         if isinstance(n, ast.Call) and isinstance(n.func, ast.Attribute) and not n.args:
             v = ev(n.func.value)
             if not isvec(v):
@@ -316,7 +325,7 @@ def direct_query_assert(body, shapes, vecs, scalars, vec_type, ln):
     if neg:
         return None
     if meth == "Inside":
-        mi = re.fullmatch(r"==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", rest)
+        mi = re.fullmatch(r"==\s*(?:vecgeom::)?(?:(?:EInside|EnumInside)::)?[ke](Inside|Outside|Surface)", rest)
         return (sh, meth, p, None, ln, {"kind": "inside", "value": mi.group(1)}) if mi else None
     mc = re.fullmatch(r"(==|<=|<|>=|>)\s*(.+)", rest)
     if not mc:
@@ -352,9 +361,11 @@ def skip_reason(body, shapes, pending, unknown_results):
     return "not a query assert (capacity, surface area, getters, bounding box)"
 
 
-def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
+def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None, getters=None):
     text = open(path).read()
     vecs, scalars, shapes, pending, arrays, extents, logicals = {}, {}, {}, {}, {}, {}, {}
+    global CURRENT_SHAPES, CURRENT_GETTERS
+    CURRENT_SHAPES, CURRENT_GETTERS = shapes, dict(getters or {})
     unknown_results = {}
     normals, valids = {}, {}  # Normal() output vector / validity variables -> (shape, p, chain, line)
     vec_arrays = set()
@@ -375,7 +386,8 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
                     raise Unsupported(a)
                 vals.extend(arrays[a])
             else:
-                vals.append(eval_scalar(a, scalars))
+                v = eval_expr(a, scalars, vecs)  # a vector argument (Tet's corners) adds its components
+                vals.extend(v if isinstance(v, list) else [v])
         return vals
 
     def normal_point(expr):
@@ -478,7 +490,7 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None):
                 except Unsupported:
                     rec = None
             else:
-                mm = re.fullmatch(r"(\w+)\s*==\s*(?:vecgeom::)?(?:EInside|EnumInside)::[ke](\w+)", body)
+                mm = re.fullmatch(r"(\w+)\s*==\s*(?:vecgeom::)?(?:(?:EInside|EnumInside)::)?[ke](Inside|Outside|Surface)", body)
                 if mm and mm.group(1) in pending and pending[mm.group(1)][1] == "Inside":
                     sh, meth, p, d, _ = pending[mm.group(1)]
                     rec = (sh, meth, p, d, ln, {"kind": "inside", "value": mm.group(2)})
@@ -836,6 +848,20 @@ SHAPES = {
     "hype": ("TestHype.cpp", r"Hype_t", (5,), r"S?UnplacedHype\w*"),
     "trd": ("TestTrd.cpp", r"Trd_t", (3, 4, 5), r"UnplacedTrd"),
     "paraboloid": ("TestParaboloid.cpp", r"Paraboloid_t", (3,), r"UnplacedParaboloid"),
+    "parallelepiped": ("TestParallelepiped.cpp", r"Parallelepiped_t", (6,), r"UnplacedParallelepiped"),
+    "tet": ("TestTet.cpp", r"Tet_t", (12,), r"UnplacedTet"),
+}
+
+
+# Getters of a shape class a test calls while building its points: ParallelepipedStruct's cached
+# tangents (VecGeom/volumes/ParallelepipedStruct.h:80-88), computed as the reference computes them.
+GETTERS = {
+    "parallelepiped": {
+        "GetTanThetaCosPhi": lambda a: math.tan(a[4]) * math.cos(a[5]),
+        "GetTanThetaSinPhi": lambda a: math.tan(a[4]) * math.sin(a[5]),
+        "GetTanAlpha": lambda a: math.tan(a[3]),
+        "GetX": lambda a: a[0], "GetY": lambda a: a[1], "GetZ": lambda a: a[2],
+    },
 }
 
 
@@ -858,7 +884,8 @@ if __name__ == "__main__":
     for name in only:
         fn, pat, counts, unplaced = SHAPES[name]
         st = {"source": "test/unit_tests/%s" % fn, "asserts": 0, "extracted": 0, "skipped": {}}
-        recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat, unplaced=unplaced, stats=st)
+        recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat, unplaced=unplaced, stats=st,
+                       getters=GETTERS.get(name))
         kept = []
         for r in recs:
             if params_ok(counts, r["shape"]):

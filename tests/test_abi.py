@@ -54,7 +54,11 @@ def test_extent_matches_oracle():
              ("sphere", [0.0, 50.0, 0.0, 2 * math.pi, 0.75 * math.pi, 0.25 * math.pi]),
              ("sphere", [6.0, 8.0, math.pi / 6, math.pi / 3, 0.0, math.pi]),
              ("trd", [10.0, 30.0, 20.0, 40.0, 40.0]), ("trd", [30.0, 10.0, 25.0, 40.0]), ("trd", [5.0, 6.0, 7.0]),
-             ("paraboloid", [6.0, 9.0, 10.0]), ("paraboloid", [0.0, 8.0, 10.0])]
+             ("paraboloid", [6.0, 9.0, 10.0]), ("paraboloid", [0.0, 8.0, 10.0]),
+             ("parallelepiped", [20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(45)]),
+             ("parallelepiped", [8.0, 5.0, 12.0, math.radians(-20), math.radians(15), math.radians(200)]),
+             ("tet", [0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0]),
+             ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0])]
     # shapes with a variable-length block need a device for creation: extent from the host record
     cases = [c for c in cases if c[0] != "polyhedron"]
     for kind, params in cases:

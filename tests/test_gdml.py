@@ -21,13 +21,14 @@ CM = 10 * MM
 DEG = math.pi / 180.
 
 
-def _one(name, mm=MM):
-    g = load_gdml(os.path.join(FIX, name), mm)
+def _one(name, mm=MM, specialized=False):
+    g = load_gdml(os.path.join(FIX, name), mm, specialized=specialized)
     assert g.world == 0 and len(g.logicals) == 1 and not g.placements
     return g.shapes[0]
 
 
-# (file, expected (type, params)) worked out by hand from each file's solid element
+# (file, expected (type, params)) worked out by hand from each file's solid element, as the reference's
+# default build (VECGEOM_NO_SPECIALIZATION=ON) makes them: every solid as written
 ONE_SOLID = [
     ("oneBox.gdml", (S.BOX, [100 * CM / 2] * 3)),
     ("oneOrb.gdml", (S.ORB, [100 * CM])),
@@ -38,8 +39,7 @@ ONE_SOLID = [
                                                 1.0, 150 * CM / 2, 10 * CM / 2, 10 * CM / 2, 1.0])),
     ("oneHyperbolicTube.gdml", (S.HYPE, [10 * CM, 20 * CM, 3.0, 4.0, 100 * CM / 2])),
     ("oneParaboloid.gdml", (S.PARABOLOID, [50 * CM, 75 * CM, 100 * CM])),
-    # two z planes: the factory's cone of half-height |z1 - z0| / 2 (centred at z = 0)
-    ("onePolycone.gdml", (S.CONE, [15 * MM, 30 * MM, 70 * MM, 100 * MM, abs(80 * MM - 50 * MM) / 2, 1.0, 4.0])),
+    ("onePolycone.gdml", (S.POLYCONE, [1.0, 4.0, 2, 50 * MM, 80 * MM, 15 * MM, 70 * MM, 30 * MM, 100 * MM])),
     ("onePolyhedron.gdml", (S.POLYHEDRON, [1.0, 4.0, 11, 2, 50 * MM, 80 * MM, 15 * MM, 70 * MM, 30 * MM, 100 * MM])),
     ("oneTorus.gdml", (S.TORUS2, [10 * CM, 100 * CM, 20 * CM, 1.0, 3.0])),
     ("oneTrapezoid.gdml", (S.TRD, [90 * CM / 2, 80 * CM / 2, 60 * CM / 2, 50 * CM / 2, 100 * CM / 2])),
@@ -47,19 +47,30 @@ ONE_SOLID = [
 ]
 
 
+# the same files through the substitutions of a build with VECGEOM_NO_SPECIALIZATION=OFF, where they differ
+ONE_SOLID_SPECIALIZED = {
+    # two z planes: the factory's cone of half-height |z1 - z0| / 2 (centred at z = 0)
+    "onePolycone.gdml": (S.CONE, [15 * MM, 30 * MM, 70 * MM, 100 * MM, abs(80 * MM - 50 * MM) / 2, 1.0, 4.0]),
+}
+
+
 @pytest.mark.parametrize("name,exp", ONE_SOLID, ids=[c[0] for c in ONE_SOLID])
 def test_one_solid_files(name, exp):
     t, p = _one(name)
     assert t == exp[0], (t, exp[0])
     assert p == [float(x) for x in exp[1]], (p, exp[1])
+    exp = ONE_SOLID_SPECIALIZED.get(name, exp)
+    t, p = _one(name, specialized=True)
+    assert (t, p) == (exp[0], [float(x) for x in exp[1]]), (t, p)
 
 
 def test_full_sphere_is_an_orb():
-    # 0..360 deg and 0..180 deg: MakeInstance<UnplacedSphere> returns an orb when the ranges are full
+    # 0..360 deg and 0..180 deg: a sphere as written; with the substitutions MakeInstance<UnplacedSphere>
+    # returns an orb when the ranges are full
+    sphere = (S.SPHERE, [0.0, 100 * CM, 0.0, 360 * DEG, 0.0, 180 * DEG])
+    assert _one("oneSphere.gdml") == sphere
     full = 360 * DEG >= 2 * math.pi and 180 * DEG == math.pi
-    t, p = _one("oneSphere.gdml")
-    assert (t, p) == ((S.ORB, [100 * CM]) if full else
-                      (S.SPHERE, [0.0, 100 * CM, 0.0, 360 * DEG, 0.0, 180 * DEG]))
+    assert _one("oneSphere.gdml", specialized=True) == ((S.ORB, [100 * CM]) if full else sphere)
 
 
 def test_tessellated_facets():
@@ -188,9 +199,18 @@ def test_make_instance_substitutions(tmp_path):
       <trd name="d1" x1="6" x2="6" y1="4" y2="4" z="20"/>
       <trd name="d2" x1="6" x2="8" y1="4" y2="4" z="20"/>
       <sphere name="s1" rmin="0" rmax="5" startphi="0" deltaphi="360" starttheta="0" deltatheta="90" aunit="deg"/>
+      <trap name="t4" z="20" theta="0.2" phi="0.3" y1="4" x1="6" x2="6" alpha1="0.1" y2="4" x3="6" x4="6" alpha2="0.1"/>
+      <para name="p1" x="6" y="4" z="20" alpha="0.1" theta="0.2" phi="0.3"/>
       <box name="w" x="100" y="100" z="100"/>"""
     text = BODY % (solids, "<volume name='W'><solidref ref='w'/></volume>", "W")
+    # the reference's default build: every solid as written
     g = load_gdml(_write(tmp_path, text), 1.0)
+    assert [t for t, _ in g.shapes] == [S.CONE] * 3 + [S.HYPE] * 3 + [S.TRAPEZOID] * 3 + [S.TRD] * 2 + [
+        S.SPHERE, S.TRAPEZOID, S.PARALLELEPIPED, S.BOX]
+    assert g.shapes[10] == (S.TRD, [3.0, 4.0, 2.0, 2.0, 10.0]) and g.shapes[0][1][6] == 7.0
+    assert g.shapes[13] == (S.PARALLELEPIPED, [3.0, 2.0, 10.0, 0.1, 0.2, 0.3])
+    # with VECGEOM_NO_SPECIALIZATION=OFF's substitutions
+    g = load_gdml(_write(tmp_path, text), 1.0, specialized=True)
     two_pi = 2 * math.pi
     exp = [(S.TUBE, [0.0, 5.0, 10.0, 0.0, two_pi]), (S.TUBE, [2.0, 5.0, 10.0, 0.0, 1.0]),
            (S.CONE, [0.0, 5.0, 0.0, 6.0, 10.0, 0.0, two_pi]),
@@ -198,7 +218,8 @@ def test_make_instance_substitutions(tmp_path):
            (S.HYPE, [0.0, 5.0, 0.0, 0.5, 10.0]),
            (S.BOX, [3.0, 2.0, 10.0]), (S.TRD, [3.0, 4.0, 2.0, 2.0, 10.0, 1.0]), (S.TRD, [3.0, 4.0, 2.0, 2.5, 10.0]),
            (S.BOX, [3.0, 2.0, 10.0]), (S.TRD, [3.0, 4.0, 2.0, 2.0, 10.0, 1.0]),
-           (S.SPHERE, [0.0, 5.0, 0.0, 360 * DEG, 0.0, 90 * DEG]), (S.BOX, [50.0] * 3)]
+           (S.SPHERE, [0.0, 5.0, 0.0, 360 * DEG, 0.0, 90 * DEG]), (S.PARALLELEPIPED, [3.0, 2.0, 10.0, 0.1, 0.2, 0.3]),
+           (S.PARALLELEPIPED, [3.0, 2.0, 10.0, 0.1, 0.2, 0.3]), (S.BOX, [50.0] * 3)]
     assert g.shapes == exp
 
 
@@ -229,7 +250,8 @@ def test_all_solids_file_bvh_matches_brute_force():
     g = load_gdml(os.path.join(FIX, "allSolids.gdml"), 1.0)
     assert [t for t, _ in g.shapes] == [S.BOX, S.BOX, S.TRD, S.TRD, S.PARABOLOID, S.HYPE, S.CONE, S.POLYCONE,
                                          S.POLYHEDRON, S.SPHERE, S.TORUS2, S.TUBE, S.ORB, S.TRAPEZOID, S.TESSELLATED]
-    assert g.shapes[3][1][5] == 1.0  # y1 == y2: the Trd1 kind
+    assert g.shapes[3][1] == [20.0, 30.0, 15.0, 15.0, 25.0]  # y1 == y2, a UniversalTrd as written
+    assert load_gdml(os.path.join(FIX, "allSolids.gdml"), 1.0, specialized=True).shapes[3][1][5] == 1.0  # Trd1
     o, ob = OracleGeometry(g), OracleGeometry(g, use_bvh=False)
     rng = np.random.default_rng(7)
     centres = np.array([t[:3] for _, _, t in g.placements]).T  # the daughters' origins (in their cores)
@@ -247,3 +269,23 @@ def test_all_solids_file_bvh_matches_brute_force():
             assert np.array_equal(x, y), s
         step, nav, last, _ = a
         p = p + step * d
+
+
+def test_tet_corners_are_named_positions(tmp_path):
+    """<tet vertex1..4="name"> refers to <define><position> elements (Middleware.cpp:1155-1172) in their own
+    length unit; an unknown name reads (0, 0, 0) as the middleware's position map does."""
+    text = """<gdml>
+ <define>
+  <position name="a" x="0" y="0" z="2" unit="cm"/>
+  <position name="b" x="0" y="0" z="0"/>
+  <position name="c" x="20" y="0" z="0"/>
+ </define>
+ <solids>
+  <tet name="t1" vertex1="a" vertex2="b" vertex3="c" vertex4="nowhere"/>
+  <box name="w" x="100" y="100" z="100"/>
+ </solids>
+ <structure><volume name="W"><solidref ref="w"/></volume></structure>
+ <setup name="default" version="1.0"><world ref="W"/></setup>
+</gdml>"""
+    g = load_gdml(_write(tmp_path, text), 1.0)
+    assert g.shapes[0] == (S.TET, [0.0, 0.0, 20.0, 0.0, 0.0, 0.0, 20.0, 0.0, 0.0, 0.0, 0.0, 0.0])

@@ -49,7 +49,29 @@ def _geometries():
             gdml_util.write(g, path)
             return load_gdml(path, 1.0)
     out.append(("calorimeter", calo))
+    out.append(("allSolids-specialized", lambda: load_gdml(os.path.join(FIX, "allSolids.gdml"), 1.0, specialized=True)))
+    out.append(("parallelepipeds-and-tets", planar_solids))
     return out
+
+
+def planar_solids():
+    """Parallelepipeds and tets placed with rotations in a box world, one parallelepiped holding a tet."""
+    from tests.test_nav_path import _rot
+    from vecgeom_amd.shapes import BOX, PARALLELEPIPED, TET
+    g = geometry.GeometryDesc()
+    w = g.add_logical(g.add_shape(BOX, [300.0, 300.0, 300.0]), "world")
+    g.set_world(w)
+    para = g.add_logical(g.add_shape(PARALLELEPIPED, [40.0, 30.0, 50.0, 0.3, 0.25, 0.7]), "para")
+    tet = g.add_logical(g.add_shape(TET, [0.0, 0.0, 30.0, -20.0, -15.0, -10.0, 25.0, -10.0, -10.0, 0.0, 25.0, -10.0]),
+                        "tet")
+    small = g.add_logical(g.add_shape(TET, [0.0, 0.0, 8.0, -6.0, -5.0, -4.0, 7.0, -4.0, -4.0, 0.0, 7.0, -4.0]),
+                          "small tet")
+    g.place(para, small, (5.0, -3.0, 2.0) + _rot("x", 0.4))
+    for k, (x, y, axis, ang) in enumerate([(-150, -150, "z", 0.3), (150, -140, "y", 0.8), (-140, 150, "x", -0.5)]):
+        g.place(w, para, (float(x), float(y), 20.0 * k) + _rot(axis, ang))
+    for k, (x, y, axis, ang) in enumerate([(150, 150, "z", 1.1), (0, 0, "x", 0.2), (0, -180, "y", -1.3)]):
+        g.place(w, tet, (float(x), float(y), -30.0 * k) + _rot(axis, ang))
+    return g
 
 
 @pytest.mark.parametrize("name,make", _geometries(), ids=[n for n, _ in _geometries()])

@@ -73,6 +73,11 @@ CASES = [
     ("paraboloid", [6.0, 9.0, 10.0]),
     ("paraboloid", [0.0, 8.0, 10.0]),
     ("paraboloid", [1.0, 12.0, 8.0]),
+    ("parallelepiped", [20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(45)]),
+    ("parallelepiped", [8.0, 5.0, 12.0, math.radians(-20), math.radians(15), math.radians(200)]),
+    ("parallelepiped", [10.0, 10.0, 10.0, 0.0, 0.0, 0.0]),
+    ("tet", [0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0]),
+    ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing then stops at a neighbouring
@@ -119,7 +124,8 @@ def test_query_all_matches_single_methods(case):
 
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
-                                   "torus2", "tessellated", "hype", "trd", "paraboloid"])
+                                   "torus2", "tessellated", "hype", "trd", "paraboloid", "parallelepiped",
+                                   "tet"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

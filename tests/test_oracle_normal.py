@@ -42,6 +42,8 @@ CASES = [
     ("hype", [10.0, 20.0, math.pi / 4, math.pi / 3, 50.0]),
     ("trd", [10.0, 30.0, 20.0, 40.0, 40.0]),
     ("paraboloid", [6.0, 9.0, 10.0]),
+    ("parallelepiped", [20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(45)]),
+    ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
 ]
 
 

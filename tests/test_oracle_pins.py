@@ -280,6 +280,23 @@ CASES.append(("paraboloid", [6.0, 9.0, 10.0], margin_paraboloid))
 # the reference's root selection takes the quadratic's leading coefficient as positive, i.e. rhi > rlo
 # (the G4Paraboloid convention R2 > R1); rlo > rhi is not pinned
 CASES.append(("paraboloid", [1.0, 12.0, 8.0], margin_paraboloid))
+
+
+def parallelepiped_corners(params):
+    """The eight corners of a parallelepiped (dx dy dz alpha theta phi): the oblique-frame corners
+    (+-dx, +-dy, +-dz) with y = y' + tan(theta) sin(phi) z and x = x' + tan(theta) cos(phi) z + tan(alpha) y'."""
+    dx, dy, dz, al, th, ph = params
+    tc, ts, ta = math.tan(th) * math.cos(ph), math.tan(th) * math.sin(ph), math.tan(al)
+    return np.array([(xs * dx + tc * zs * dz + ta * ys * dy, ys * dy + ts * zs * dz, zs * dz)
+                     for zs in (-1, 1) for ys in (-1, 1) for xs in (-1, 1)])
+
+
+for _t in ([20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(45)],
+           [8.0, 5.0, 12.0, math.radians(-20), math.radians(15), math.radians(200)]):
+    CASES.append(("parallelepiped", _t, margin_convex(parallelepiped_corners(_t))))
+for _t in ([0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0],
+           [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]):
+    CASES.append(("tet", _t, margin_convex(np.array(_t).reshape(4, 3))))
 IDS = ["%s-%d" % (c[0], i) for i, c in enumerate(CASES)]
 
 

@@ -145,6 +145,8 @@ static int method_groups(int type, MethodGroup g[3])
   case kTube:
   case kTrd:
   case kParaboloid:
+  case kParallelepiped:
+  case kTet:
     g[0] = {kMAll, 2};
     return 1;
   case kCone:

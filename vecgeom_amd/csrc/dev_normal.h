@@ -635,5 +635,53 @@ VGMI_DEV bool Normal(TessellatedData const &s, V3d const &p, V3d &n, bool &set)
   return true;
 }
 
+// ParallelepipedImplementation::NormalKernel (VecGeom/volumes/kernel/ParallelepipedImplementation.h:185-232)
+VGMI_DEV bool Normal(ParallelepipedData const &s, V3d const &point, V3d &n, bool &set)
+{
+  const V3d p = ppd::oblique(s, point), sv = ppd::safety_vector(s, p);
+  const V3d signs(Sign(p.x), Sign(p.y), Sign(p.z));
+  const V3d n0(s.n0[0], s.n0[1], s.n0[2]), n1(s.n1[0], s.n1[1], s.n1[2]);
+  V3d normal(0., 0., 0.);
+  if (vabs(sv.z) <= kHalfTolerance) normal = V3d(0., 0., signs.z);
+  if (vabs(sv.y) <= kHalfTolerance) normal = normal + signs.y * n1;
+  if (vabs(sv.x) <= kHalfTolerance) normal = normal + signs.x * n0;
+  const double mag2 = normal.Mag2();
+  if (mag2 > 1.) normal = Unit(normal);
+  set = true;
+  if (mag2 > 0.) {
+    n = normal;
+    return true;
+  }
+  const double safety = Max2(Max2(sv.x, sv.y), sv.z);
+  normal              = signs.x * n0;
+  if (sv.y == safety) normal = signs.y * n1;
+  if (sv.z == safety) normal = V3d(0., 0., signs.z);
+  n = normal;
+  return false;
+}
+
+// TetImplementation::NormalKernel (VecGeom/volumes/kernel/TetImplementation.h:187-218)
+VGMI_DEV bool Normal(TetData const &s, V3d const &p, V3d &n, bool &set)
+{
+  V3d normal(0., 0., 0.);
+  double dist[4];
+  for (int i = 0; i < 4; ++i) {
+    const V3d ni = tet::normal<double>(s, i);
+    dist[i]      = ni.Dot(p) + s.d[i];
+    if (vabs(dist[i]) <= kHalfTolerance) normal = normal + ni;
+  }
+  if (normal.Mag2() > 1) normal = Unit(normal);
+  set = true;
+  if (normal.Mag2() > 0.) {
+    n = normal;
+    return true;
+  }
+  double safety = -kInfLength;
+  for (int i = 0; i < 4; ++i)
+    if (dist[i] > safety) normal = tet::normal<double>(s, i), safety = dist[i];
+  n = normal;
+  return false;
+}
+
 } // namespace nrm
 } // namespace vgmi

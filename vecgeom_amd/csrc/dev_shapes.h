@@ -5,6 +5,7 @@
 #include "dev_cone.h"
 #include "dev_hype.h"
 #include "dev_orb.h"
+#include "dev_para_tet.h"
 #include "dev_paraboloid.h"
 #include "dev_polycone.h"
 #include "dev_polyhedron.h"
@@ -32,7 +33,9 @@
   X(kTorus2, TorusData, tor, tor)                                                                                    \
   X(kHype, HypeData, hype, hype)                                                                                     \
   X(kTrd, TrdData, trd, trd)                                                                                         \
-  X(kParaboloid, ParaboloidData, para, para)
+  X(kParaboloid, ParaboloidData, para, para)                                                                         \
+  X(kParallelepiped, ParallelepipedData, ppd, ppd)                                                                   \
+  X(kTet, TetData, tet, tet)
 
 namespace vgmi {
 

@@ -219,7 +219,7 @@ struct V3 {
 
 class Reader {
 public:
-  explicit Reader(double mm) : mm_(mm) {}
+  Reader(double mm, bool specialized) : mm_(mm), specialized_(specialized) {}
   Model model;
   std::string err;
 
@@ -241,6 +241,7 @@ public:
 
 private:
   double mm_;
+  bool specialized_;
   std::map<std::string, double> constants_;
   std::map<std::string, V3> positions_, rotations_, scales_;
   std::map<std::string, int> solids_;  // solid name -> shape index
@@ -340,16 +341,19 @@ private:
     return int(model.shape_type.size()) - 1;
   }
 
-  // GeoManager::MakeInstance: the solid that the factory returns for these parameters.
+  // GeoManager::MakeInstance: the solid that the factory returns for these parameters. A build with
+  // VECGEOM_NO_SPECIALIZATION=ON (the reference's default, CMakeLists.txt:155) returns every solid as
+  // written (each Maker's #else branch); the substitutions below are those of a build with it OFF,
+  // applied when the caller asks for them (specialized_).
   //  cone -> tube when the radii allow it (source/UnplacedCone.cpp:449-540; a full phi range is set
   //          to exactly 2 pi); hype -> tube or the solid hype (source/UnplacedHype.cpp:40-60);
   //  polycone of two planes -> cone of half-height |z1 - z0| / 2 centred at z = 0, as the factory
   //          builds it (source/UnplacedPolycone.cpp:50-60); sphere -> orb (source/UnplacedSphere.cpp:
-  //          285-296); trapezoid -> box or trd (source/UnplacedTrapezoid.cpp:100-125; the
-  //          parallelepiped-like case stays a trapezoid: no parallelepiped here); trd -> box or
-  //          the Trd1 kind (source/UnplacedTrd.cpp:48-85).
+  //          285-296); trapezoid -> box, trd or parallelepiped (source/UnplacedTrapezoid.cpp:
+  //          100-125); trd -> box or the Trd1 kind (source/UnplacedTrd.cpp:48-85).
   int make_instance(int type, std::vector<double> p)
   {
+    if (!specialized_) return add_shape(type, p);
     const double twoPi = 2 * kPiGdml;
     switch (type) {
     case kCone: { // rmin1 rmax1 rmin2 rmax2 dz sphi dphi
@@ -394,6 +398,8 @@ private:
       if (square && dx1 == dx2 && dx3 == dx4 && dx2 != dx3 && dy1 == dy2)
         return add_shape(kTrd, {dx1, dx3, dy1, dy1, dz, 1.});
       if (square && dx1 == dx2 && dx3 == dx4 && dx2 != dx3 && dy1 != dy2) return add_shape(kTrd, {dx1, dx3, dy1, dy2, dz});
+      if (a1 == a2 && dx1 == dx2 && dx2 == dx3 && dx3 == dx4 && dy1 == dy2)
+        return add_shape(kParallelepiped, {dx1, dy1, dz, a1, th, ph});
       return add_shape(kTrapezoid, p);
     }
     case kTrd: { // x1 x2 y1 y2 z
@@ -512,6 +518,16 @@ private:
     } else if (n == "tessellated") {
       type = kTessellated;
       if (!tessellated(e, p)) return false;
+    } else if (n == "para") { // processParallelepiped (Middleware.cpp:967-988): x y z halved
+      type = kParallelepiped;
+      p    = {l * num(e, "x") / 2., l * num(e, "y") / 2., l * num(e, "z") / 2., a * num(e, "alpha"),
+           a * num(e, "theta"), a * num(e, "phi")};
+    } else if (n == "tet") { // processTet (Middleware.cpp:1155-1172): the corners are named positions
+      type = kTet;
+      for (const char *v : {"vertex1", "vertex2", "vertex3", "vertex4"}) {
+        V3 const q = positions_[e.attr(v)]; // an unknown name reads (0, 0, 0), as the middleware's map does
+        p.insert(p.end(), {q.x, q.y, q.z});
+      }
     } else {
       return fail("solid <" + n + "> (\"" + name + "\") is not supported");
     }
@@ -606,7 +622,7 @@ private:
   }
 };
 
-bool Read(std::string const &path, double mm, Model &out, std::string &err)
+bool Read(std::string const &path, double mm, bool specialized, Model &out, std::string &err)
 {
   std::ifstream f(path, std::ios::binary);
   if (!f) return err = "cannot open " + path, false;
@@ -617,7 +633,7 @@ bool Read(std::string const &path, double mm, Model &out, std::string &err)
   XmlParser parser(text);
   if (!parser.parse(root, err)) return false;
   if (root.name != "gdml") return err = "root element is <" + root.name + ">, not <gdml>", false;
-  Reader r(mm);
+  Reader r(mm, specialized);
   r.model.shape_offset.push_back(0);
   if (!r.node(root)) return err = r.err, false;
   if (r.model.world < 0) return err = "no <world> in the <setup>", false;
@@ -636,17 +652,18 @@ struct vgmi_gdml {
 
 extern "C" {
 
-vgmi_status vgmi_gdml_read(const char *path, double mm_unit, vgmi_gdml **out)
+vgmi_status vgmi_gdml_read(const char *path, double mm_unit, int flags, vgmi_gdml **out)
 {
   if (!out) return set_error(VGMI_ERR_INVALID, "vgmi_gdml_read: null output handle");
   *out = nullptr;
   if (!path) return set_error(VGMI_ERR_INVALID, "vgmi_gdml_read: null path");
   if (!(mm_unit > 0) || !std::isfinite(mm_unit))
     return set_error(VGMI_ERR_INVALID, "vgmi_gdml_read: the millimetre unit must be positive");
+  if (flags & ~VGMI_GDML_SPECIALIZED) return set_error(VGMI_ERR_INVALID, "vgmi_gdml_read: unknown flags %d", flags);
   auto *g = new (std::nothrow) vgmi_gdml();
   if (!g) return set_error(VGMI_ERR_INVALID, "vgmi_gdml_read: out of host memory");
   std::string err;
-  if (!gdml::Read(path, mm_unit, g->m, err)) {
+  if (!gdml::Read(path, mm_unit, (flags & VGMI_GDML_SPECIALIZED) != 0, g->m, err)) {
     delete g;
     return set_error(VGMI_ERR_INVALID, "vgmi_gdml_read: %s", err.c_str());
   }

@@ -17,8 +17,9 @@ struct Model {
   int world = -1;
 };
 
-// Reads `path`; lengths are scaled by `mm` per millimetre. False with a message on failure.
-bool Read(std::string const &path, double mm, Model &out, std::string &err);
+// Reads `path`; lengths are scaled by `mm` per millimetre; `specialized` applies GeoManager::MakeInstance's
+// substitutions of a build with VECGEOM_NO_SPECIALIZATION=OFF. False with a message on failure.
+bool Read(std::string const &path, double mm, bool specialized, Model &out, std::string &err);
 
 } // namespace gdml
 } // namespace vgmi

@@ -24,7 +24,9 @@ enum ShapeType : int32_t {
   kHype = 10,
   kTrd = 11,
   kParaboloid = 12,
-  kNumShapeTypes = 13
+  kParallelepiped = 13,
+  kTet = 14,
+  kNumShapeTypes = 15
 };
 
 enum QueryOp : int32_t {
@@ -165,6 +167,22 @@ struct ParaboloidData {
   double rlo, rhi, dz, dx, a, invA, b, k1, k2, rlo2, rhi2;
 };
 
+// Parallelepiped (VecGeom/volumes/ParallelepipedStruct.h:80-124): half-lengths in the oblique frame,
+// the cached tangents, the x and y face normals (the z face's is +z) and the safety scale factors.
+struct ParallelepipedData {
+  double dim[3];
+  double tanAlpha, tanThetaSinPhi, tanThetaCosPhi;
+  double n0[3], n1[3];
+  double ctx, cty;
+};
+
+// Tet (VecGeom/volumes/TetStruct.h:57-90): the corners and the four outward face planes n.p + d.
+struct TetData {
+  double v[4][3];
+  double n[4][3];
+  double d[4];
+};
+
 // Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
 // and a threaded BVH over them live in the variable-length pool:
 //   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
@@ -219,6 +237,8 @@ struct alignas(16) ShapeRecord {
     HypeData hype;
     TrdData trd;
     ParaboloidData para;
+    ParallelepipedData ppd;
+    TetData tet;
     double raw[64];
   } u;
 };

@@ -2,6 +2,7 @@
 // by the reference constructors). Runs once per shape at geometry build time.
 #include "shape_host.h"
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 
 namespace vgmi {
@@ -335,6 +336,67 @@ int MakeParaboloid(double const *a, int n, ShapeRecord &r, std::string &err)
   q.k2            = (q.rhi2 + q.rlo2) * 0.5;
   q.invA          = 1 / q.a;
   q.dx            = q.rhi < q.rlo ? q.rlo : q.rhi;
+  return 0;
+}
+
+// Parallelepiped: UnplacedParallelepiped(dx, dy, dz, alpha, theta, phi) (VecGeom/volumes/
+// UnplacedParallelepiped.h:65-71) and ParallelepipedStruct's SetAlpha, SetThetaAndPhi and ComputeNormals
+// (VecGeom/volumes/ParallelepipedStruct.h:62-124): tangents, face normals, safety scale factors.
+int MakeParallelepiped(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n != 6) return err = "parallelepiped takes dx dy dz alpha theta phi", -1;
+  if (!(a[0] > 0 && a[1] > 0 && a[2] > 0) || !std::isfinite(a[3]) || !std::isfinite(a[4]) || !std::isfinite(a[5]))
+    return err = "parallelepiped needs positive half-lengths and finite angles", -1;
+  r                     = ShapeRecord();
+  r.type                = kParallelepiped;
+  ParallelepipedData &q = r.u.ppd;
+  q.dim[0] = a[0], q.dim[1] = a[1], q.dim[2] = a[2];
+  q.tanAlpha       = std::tan(a[3]);
+  q.tanThetaCosPhi = std::tan(a[4]) * std::cos(a[5]);
+  q.tanThetaSinPhi = std::tan(a[4]) * std::sin(a[5]);
+  // n0 = (tanAlpha, 1, 0) x (tcp, tsp, 1), n1 = (tcp, tsp, 1) x (1, 0, 0); each times 1 / |n|
+  const double vy[3] = {q.tanAlpha, 1., 0.}, vz[3] = {q.tanThetaCosPhi, q.tanThetaSinPhi, 1.}, vx[3] = {1., 0., 0.};
+  auto cross = [](const double *l, const double *rr, double *o) {
+    o[0] = l[1] * rr[2] - l[2] * rr[1];
+    o[1] = l[2] * rr[0] - l[0] * rr[2];
+    o[2] = l[0] * rr[1] - l[1] * rr[0];
+  };
+  cross(vy, vz, q.n0);
+  cross(vz, vx, q.n1);
+  for (double *v : {q.n0, q.n1}) {
+    const double f = 1. / std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] *= f, v[1] *= f, v[2] *= f;
+  }
+  q.ctx = std::fabs(q.n0[0]);
+  q.cty = std::fabs(q.n1[1]);
+  return 0;
+}
+
+// Tet: UnplacedTet(p0, p1, p2, p3) (VecGeom/volumes/UnplacedTet.h:48-49) and TetStruct::CalculateCached
+// (VecGeom/volumes/TetStruct.h:57-90): unit face normals (Vector3D::Unit), flipped to point away from
+// the opposite corner, and the plane offsets.
+int MakeTet(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n != 12) return err = "tet takes its four corners (12 values)", -1;
+  for (int i = 0; i < 12; ++i)
+    if (!std::isfinite(a[i])) return err = "tet corners must be finite", -1;
+  r          = ShapeRecord();
+  r.type     = kTet;
+  TetData &t = r.u.tet;
+  for (int k = 0; k < 4; ++k)
+    for (int c = 0; c < 3; ++c) t.v[k][c] = a[3 * k + c];
+  for (int k = 0; k < 4; ++k) {
+    const double *v0 = t.v[k], *v1 = t.v[(k + 1) % 4], *v2 = t.v[(k + 2) % 4], *v3 = t.v[(k + 3) % 4];
+    const double e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+    const double e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+    double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double m = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2] + DBL_MIN);
+    c[0] /= m, c[1] /= m, c[2] /= m;
+    const double o[3] = {v3[0] - v0[0], v3[1] - v0[1], v3[2] - v0[2]};
+    if (c[0] * o[0] + c[1] * o[1] + c[2] * o[2] > 0) c[0] = -c[0], c[1] = -c[1], c[2] = -c[2];
+    for (int i = 0; i < 3; ++i) t.n[k][i] = c[i];
+    t.d[k] = -(c[0] * v0[0] + c[1] * v0[1] + c[2] * v0[2]);
+  }
   return 0;
 }
 
@@ -917,6 +979,10 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeTrd(params, nparams, r, err);
   case kParaboloid:
     return MakeParaboloid(params, nparams, r, err);
+  case kParallelepiped:
+    return MakeParallelepiped(params, nparams, r, err);
+  case kTet:
+    return MakeTet(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -988,6 +1054,23 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     TrdData const &t = r.u.trd;
     const double mx = t.dx1 < t.dx2 ? t.dx2 : t.dx1, my = t.dy1 < t.dy2 ? t.dy2 : t.dy1;
     lo[0] = -mx, lo[1] = -my, lo[2] = -t.dz, hi[0] = mx, hi[1] = my, hi[2] = t.dz;
+    return;
+  }
+  case kParallelepiped: {
+    // source/UnplacedParallelepiped.cpp:79-89
+    ParallelepipedData const &q = r.u.ppd;
+    const double dx = q.dim[0] + q.dim[1] * std::fabs(q.tanAlpha) + q.dim[2] * std::fabs(q.tanThetaCosPhi);
+    const double dy = q.dim[1] + q.dim[2] * std::fabs(q.tanThetaSinPhi);
+    lo[0] = -dx, lo[1] = -dy, lo[2] = -q.dim[2], hi[0] = dx, hi[1] = dy, hi[2] = q.dim[2];
+    return;
+  }
+  case kTet: {
+    // source/UnplacedTet.cpp:34-50
+    TetData const &t = r.u.tet;
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = hi[c] = t.v[0][c];
+      for (int k = 1; k < 4; ++k) lo[c] = std::min(lo[c], t.v[k][c]), hi[c] = std::max(hi[c], t.v[k][c]);
+    }
     return;
   }
   case kParaboloid: {

@@ -18,7 +18,7 @@ def _declare():
     if getattr(L, "_gdml_declared", False):
         return L
     vp, ci = ctypes.c_void_p, ctypes.c_int
-    for name, args in (("vgmi_gdml_read", [ctypes.c_char_p, ctypes.c_double, ctypes.POINTER(vp)]),
+    for name, args in (("vgmi_gdml_read", [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(vp)]),
                        ("vgmi_gdml_desc", [vp, ctypes.POINTER(_Desc)]),
                        ("vgmi_gdml_name", [vp, ci, ci, ctypes.POINTER(ctypes.c_char_p)]),
                        ("vgmi_gdml_destroy", [vp])):
@@ -35,12 +35,15 @@ def _array(ptr, n, ctype):
     return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctype)), shape=(n,)).copy()
 
 
-def load_gdml(path, mm_unit=MM_UNIT_DEFAULT):
-    """Read a GDML file into a GeometryDesc (lengths in units of 1 / mm_unit millimetres). Raises
+def load_gdml(path, mm_unit=MM_UNIT_DEFAULT, specialized=False):
+    """Read a GDML file into a GeometryDesc (lengths in units of 1 / mm_unit millimetres). Solids are
+    those of the reference's default build (as written); specialized=True applies the
+    GeoManager::MakeInstance substitutions of a build with VECGEOM_NO_SPECIALIZATION=OFF. Raises
     VgmiError for an unsupported solid, a reflected placement or a malformed document."""
     L = _declare()
     h = ctypes.c_void_p()
-    check(L.vgmi_gdml_read(os.fsencode(path), float(mm_unit), ctypes.byref(h)), "vgmi_gdml_read")
+    check(L.vgmi_gdml_read(os.fsencode(path), float(mm_unit), 1 if specialized else 0, ctypes.byref(h)),
+          "vgmi_gdml_read")
     try:
         d = _Desc()
         check(L.vgmi_gdml_desc(h, ctypes.byref(d)), "vgmi_gdml_desc")
