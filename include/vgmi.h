@@ -53,7 +53,9 @@ enum vgmi_shape_type {
   VGMI_TRD = 11,        /* x1 x2 y1 y2 z (or x1 x2 y z, or x y z; a sixth value 1: the Trd1 kind)  volumes/UnplacedTrd.h:51-80 */
   VGMI_PARABOLOID = 12, /* rlo rhi dz (rhi > rlo, as G4Paraboloid)  volumes/UnplacedParaboloid.h:60 */
   VGMI_PARALLELEPIPED = 13, /* dx dy dz alpha theta phi            volumes/UnplacedParallelepiped.h:65-71 */
-  VGMI_TET = 14         /* the four corners p0 p1 p2 p3 (12 values) volumes/UnplacedTet.h:48-49 */
+  VGMI_TET = 14,        /* the four corners p0 p1 p2 p3 (12 values) volumes/UnplacedTet.h:48-49 */
+  VGMI_ELLIPTICALTUBE = 15, /* dx dy dz                             volumes/UnplacedEllipticalTube.h:51 */
+  VGMI_ELLIPSOID = 16   /* dx dy dz [zBottomCut zTopCut]           volumes/UnplacedEllipsoid.h:62 */
 };
 
 /* Query methods of VUnplacedVolume / VPlacedVolume (volumes/UnplacedVolume.h:88-223). */

@@ -13,6 +13,8 @@
 //  12 paraboloid  rlo rhi dz                                 (volumes/UnplacedParaboloid.h:60)
 //  13 parallelepiped dx dy dz alpha theta phi               (volumes/UnplacedParallelepiped.h:65-71)
 //  14 tet         p0 p1 p2 p3 (12 doubles)                    (volumes/UnplacedTet.h:48-49)
+//  15 elliptical tube dx dy dz                               (volumes/UnplacedEllipticalTube.h:51)
+//  16 ellipsoid   dx dy dz [zBottomCut zTopCut]               (volumes/UnplacedEllipsoid.h:62; cuts default 0)
 //   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
 //                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
 #include "shapes.h"
@@ -82,6 +84,11 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 3 ? new Paraboloid(a[0], a[1], a[2]) : nullptr;
   case 13:
     return n >= 6 ? new Parallelepiped(a[0], a[1], a[2], a[3], a[4], a[5]) : nullptr;
+  case 15:
+    return n >= 3 ? new EllipticalTube(a[0], a[1], a[2]) : nullptr;
+  case 16:
+    return n == 3 ? new Ellipsoid(a[0], a[1], a[2], 0., 0.)
+                  : (n >= 5 ? new Ellipsoid(a[0], a[1], a[2], a[3], a[4]) : nullptr);
   case 14:
     return n >= 12 ? new Tet(V3(a[0], a[1], a[2]), V3(a[3], a[4], a[5]), V3(a[6], a[7], a[8]), V3(a[9], a[10], a[11]))
                    : nullptr;

@@ -284,6 +284,41 @@ public:
   double MaxPlaneDistance(V3 const &p) const;
 };
 
+// ---------------------------------------------------------------- EllipticalTube
+// source/UnplacedEllipticalTube.cpp:28-71 (CheckParameters: cached scale factors and coefficients),
+// VecGeom/volumes/kernel/EllipticalTubeImplementation.h:61-300
+class EllipticalTube : public Shape {
+public:
+  double dx, dy, dz, rsph, ddx, ddy, sx, sy, r, q1, q2, scratch;
+  EllipticalTube(double dx, double dy, double dz);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
+  void Extent(V3 &lo, V3 &hi) const override { lo = V3(-dx, -dy, -dz), hi = V3(dx, dy, dz); }
+  double Safety(V3 const &p) const;
+};
+
+// ---------------------------------------------------------------- Ellipsoid
+// source/UnplacedEllipsoid.cpp:27-116 (CheckParameters), VecGeom/volumes/kernel/EllipsoidImplementation.h:61-318
+class Ellipsoid : public Shape {
+public:
+  double dx, dy, dz, zBottomCut, zTopCut, xmax, ymax, rsph, r, sx, sy, sz, scZMidCut, scZDimCut, q1, q2;
+  Ellipsoid(double dx, double dy, double dz, double zBottomCut, double zTopCut);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
+  void Extent(V3 &lo, V3 &hi) const override { lo = V3(-xmax, -ymax, zBottomCut), hi = V3(xmax, ymax, zTopCut); }
+  double Safety(V3 const &p) const;
+};
+
 // ---------------------------------------------------------------- Cone (UniversalCone)
 // VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
 // ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820

@@ -361,7 +361,7 @@ def skip_reason(body, shapes, pending, unknown_results):
     return "not a query assert (capacity, surface area, getters, bounding box)"
 
 
-def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None, getters=None):
+def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None, getters=None, setters=None):
     text = open(path).read()
     vecs, scalars, shapes, pending, arrays, extents, logicals = {}, {}, {}, {}, {}, {}, {}
     global CURRENT_SHAPES, CURRENT_GETTERS
@@ -386,6 +386,11 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None, getters=
                     raise Unsupported(a)
                 vals.extend(arrays[a])
             else:
+                ma = re.fullmatch(r"(\w+)\s*=\s*(.+)", a)  # an argument that also assigns: Shape(n, a = 3., ..)
+                if ma:
+                    scalars[ma.group(1)] = eval_scalar(ma.group(2), scalars)
+                    vals.append(scalars[ma.group(1)])
+                    continue
                 v = eval_expr(a, scalars, vecs)  # a vector argument (Tet's corners) adds its components
                 vals.extend(v if isinstance(v, list) else [v])
         return vals
@@ -824,6 +829,18 @@ def extract(path, shape_t, vec_type="Vec_t", unplaced=None, stats=None, getters=
             except Unsupported:
                 vecs.pop(m.group(1), None)
             continue
+        # a setter of a known shape changes its parameters (a setter the fixture does not describe, or
+        # arguments the evaluator cannot reach, make the shape unknown)
+        m = re.match(r"(\w+)\s*(?:\.|->)\s*(Set\w*)\s*\((.*)\)$", st)
+        if m and m.group(1) in shapes:
+            f = (setters or {}).get(m.group(2))
+            try:
+                if f is None:
+                    raise Unsupported(m.group(2))
+                shapes[m.group(1)] = f(list(shapes[m.group(1)]), shape_params(m.group(3)))
+            except Unsupported:
+                shapes.pop(m.group(1), None)
+            continue
         # anything else that mutates a known vector or scalar makes it unknown
         m = re.match(r"(\w+)\s*(?:\.\s*\w+\(\))?\s*(?:[-+*/]?=)", st)
         if m:
@@ -850,8 +867,18 @@ SHAPES = {
     "paraboloid": ("TestParaboloid.cpp", r"Paraboloid_t", (3,), r"UnplacedParaboloid"),
     "parallelepiped": ("TestParallelepiped.cpp", r"Parallelepiped_t", (6,), r"UnplacedParallelepiped"),
     "tet": ("TestTet.cpp", r"Tet_t", (12,), r"UnplacedTet"),
+    "ellipticaltube": ("TestEllipticalTube.cpp", r"EllipticalTube_t", (3,), r"UnplacedEllipticalTube"),
+    "ellipsoid": ("TestEllipsoid.cpp", r"Ellipsoid_t", (5,), r"UnplacedEllipsoid"),
 }
 
+
+# Setters of a shape class a test calls between queries: the constructor parameters they replace
+# (UnplacedEllipsoid::SetSemiAxes / SetZCuts, UnplacedEllipticalTube::SetParameters / SetDx / SetDy / SetDz).
+SETTERS = {
+    "ellipsoid": {"SetSemiAxes": lambda p, a: a[:3] + p[3:], "SetZCuts": lambda p, a: p[:3] + a[:2]},
+    "ellipticaltube": {"SetParameters": lambda p, a: a[:3], "SetDx": lambda p, a: [a[0]] + p[1:],
+                       "SetDy": lambda p, a: p[:1] + [a[0]] + p[2:], "SetDz": lambda p, a: p[:2] + [a[0]]},
+}
 
 # Getters of a shape class a test calls while building its points: ParallelepipedStruct's cached
 # tangents (VecGeom/volumes/ParallelepipedStruct.h:80-88), computed as the reference computes them.
@@ -885,7 +912,7 @@ if __name__ == "__main__":
         fn, pat, counts, unplaced = SHAPES[name]
         st = {"source": "test/unit_tests/%s" % fn, "asserts": 0, "extracted": 0, "skipped": {}}
         recs = extract(os.path.join(REF_ROOT, "test", "unit_tests", fn), pat, unplaced=unplaced, stats=st,
-                       getters=GETTERS.get(name))
+                       getters=GETTERS.get(name), setters=SETTERS.get(name))
         kept = []
         for r in recs:
             if params_ok(counts, r["shape"]):

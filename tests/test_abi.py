@@ -58,7 +58,10 @@ def test_extent_matches_oracle():
              ("parallelepiped", [20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(45)]),
              ("parallelepiped", [8.0, 5.0, 12.0, math.radians(-20), math.radians(15), math.radians(200)]),
              ("tet", [0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0]),
-             ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0])]
+             ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
+             ("ellipticaltube", [5.0, 10.0, 20.0]), ("ellipsoid", [3.0, 4.0, 5.0, -4.5, 3.5]),
+             ("ellipsoid", [6.0, 9.0, 4.0, 1.0, 4.0]), ("ellipsoid", [6.0, 9.0, 4.0, -4.0, -1.0]),
+             ("ellipsoid", [10.0, 7.0, 12.0])]
     # shapes with a variable-length block need a device for creation: extent from the host record
     cases = [c for c in cases if c[0] != "polyhedron"]
     for kind, params in cases:

@@ -21,7 +21,7 @@ FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gdml")
 FILES = ["allSolids.gdml", "aRotation.gdml", "aPosition.gdml", "twoBoxes.gdml", "oneBox.gdml", "oneConeSegment.gdml",
          "oneGeneralTrapezoid.gdml", "oneHyperbolicTube.gdml", "oneOrb.gdml", "oneParaboloid.gdml",
          "onePolycone.gdml", "onePolyhedron.gdml", "oneSphere.gdml", "oneTesselatedSolid.gdml", "oneTorus.gdml",
-         "oneTrapezoid.gdml", "oneTube.gdml"]
+         "oneTrapezoid.gdml", "oneTube.gdml", "oneEllipsoid.gdml"]
 KIND = {v: k for k, v in __import__("oracle.pyoracle", fromlist=["SHAPE_TYPES"]).SHAPE_TYPES.items()}
 
 
@@ -55,9 +55,10 @@ def _geometries():
 
 
 def planar_solids():
-    """Parallelepipeds and tets placed with rotations in a box world, one parallelepiped holding a tet."""
+    """Parallelepipeds, tets, elliptical tubes and ellipsoids placed with rotations in a box world, one
+    parallelepiped holding a tet, one elliptical tube holding a cut ellipsoid."""
     from tests.test_nav_path import _rot
-    from vecgeom_amd.shapes import BOX, PARALLELEPIPED, TET
+    from vecgeom_amd.shapes import BOX, ELLIPSOID, ELLIPTICALTUBE, PARALLELEPIPED, TET
     g = geometry.GeometryDesc()
     w = g.add_logical(g.add_shape(BOX, [300.0, 300.0, 300.0]), "world")
     g.set_world(w)
@@ -71,6 +72,12 @@ def planar_solids():
         g.place(w, para, (float(x), float(y), 20.0 * k) + _rot(axis, ang))
     for k, (x, y, axis, ang) in enumerate([(150, 150, "z", 1.1), (0, 0, "x", 0.2), (0, -180, "y", -1.3)]):
         g.place(w, tet, (float(x), float(y), -30.0 * k) + _rot(axis, ang))
+    eltube = g.add_logical(g.add_shape(ELLIPTICALTUBE, [45.0, 25.0, 60.0]), "elliptical tube")
+    ell = g.add_logical(g.add_shape(ELLIPSOID, [20.0, 15.0, 40.0, -30.0, 25.0]), "ellipsoid")
+    g.place(eltube, ell, (8.0, 2.0, -5.0) + _rot("z", 0.5))
+    g.place(w, eltube, (150.0, 0.0, 100.0) + _rot("x", 0.6))
+    g.place(w, eltube, (-150.0, 10.0, -120.0) + _rot("y", 1.2))
+    g.place(w, ell, (80.0, 200.0, 0.0) + _rot("z", -0.4))
     return g
 
 

@@ -77,7 +77,8 @@ def test_normal_keeps_caller_vectors_and_sizes():
 
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "torus2",
-                                   "tessellated", "hype", "trd", "paraboloid"])
+                                   "tessellated", "hype", "trd", "paraboloid", "ellipticaltube",
+                                   "ellipsoid"])
 def test_gpu_reference_normal_kats(shape):
     """The reference's Normal asserts (test/unit_tests/Test<Shape>.cpp), evaluated on the GPU; chained
     points p0 + Dist * v take Dist from the GPU's own DistanceToIn/Out."""

@@ -78,6 +78,11 @@ CASES = [
     ("parallelepiped", [10.0, 10.0, 10.0, 0.0, 0.0, 0.0]),
     ("tet", [0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0]),
     ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
+    ("ellipticaltube", [5.0, 10.0, 20.0]),
+    ("ellipticaltube", [30.0, 12.0, 8.0]),
+    ("ellipsoid", [3.0, 4.0, 5.0, -4.5, 3.5]),
+    ("ellipsoid", [10.0, 7.0, 12.0]),
+    ("ellipsoid", [6.0, 9.0, 4.0, 1.0, 4.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing then stops at a neighbouring
@@ -125,7 +130,7 @@ def test_query_all_matches_single_methods(case):
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
                                    "torus2", "tessellated", "hype", "trd", "paraboloid", "parallelepiped",
-                                   "tet"])
+                                   "tet", "ellipticaltube", "ellipsoid"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

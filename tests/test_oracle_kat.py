@@ -10,7 +10,7 @@ from oracle.pyoracle import OracleShape
 from tests import kat_util
 
 SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated",
-          "hype", "trd", "paraboloid", "parallelepiped", "tet"]
+          "hype", "trd", "paraboloid", "parallelepiped", "tet", "ellipticaltube", "ellipsoid"]
 
 
 def _oracle_distance(kind):

@@ -44,6 +44,8 @@ CASES = [
     ("paraboloid", [6.0, 9.0, 10.0]),
     ("parallelepiped", [20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(45)]),
     ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
+    ("ellipticaltube", [5.0, 10.0, 20.0]),
+    ("ellipsoid", [3.0, 4.0, 5.0, -4.5, 3.5]),
 ]
 
 

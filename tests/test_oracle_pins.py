@@ -297,6 +297,32 @@ for _t in ([20.0, 30.0, 40.0, math.radians(30), math.radians(30), math.radians(4
 for _t in ([0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0, 0.0, 0.0, 2.0, 0.0],
            [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]):
     CASES.append(("tet", _t, margin_convex(np.array(_t).reshape(4, 3))))
+
+
+def margin_ellipsoid(params, p):
+    """x^2/a^2 + y^2/b^2 + z^2/c^2 <= 1 between the z cuts: min(a, b, c) (1 - |(x/a, y/b, z/c)|) is
+    1-Lipschitz (the scaled norm's gradient is at most 1 / min(a, b, c)) and vanishes on the surface;
+    the cuts clamped to [-c, c] as the solid's constructor does (both 0: no cut)."""
+    a, b, c = params[:3]
+    zb, zt = (params[3], params[4]) if len(params) == 5 else (0.0, 0.0)
+    if (zb == 0 and zt == 0) or zb >= c or zt <= -c or zb >= zt:
+        zb, zt = -c, c
+    zb, zt = max(zb, -c), min(zt, c)
+    q = np.sqrt((p[0] / a) ** 2 + (p[1] / b) ** 2 + (p[2] / c) ** 2)
+    return np.minimum(min(a, b, c) * (1 - q), np.minimum(zt - p[2], p[2] - zb))
+
+
+def margin_eltube(params, p):
+    """An elliptical cylinder |z| <= dz: min(dx, dy) (1 - |(x/dx, y/dy)|), 1-Lipschitz as above."""
+    a, b, dz = params
+    q = np.sqrt((p[0] / a) ** 2 + (p[1] / b) ** 2)
+    return np.minimum(min(a, b) * (1 - q), dz - np.abs(p[2]))
+
+
+for _t in ([5.0, 10.0, 20.0], [30.0, 12.0, 8.0]):
+    CASES.append(("ellipticaltube", _t, margin_eltube))
+for _t in ([3.0, 4.0, 5.0, -4.5, 3.5], [10.0, 7.0, 12.0], [6.0, 9.0, 4.0, 1.0, 4.0], [6.0, 6.0, 6.0, -2.0, 5.0]):
+    CASES.append(("ellipsoid", _t, margin_ellipsoid))
 IDS = ["%s-%d" % (c[0], i) for i, c in enumerate(CASES)]
 
 

@@ -147,6 +147,8 @@ static int method_groups(int type, MethodGroup g[3])
   case kParaboloid:
   case kParallelepiped:
   case kTet:
+  case kEllipticalTube:
+  case kEllipsoid:
     g[0] = {kMAll, 2};
     return 1;
   case kCone:

@@ -683,5 +683,52 @@ VGMI_DEV bool Normal(TetData const &s, V3d const &p, V3d &n, bool &set)
   return false;
 }
 
+// EllipticalTubeImplementation::NormalKernel (VecGeom/volumes/kernel/EllipticalTubeImplementation.h:233-268)
+VGMI_DEV bool Normal(EllipticalTubeData const &s, V3d const &point, V3d &n, bool &set)
+{
+  V3d normal(0., 0., 0.);
+  const double x = point.x * s.sx, y = point.y * s.sy;
+  double distR = s.q1 * (x * x + y * y) - s.q2;
+  if (vabs(distR) <= kHalfTolerance) normal = Unit(V3d(point.x * s.ddy, point.y * s.ddx, 0.));
+  const double distZ = vabs(point.z) - s.dz;
+  if (vabs(distZ) <= kHalfTolerance) normal.z = Sign(point.z);
+  if (normal.Mag2() > 1.) normal = Unit(normal);
+  set = true;
+  if (normal.Mag2() > 0.) {
+    n = normal;
+    return true;
+  }
+  normal.z = Sign(point.z);
+  distR    = vsqrt(x * x + y * y) - s.r;
+  if (distR > distZ && (x * x + y * y) > 0.) normal = Unit(V3d(point.x * s.ddy, point.y * s.ddx, 0.));
+  n = normal;
+  return false;
+}
+
+// EllipsoidImplementation::NormalKernel (VecGeom/volumes/kernel/EllipsoidImplementation.h:245-287)
+VGMI_DEV bool Normal(EllipsoidData const &s, V3d const &point, V3d &n, bool &set)
+{
+  V3d normal(0., 0., 0.);
+  const double px = point.x, py = point.y, pz = point.z;
+  const double A = s.dx, B = s.dy, C = s.dz;
+  const double x = px * s.sx, y = py * s.sy, z = pz * s.sz;
+  const double mag2 = x * x + y * y + z * z;
+  double distR      = s.q1 * mag2 - s.q2;
+  if (vabs(distR) <= kHalfTolerance) normal = Unit(V3d(px / (A * A), py / (B * B), pz / (C * C)));
+  const double distZ = vabs(z - s.scZMidCut) - s.scZDimCut;
+  if (vabs(distZ) <= kHalfTolerance) normal.z = normal.z + Sign(z - s.scZMidCut);
+  if (normal.Mag2() > 1.) normal = Unit(normal);
+  set = true;
+  if (normal.Mag2() > 0.) {
+    n = normal;
+    return true;
+  }
+  normal.z = Sign(z - s.scZMidCut);
+  distR    = vsqrt(mag2) - s.r;
+  if (distR > distZ && mag2 > 0.) normal = Unit(V3d(px / (A * A), py / (B * B), pz / (C * C)));
+  n = normal;
+  return false;
+}
+
 } // namespace nrm
 } // namespace vgmi

@@ -3,6 +3,7 @@
 #pragma once
 #include "dev_box.h"
 #include "dev_cone.h"
+#include "dev_ellipsoids.h"
 #include "dev_hype.h"
 #include "dev_orb.h"
 #include "dev_para_tet.h"
@@ -35,7 +36,9 @@
   X(kTrd, TrdData, trd, trd)                                                                                         \
   X(kParaboloid, ParaboloidData, para, para)                                                                         \
   X(kParallelepiped, ParallelepipedData, ppd, ppd)                                                                   \
-  X(kTet, TetData, tet, tet)
+  X(kTet, TetData, tet, tet)                                                                                         \
+  X(kEllipticalTube, EllipticalTubeData, eltu, eltu)                                                                 \
+  X(kEllipsoid, EllipsoidData, ells, ells)
 
 namespace vgmi {
 

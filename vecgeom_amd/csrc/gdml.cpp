@@ -518,6 +518,11 @@ private:
     } else if (n == "tessellated") {
       type = kTessellated;
       if (!tessellated(e, p)) return false;
+    } else if (n == "eltube") { // processElTube (Middleware.cpp:747-761): semi-axes, not halved
+      type = kEllipticalTube, p = {l * num(e, "dx"), l * num(e, "dy"), l * num(e, "dz")};
+    } else if (n == "ellipsoid") { // processEllipsoid (Middleware.cpp:948-965)
+      type = kEllipsoid;
+      p    = {l * num(e, "ax"), l * num(e, "by"), l * num(e, "cz"), l * num(e, "zcut1"), l * num(e, "zcut2")};
     } else if (n == "para") { // processParallelepiped (Middleware.cpp:967-988): x y z halved
       type = kParallelepiped;
       p    = {l * num(e, "x") / 2., l * num(e, "y") / 2., l * num(e, "z") / 2., a * num(e, "alpha"),

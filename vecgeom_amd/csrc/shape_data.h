@@ -26,7 +26,9 @@ enum ShapeType : int32_t {
   kParaboloid = 12,
   kParallelepiped = 13,
   kTet = 14,
-  kNumShapeTypes = 15
+  kEllipticalTube = 15,
+  kEllipsoid = 16,
+  kNumShapeTypes = 17
 };
 
 enum QueryOp : int32_t {
@@ -183,6 +185,18 @@ struct TetData {
   double d[4];
 };
 
+// EllipticalTube (VecGeom/volumes/EllipticalTubeStruct.h; cached by source/UnplacedEllipticalTube.cpp:28-62):
+// the ellipse scaled to a circle of radius r = min(dx, dy); dist ~ q1 (x^2 + y^2) - q2.
+struct EllipticalTubeData {
+  double dx, dy, dz, rsph, ddx, ddy, sx, sy, r, q1, q2, scratch;
+};
+
+// Ellipsoid (VecGeom/volumes/EllipsoidStruct.h; cached by source/UnplacedEllipsoid.cpp:40-116): scaled to a
+// sphere of radius r = min(dx, dy, dz), z cuts (also scaled), x-y extent.
+struct EllipsoidData {
+  double dx, dy, dz, zBottomCut, zTopCut, xmax, ymax, rsph, r, sx, sy, sz, scZMidCut, scZDimCut, q1, q2;
+};
+
 // Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
 // and a threaded BVH over them live in the variable-length pool:
 //   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
@@ -239,6 +253,8 @@ struct alignas(16) ShapeRecord {
     ParaboloidData para;
     ParallelepipedData ppd;
     TetData tet;
+    EllipticalTubeData eltu;
+    EllipsoidData ells;
     double raw[64];
   } u;
 };

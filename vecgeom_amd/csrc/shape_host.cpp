@@ -400,6 +400,76 @@ int MakeTet(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
+// EllipticalTube: UnplacedEllipticalTube(dx, dy, dz) -> SetParameters -> CheckParameters
+// (source/UnplacedEllipticalTube.cpp:20-62). Semi-axes below 2 kTolerance reset all three to it, as there
+// (whose test reads dy twice and never dz).
+int MakeEllipticalTube(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n != 3) return err = "elliptical tube takes dx dy dz", -1;
+  if (!std::isfinite(a[0]) || !std::isfinite(a[1]) || !std::isfinite(a[2]))
+    return err = "elliptical tube semi-axes must be finite", -1;
+  r                     = ShapeRecord();
+  r.type                = kEllipticalTube;
+  EllipticalTubeData &e = r.u.eltu;
+  e.dx = a[0], e.dy = a[1], e.dz = a[2];
+  const double tol = 2. * 1e-9;
+  if (e.dx < tol || e.dy < tol || e.dy < tol) e.dx = e.dy = e.dz = tol;
+  const double R = e.dx < e.dy ? e.dx : e.dy;
+  e.rsph    = std::sqrt(e.dx * e.dx + e.dy * e.dy + e.dz * e.dz);
+  e.ddx     = e.dx * e.dx;
+  e.ddy     = e.dy * e.dy;
+  e.sx      = R / e.dx;
+  e.sy      = R / e.dy;
+  e.r       = R;
+  e.q1      = 0.5 / R;
+  e.q2      = 0.5 * (R + 0.5e-9 * 0.5e-9 / R);
+  e.scratch = 2. * R * R * DBL_EPSILON;
+  return 0;
+}
+
+// Ellipsoid: UnplacedEllipsoid(dx, dy, dz, zBottomCut = 0, zTopCut = 0) -> CheckParameters
+// (source/UnplacedEllipsoid.cpp:27-116): semi-axes below 2 kTolerance reset as for the elliptical tube;
+// cuts both 0 or invalid become -dz, +dz, then clamped to [-dz, dz]; the x-y extent shrinks when both
+// cuts lie on one side of z = 0.
+int MakeEllipsoid(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n != 3 && n != 5) return err = "ellipsoid takes dx dy dz [zBottomCut zTopCut]", -1;
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return err = "ellipsoid parameters must be finite", -1;
+  r                = ShapeRecord();
+  r.type           = kEllipsoid;
+  EllipsoidData &e = r.u.ells;
+  e.dx = a[0], e.dy = a[1], e.dz = a[2];
+  e.zBottomCut = n == 5 ? a[3] : 0., e.zTopCut = n == 5 ? a[4] : 0.;
+  const double tol = 2. * 1e-9;
+  if (e.dx < tol || e.dy < tol || e.dy < tol) e.dx = e.dy = e.dz = tol;
+  const double A = e.dx, B = e.dy, C = e.dz;
+  if (e.zBottomCut == 0 && e.zTopCut == 0) e.zBottomCut = -C, e.zTopCut = C;
+  if (e.zBottomCut >= C || e.zTopCut <= -C || e.zBottomCut >= e.zTopCut) e.zBottomCut = -C, e.zTopCut = C;
+  e.zBottomCut = e.zBottomCut > -e.dz ? e.zBottomCut : -e.dz;
+  e.zTopCut    = e.zTopCut < e.dz ? e.zTopCut : e.dz;
+  e.xmax = A, e.ymax = B;
+  if (e.zBottomCut > 0.) {
+    const double ratio = e.zBottomCut / C, scale = std::sqrt((1. - ratio) * (1 + ratio));
+    e.xmax = A * scale, e.ymax = B * scale;
+  }
+  if (e.zTopCut < 0.) {
+    const double ratio = e.zTopCut / C, scale = std::sqrt((1. - ratio) * (1 + ratio));
+    e.xmax = A * scale, e.ymax = B * scale;
+  }
+  e.rsph = std::max(std::max(A, B), C);
+  e.r    = std::min(std::min(A, B), C);
+  e.sx   = e.r / A;
+  e.sy   = e.r / B;
+  e.sz   = e.r / C;
+  const double bot = e.zBottomCut * e.sz, top = e.zTopCut * e.sz;
+  e.scZMidCut = 0.5 * (top + bot);
+  e.scZDimCut = 0.5 * (top - bot);
+  e.q1        = 0.5 / e.r;
+  e.q2        = 0.5 * (e.r + 0.5e-9 * 0.5e-9 / e.r);
+  return 0;
+}
+
 // Cone: VecGeom/volumes/ConeStruct.h:415-426 (constructor), :309-346 (angle normalisation: the
 // wedge keeps the raw angles, the phi-plane vectors use the normalised ones) and :76-140 (cache).
 int MakeCone(double const *a, int n, ShapeRecord &r, std::string &err)
@@ -983,6 +1053,10 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeParallelepiped(params, nparams, r, err);
   case kTet:
     return MakeTet(params, nparams, r, err);
+  case kEllipticalTube:
+    return MakeEllipticalTube(params, nparams, r, err);
+  case kEllipsoid:
+    return MakeEllipsoid(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -1062,6 +1136,18 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     const double dx = q.dim[0] + q.dim[1] * std::fabs(q.tanAlpha) + q.dim[2] * std::fabs(q.tanThetaCosPhi);
     const double dy = q.dim[1] + q.dim[2] * std::fabs(q.tanThetaSinPhi);
     lo[0] = -dx, lo[1] = -dy, lo[2] = -q.dim[2], hi[0] = dx, hi[1] = dy, hi[2] = q.dim[2];
+    return;
+  }
+  case kEllipticalTube: {
+    // source/UnplacedEllipticalTube.cpp:64-71
+    EllipticalTubeData const &e = r.u.eltu;
+    lo[0] = -e.dx, lo[1] = -e.dy, lo[2] = -e.dz, hi[0] = e.dx, hi[1] = e.dy, hi[2] = e.dz;
+    return;
+  }
+  case kEllipsoid: {
+    // source/UnplacedEllipsoid.cpp:167-171
+    EllipsoidData const &e = r.u.ells;
+    lo[0] = -e.xmax, lo[1] = -e.ymax, lo[2] = e.zBottomCut, hi[0] = e.xmax, hi[1] = e.ymax, hi[2] = e.zTopCut;
     return;
   }
   case kTet: {
