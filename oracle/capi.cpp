@@ -15,6 +15,7 @@
 //  14 tet         p0 p1 p2 p3 (12 doubles)                    (volumes/UnplacedTet.h:48-49)
 //  15 elliptical tube dx dy dz                               (volumes/UnplacedEllipticalTube.h:51)
 //  16 ellipsoid   dx dy dz [zBottomCut zTopCut]               (volumes/UnplacedEllipsoid.h:62; cuts default 0)
+//  17 gentrap     x[8] y[8] dz (planar ones)                  (volumes/UnplacedGenTrap.h:46)
 //   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
 //                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
 #include "shapes.h"
@@ -86,6 +87,15 @@ Shape *MakeShape(int type, const double *a, int n)
     return n >= 6 ? new Parallelepiped(a[0], a[1], a[2], a[3], a[4], a[5]) : nullptr;
   case 15:
     return n >= 3 ? new EllipticalTube(a[0], a[1], a[2]) : nullptr;
+  case 17: {
+    if (n < 17) return nullptr;
+    auto *g = new GenTrap(a, a + 8, a[16]);
+    if (!g->ok) {
+      delete g;
+      return nullptr;
+    }
+    return g;
+  }
   case 16:
     return n == 3 ? new Ellipsoid(a[0], a[1], a[2], 0., 0.)
                   : (n >= 5 ? new Ellipsoid(a[0], a[1], a[2], a[3], a[4]) : nullptr);

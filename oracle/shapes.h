@@ -319,6 +319,34 @@ public:
   double Safety(V3 const &p) const;
 };
 
+// ---------------------------------------------------------------- GenTrap (planar)
+// VecGeom/volumes/GenTrapStruct.h:57-190 (vertex checks, clockwise reversal, twist), the scalar CPU path of
+// VecGeom/volumes/kernel/GenTrapImplementation.h:330-608, which hands planar traps to the
+// TessellatedSection helper (VecGeom/volumes/TessellatedSection.h, TessellatedCluster.h, Tile.h with one
+// facet per cluster), and NormalKernel (:610-735) with SecondOrderSurfaceShell's face normals
+// (VecGeom/volumes/SecondOrderSurfaceShell.h:65-122). Twisted traps are not built (ok = false).
+class GenTrap : public Shape {
+public:
+  double dz, halfInverseDz;
+  V3 vertices[8];
+  double vx[8], vy[8], connX[4], connY[4];
+  bool twisted = false, ok = false;
+  int nfacets = 0;
+  V3 fnormal[4];
+  double fdist[4];
+  V3 shellNormal[4];
+  GenTrap(const double *x, const double *y, double h);
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  double DistPlane(int i, V3 const &p) const { return p.Dot(fnormal[i]) + fdist[i]; }
+};
+
 // ---------------------------------------------------------------- Cone (UniversalCone)
 // VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
 // ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820

@@ -869,6 +869,7 @@ SHAPES = {
     "tet": ("TestTet.cpp", r"Tet_t", (12,), r"UnplacedTet"),
     "ellipticaltube": ("TestEllipticalTube.cpp", r"EllipticalTube_t", (3,), r"UnplacedEllipticalTube"),
     "ellipsoid": ("TestEllipsoid.cpp", r"Ellipsoid_t", (5,), r"UnplacedEllipsoid"),
+    "gentrap": ("TestGenTrap.cpp", r"SimpleGenTrap", (17,), r"UnplacedGenTrap"),
 }
 
 

@@ -23,6 +23,21 @@ def approx_equal(x, y, tol=1e-6):
     return abs(x - y) / (abs(x) + abs(y)) < tol
 
 
+def gentrap_twisted(params):
+    """True for a twisted GenTrap (VecGeom/volumes/GenTrapStruct.h:246-276: some lateral face whose bottom
+    and top edges are not parallel); params = x[8] y[8] dz."""
+    x, y = params[:8], params[8:16]
+    for i in range(4):
+        j = (i + 1) % 4
+        dx1, dy1 = x[j] - x[i], y[j] - y[i]
+        dx2, dy2 = x[j + 4] - x[i + 4], y[j + 4] - y[i + 4]
+        if (dx1 == 0 and dy1 == 0) or (dx2 == 0 and dy2 == 0):
+            continue
+        if abs(dy1 * dx2 - dx1 * dy2) >= 1e-9:
+            return True
+    return False
+
+
 def load(name):
     with open(os.path.join(GOLDEN, "kat_%s.json" % name)) as f:
         return json.load(f)["records"]

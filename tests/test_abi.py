@@ -61,7 +61,9 @@ def test_extent_matches_oracle():
              ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
              ("ellipticaltube", [5.0, 10.0, 20.0]), ("ellipsoid", [3.0, 4.0, 5.0, -4.5, 3.5]),
              ("ellipsoid", [6.0, 9.0, 4.0, 1.0, 4.0]), ("ellipsoid", [6.0, 9.0, 4.0, -4.0, -1.0]),
-             ("ellipsoid", [10.0, 7.0, 12.0])]
+             ("ellipsoid", [10.0, 7.0, 12.0]),
+             ("gentrap", [-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, -2.0, 2.0, 2.0, -2.0, 5.0]),
+             ("gentrap", [-4.0, 5.0, 6.0, -3.0, -1.4, 4.0, 4.6, -0.8, -5.0, -4.0, 4.0, 3.0, -2.5, -1.9, 2.9, 2.3, 8.0])]
     # shapes with a variable-length block need a device for creation: extent from the host record
     cases = [c for c in cases if c[0] != "polyhedron"]
     for kind, params in cases:

@@ -119,10 +119,24 @@ def test_rotation_is_the_passive_xyz_rotation():
     assert np.allclose(r @ r.T, np.eye(3), atol=1e-15) and abs(np.linalg.det(r) - 1) < 1e-15
 
 
-@pytest.mark.parametrize("name,word", [("oneArbitraryTrapezoid.gdml", "arb8")])
-def test_unsupported_solids_fail(name, word):
-    with pytest.raises(VgmiError, match=word):
-        load_gdml(os.path.join(FIX, name))
+def test_unsupported_solids_fail(tmp_path):
+    text = BODY % ('<cutTube name="c" z="10" rmin="0" rmax="5" startphi="0" deltaphi="360" aunit="deg" '
+                   'lowX="0" lowY="0" lowZ="-1" highX="0" highY="0" highZ="1"/>',
+                   "<volume name='W'><solidref ref='c'/></volume>", "W")
+    with pytest.raises(VgmiError, match="cutTube"):
+        load_gdml(_write(tmp_path, text))
+
+
+def test_arb8_vertices_carry_no_unit(tmp_path):
+    # processGenTrap reads v1x..v8y without the length unit (DECLAREANDGETPLAINVAR) and dz with it; the
+    # reference's oneArbitraryTrapezoid.gdml names two solids "world", which the reader refuses
+    with pytest.raises(VgmiError, match="defined twice"):
+        load_gdml(os.path.join(FIX, "oneArbitraryTrapezoid.gdml"))
+    arb8 = ('<arb8 name="a" v1x="-3" v1y="-3" v2x="-3" v2y="3" v3x="3" v3y="3" v4x="3" v4y="-3" v5x="-2" '
+            'v5y="-2" v6x="-2" v6y="2" v7x="2" v7y="2" v8x="2" v8y="-2" dz="5" lunit="cm"/>')
+    g = load_gdml(_write(tmp_path, BODY % (arb8, "<volume name='W'><solidref ref='a'/></volume>", "W")))
+    assert g.shapes[0] == (S.GENTRAP, [-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, -2.0, 2.0,
+                                       2.0, -2.0, 5 * CM])
 
 
 def _write(tmp_path, text, name="t.gdml"):

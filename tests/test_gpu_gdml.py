@@ -55,10 +55,11 @@ def _geometries():
 
 
 def planar_solids():
-    """Parallelepipeds, tets, elliptical tubes and ellipsoids placed with rotations in a box world, one
-    parallelepiped holding a tet, one elliptical tube holding a cut ellipsoid."""
+    """Parallelepipeds, tets, elliptical tubes, ellipsoids and planar generic trapezoids placed with
+    rotations in a box world, one parallelepiped holding a tet, one elliptical tube holding a cut
+    ellipsoid, one generic trapezoid holding a tet."""
     from tests.test_nav_path import _rot
-    from vecgeom_amd.shapes import BOX, ELLIPSOID, ELLIPTICALTUBE, PARALLELEPIPED, TET
+    from vecgeom_amd.shapes import BOX, ELLIPSOID, ELLIPTICALTUBE, GENTRAP, PARALLELEPIPED, TET
     g = geometry.GeometryDesc()
     w = g.add_logical(g.add_shape(BOX, [300.0, 300.0, 300.0]), "world")
     g.set_world(w)
@@ -78,6 +79,12 @@ def planar_solids():
     g.place(w, eltube, (150.0, 0.0, 100.0) + _rot("x", 0.6))
     g.place(w, eltube, (-150.0, 10.0, -120.0) + _rot("y", 1.2))
     g.place(w, ell, (80.0, 200.0, 0.0) + _rot("z", -0.4))
+    gtra = g.add_logical(g.add_shape(GENTRAP, [-40.0, 50.0, 60.0, -30.0, -14.0, 40.0, 46.0, -8.0,
+                                                -50.0, -40.0, 40.0, 30.0, -25.0, -19.0, 29.0, 23.0, 45.0]),
+                         "generic trapezoid")
+    g.place(gtra, small, (0.0, 0.0, -10.0) + _rot("z", 0.9))
+    g.place(w, gtra, (-200.0, -20.0, 150.0) + _rot("y", 0.7))
+    g.place(w, gtra, (220.0, 60.0, -180.0) + _rot("x", -1.0))
     return g
 
 

@@ -78,7 +78,7 @@ def test_normal_keeps_caller_vectors_and_sizes():
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "torus2",
                                    "tessellated", "hype", "trd", "paraboloid", "ellipticaltube",
-                                   "ellipsoid"])
+                                   "ellipsoid", "gentrap"])
 def test_gpu_reference_normal_kats(shape):
     """The reference's Normal asserts (test/unit_tests/Test<Shape>.cpp), evaluated on the GPU; chained
     points p0 + Dist * v take Dist from the GPU's own DistanceToIn/Out."""
@@ -90,6 +90,8 @@ def test_gpu_reference_normal_kats(shape):
                                          gpu_util.to_dev(kat_util.soa([d]))).cpu()[0])
 
     recs = [r for r in kat_util.load(shape) if r["method"] == "Normal"]
+    if shape == "gentrap":  # twisted traps are not built
+        recs = [r for r in recs if not kat_util.gentrap_twisted(r["shape"])]
     assert recs
     fails = []
     for r in recs:

@@ -46,6 +46,10 @@ CASES = [
     ("tet", [6.0, 4.0, 5.0, -5.0, -6.0, 4.0, -4.0, 5.0, -6.0, 5.0, -5.0, -4.0]),
     ("ellipticaltube", [5.0, 10.0, 20.0]),
     ("ellipsoid", [3.0, 4.0, 5.0, -4.5, 3.5]),
+    # a GenTrap given clockwise: given anticlockwise, the reference reverts the vertices but takes the face
+    # normal by the reverted polygon's edge index from normals made in the order given
+    # (GenTrapStruct.h:67-101, GenTrapImplementation.h:654-660), which is not the exit face's
+    ("gentrap", [-4.0, -3.0, 6.0, 5.0, -1.4, -0.8, 4.6, 4.0, -5.0, 3.0, 4.0, -4.0, -2.5, 2.3, 2.9, -1.9, 8.0]),
 ]
 
 

@@ -321,6 +321,20 @@ def margin_eltube(params, p):
 
 for _t in ([5.0, 10.0, 20.0], [30.0, 12.0, 8.0]):
     CASES.append(("ellipticaltube", _t, margin_eltube))
+
+
+def gentrap_corners(params):
+    """The eight corners of a GenTrap: (x[i], y[i], -dz) for i < 4, (x[i], y[i], +dz) above."""
+    x, y, dz = params[:8], params[8:16], params[16]
+    return np.unique(np.array([(x[i], y[i], -dz if i < 4 else dz) for i in range(8)]), axis=0)
+
+
+# planar traps (twisted ones are not built): the reference test's frustum (clockwise), an anticlockwise
+# skewed one (the constructor reverts it), one whose top is a segment (two collapsed edges)
+for _t in ([-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, -2.0, 2.0, 2.0, -2.0, 5.0],
+           [-4.0, 5.0, 6.0, -3.0, -1.4, 4.0, 4.6, -0.8, -5.0, -4.0, 4.0, 3.0, -2.5, -1.9, 2.9, 2.3, 8.0],
+           [-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, 0.0, 0.0, 0.0, 0.0, 6.0]):
+    CASES.append(("gentrap", _t, margin_convex(gentrap_corners(_t))))
 for _t in ([3.0, 4.0, 5.0, -4.5, 3.5], [10.0, 7.0, 12.0], [6.0, 9.0, 4.0, 1.0, 4.0], [6.0, 6.0, 6.0, -2.0, 5.0]):
     CASES.append(("ellipsoid", _t, margin_ellipsoid))
 IDS = ["%s-%d" % (c[0], i) for i, c in enumerate(CASES)]

@@ -149,6 +149,7 @@ static int method_groups(int type, MethodGroup g[3])
   case kTet:
   case kEllipticalTube:
   case kEllipsoid:
+  case kGenTrap:
     g[0] = {kMAll, 2};
     return 1;
   case kCone:

@@ -730,5 +730,44 @@ VGMI_DEV bool Normal(EllipsoidData const &s, V3d const &point, V3d &n, bool &set
   return false;
 }
 
+// GenTrapImplementation::NormalKernel and GetClosestEdge (VecGeom/volumes/kernel/GenTrapImplementation.h:
+// 610-735), planar faces: +-z within 10 kTolerance of a z plane, else the shell normal of the section edge
+// closest to the point (the section from the reverted vertices, the normals in the order given)
+VGMI_DEV bool Normal(GenTrapData const &s, V3d const &p, V3d &n, bool &set)
+{
+  set               = true;
+  const double safz = vabs(s.dz - vabs(p.z));
+  if (safz < 10. * kTolerance) {
+    n = V3d(0., 0., p.z > 0. ? 1. : (p.z < 0. ? -1. : 0.));
+    return true;
+  }
+  const double cf = s.halfInverseDz * (s.dz - p.z);
+  double vertexX[4], vertexY[4];
+  for (int i = 0; i < 4; i++) {
+    vertexX[i] = s.vx[i + 4] + cf * s.connX[i];
+    vertexY[i] = s.vy[i + 4] + cf * s.connY[i];
+  }
+  int iseg      = 0;
+  double safe   = kInfLength;
+  for (int i = 0; i < 4; ++i) {
+    const int j     = (i + 1) % 4;
+    const double dx = vertexX[j] - vertexX[i], dy = vertexY[j] - vertexY[i];
+    double dpx = p.x - vertexX[i], dpy = p.y - vertexY[i];
+    const double lsq = dx * dx + dy * dy;
+    if (lsq < kTolerance) {
+      const double ssq = dpx * dpx + dpy * dpy;
+      if (ssq < safe) iseg = i, safe = ssq;
+      continue;
+    }
+    const double u = (dpx * dx + dpy * dy) / NonZero(lsq);
+    if (u > 1) dpx = p.x - vertexX[j], dpy = p.y - vertexY[j];
+    if (u >= 0 && u <= 1) dpx = dpx - u * dx, dpy = dpy - u * dy;
+    const double ssq = dpx * dpx + dpy * dpy;
+    if (ssq < safe) iseg = i, safe = ssq;
+  }
+  n = V3d(s.shellNormal[iseg][0], s.shellNormal[iseg][1], s.shellNormal[iseg][2]);
+  return true;
+}
+
 } // namespace nrm
 } // namespace vgmi

@@ -4,6 +4,7 @@
 #include "dev_box.h"
 #include "dev_cone.h"
 #include "dev_ellipsoids.h"
+#include "dev_gentrap.h"
 #include "dev_hype.h"
 #include "dev_orb.h"
 #include "dev_para_tet.h"
@@ -38,7 +39,8 @@
   X(kParallelepiped, ParallelepipedData, ppd, ppd)                                                                   \
   X(kTet, TetData, tet, tet)                                                                                         \
   X(kEllipticalTube, EllipticalTubeData, eltu, eltu)                                                                 \
-  X(kEllipsoid, EllipsoidData, ells, ells)
+  X(kEllipsoid, EllipsoidData, ells, ells)                                                                           \
+  X(kGenTrap, GenTrapData, gtra, gtra)
 
 namespace vgmi {
 

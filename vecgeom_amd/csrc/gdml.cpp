@@ -523,6 +523,11 @@ private:
     } else if (n == "ellipsoid") { // processEllipsoid (Middleware.cpp:948-965)
       type = kEllipsoid;
       p    = {l * num(e, "ax"), l * num(e, "by"), l * num(e, "cz"), l * num(e, "zcut1"), l * num(e, "zcut2")};
+    } else if (n == "arb8") { // processGenTrap (Middleware.cpp:1047-1096): the vertices without the length unit
+      type = kGenTrap;
+      for (const char *c : {"v1x", "v2x", "v3x", "v4x", "v5x", "v6x", "v7x", "v8x"}) p.push_back(num(e, c));
+      for (const char *c : {"v1y", "v2y", "v3y", "v4y", "v5y", "v6y", "v7y", "v8y"}) p.push_back(num(e, c));
+      p.push_back(l * num(e, "dz"));
     } else if (n == "para") { // processParallelepiped (Middleware.cpp:967-988): x y z halved
       type = kParallelepiped;
       p    = {l * num(e, "x") / 2., l * num(e, "y") / 2., l * num(e, "z") / 2., a * num(e, "alpha"),

@@ -40,6 +40,9 @@ VGMI_DEV double placed_dist(Data const &s, int op, bool mixed, Vec3<double> cons
     const tess::V3d pp(p.x, p.y, p.z), dd(d.x, d.y, d.z);
     return in ? tess::dist_in_step64(s, tess::pool_walk(s), pp, dd, stepMax)
               : tess::dist_out_step64(s, tess::pool_walk(s), pp, dd, stepMax);
+  } else if constexpr (S::kType == kGenTrap) {
+    if (in) return gtra::DistanceToInStep(s, p, d, stepMax); // the step limit bounds the entry search
+    return S::template DistOut<double>(s, p, d);
   } else {
     (void)stepMax; // the reference's other kernels ignore the step limit
     if (mixed) return in ? mixed_point<Data, kOpDistIn>(s, p, d) : mixed_point<Data, kOpDistOut>(s, p, d);

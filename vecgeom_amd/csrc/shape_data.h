@@ -28,7 +28,8 @@ enum ShapeType : int32_t {
   kTet = 14,
   kEllipticalTube = 15,
   kEllipsoid = 16,
-  kNumShapeTypes = 17
+  kGenTrap = 17,
+  kNumShapeTypes = 18
 };
 
 enum QueryOp : int32_t {
@@ -197,6 +198,18 @@ struct EllipsoidData {
   double dx, dy, dz, zBottomCut, zTopCut, xmax, ymax, rsph, r, sx, sy, sz, scZMidCut, scZDimCut, q1, q2;
 };
 
+// GenTrap with planar lateral faces (VecGeom/volumes/GenTrapStruct.h:57-190): the vertices after the
+// clockwise reversal, bottom-to-top connecting components, the lateral faces that the reference's
+// TessellatedSection keeps (outward unit normal, offset: n.p + d), and SecondOrderSurfaceShell's face
+// normals in the vertex order as given (used by Normal).
+struct GenTrapData {
+  double dz, halfInverseDz;
+  double vx[8], vy[8], connX[4], connY[4];
+  double fnormal[4][3], fdist[4];
+  double shellNormal[4][3];
+  int32_t nfacets, pad_;
+};
+
 // Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
 // and a threaded BVH over them live in the variable-length pool:
 //   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
@@ -255,6 +268,7 @@ struct alignas(16) ShapeRecord {
     TetData tet;
     EllipticalTubeData eltu;
     EllipsoidData ells;
+    GenTrapData gtra;
     double raw[64];
   } u;
 };
@@ -262,5 +276,6 @@ static_assert(sizeof(PolyhedronData) <= 64 * sizeof(double), "shape records must
 static_assert(sizeof(SphereData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 static_assert(sizeof(TorusData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 static_assert(sizeof(ConeData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
+static_assert(sizeof(GenTrapData) <= 64 * sizeof(double), "shape records must fit the 512-byte union");
 
 } // namespace vgmi

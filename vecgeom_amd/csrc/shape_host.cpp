@@ -2,6 +2,7 @@
 // by the reference constructors). Runs once per shape at geometry build time.
 #include "shape_host.h"
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 
@@ -467,6 +468,122 @@ int MakeEllipsoid(double const *a, int n, ShapeRecord &r, std::string &err)
   e.scZDimCut = 0.5 * (top - bot);
   e.q1        = 0.5 / e.r;
   e.q2        = 0.5 * (e.r + 0.5e-9 * 0.5e-9 / e.r);
+  return 0;
+}
+
+// GenTrap: UnplacedGenTrap(x[8], y[8], dz) (VecGeom/volumes/UnplacedGenTrap.h:46) and GenTrapStruct::
+// Initialize (VecGeom/volumes/GenTrapStruct.h:57-188): SecondOrderSurfaceShell's face normals from the
+// vertices as given; anticlockwise polygons reverted to clockwise; opposite orientations, crossing opposite
+// segments and non-convex polygons refused (the reference prints an error and leaves the solid half made);
+// the lateral faces of the TessellatedSection (Tile<4>::AddVertex, VecGeom/volumes/Tile.h:70-130: a face
+// with fewer than three distinct corners is left out). Twisted traps (non-parallel bottom and top edges of a
+// face) are refused: only the planar path is built.
+int MakeGenTrap(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n != 17) return err = "gentrap takes x[8] y[8] dz", -1;
+  for (int i = 0; i < 17; ++i)
+    if (!std::isfinite(a[i])) return err = "gentrap parameters must be finite", -1;
+  const double h = a[16], tol = 1e-9;
+  if (!(h > 0)) return err = "gentrap half-length must be positive", -1;
+  const double *x = a, *y = a + 8;
+  using V = std::array<double, 3>;
+  auto sub3  = [](V const &p, V const &q) { return V{p[0] - q[0], p[1] - q[1], p[2] - q[2]}; };
+  auto cross = [](V const &l, V const &rr) {
+    return V{l[1] * rr[2] - l[2] * rr[1], l[2] * rr[0] - l[0] * rr[2], l[0] * rr[1] - l[1] * rr[0]};
+  };
+  auto mag2 = [](V const &v) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2]; };
+  auto normalized = [&](V v) {
+    const double f = 1. / std::sqrt(mag2(v));
+    return V{v[0] * f, v[1] * f, v[2] * f};
+  };
+  auto dot = [](V const &l, V const &rr) { return l[0] * rr[0] + l[1] * rr[1] + l[2] * rr[2]; };
+  r              = ShapeRecord();
+  r.type         = kGenTrap;
+  GenTrapData &g = r.u.gtra;
+  g.dz = h, g.halfInverseDz = 0.5 / h;
+  for (int i = 0; i < 4; ++i) { // SecondOrderSurfaceShell::Initialize (SecondOrderSurfaceShell.h:65-105)
+    const int j = (i + 1) % 4;
+    const V va{x[i], y[i], -h}, vb{x[i + 4], y[i + 4], h}, vc{x[j], y[j], -h}, vd{x[j + 4], y[j + 4], h};
+    V nn = cross(sub3(vb, va), sub3(vc, va));
+    if (mag2(nn) < tol) {
+      nn = cross(sub3(vb, va), sub3(vd, vb));
+      if (mag2(nn) < tol) nn = V{0., 0., 1.};
+    }
+    nn = normalized(nn);
+    for (int c = 0; c < 3; ++c) g.shellNormal[i][c] = nn[c];
+  }
+  V v[8];
+  for (int i = 0; i < 8; ++i) v[i] = V{x[i], y[i], i < 4 ? -h : h};
+  double sum1 = 0., sum2 = 0.;
+  for (int i = 0; i < 4; ++i) {
+    const int j = (i + 1) % 4;
+    sum1 += v[i][0] * v[j][1] - v[j][0] * v[i][1];
+    sum2 += v[i + 4][0] * v[j + 4][1] - v[j + 4][0] * v[i + 4][1];
+  }
+  if (sum1 * sum2 < -tol) return err = "gentrap bottom and top polygons have opposite orientations", -1;
+  if (sum1 > tol) std::swap(v[1], v[3]), std::swap(v[5], v[7]);
+  for (int i = 0; i < 4; ++i) g.connX[i] = sub3(v[i], v[i + 4])[0], g.connY[i] = sub3(v[i], v[i + 4])[1];
+  for (int i = 0; i < 8; ++i) g.vx[i] = v[i][0], g.vy[i] = v[i][1];
+  auto crossing = [&](V const &p, V const &p1, V const &q, V const &q1) { // GenTrapStruct.h:296-317
+    const V rr = sub3(p1, p), ss = sub3(q1, q), rxs = cross(rr, ss);
+    if (mag2(rxs) < tol) return false;
+    const double t = cross(sub3(q, p), ss)[2] / rxs[2];
+    if (t < tol || t > 1. - tol) return false;
+    const double u = cross(sub3(q, p), rr)[2] / rxs[2];
+    return !(u < tol || u > 1. - tol);
+  };
+  if (crossing(v[0], v[1], v[3], v[2]) || crossing(v[1], v[2], v[0], v[3]) || crossing(v[4], v[5], v[7], v[6]) ||
+      crossing(v[5], v[6], v[4], v[7]))
+    return err = "gentrap polygons have crossing opposite segments", -1;
+  double dX[8], dY[8];
+  for (int i = 0; i < 4; ++i) {
+    const int j = (i + 1) % 4;
+    dX[i] = g.vx[j] - g.vx[i], dX[i + 4] = g.vx[j + 4] - g.vx[i + 4];
+    dY[i] = g.vy[j] - g.vy[i], dY[i + 4] = g.vy[j + 4] - g.vy[i + 4];
+  }
+  for (int i = 0; i < 4; ++i) {
+    const int j = (i + 1) % 4;
+    if (dX[i] * dY[j] - dY[i] * dX[j] > tol || dX[i + 4] * dY[j + 4] - dY[i + 4] * dX[j + 4] > tol)
+      return err = "gentrap bottom or top polygon is not convex", -1;
+  }
+  for (int i = 0; i < 4; ++i) { // ComputeIsTwisted (GenTrapStruct.h:246-276)
+    const double dx1 = v[(i + 1) % 4][0] - v[i][0], dy1 = v[(i + 1) % 4][1] - v[i][1];
+    const double dx2 = v[4 + (i + 1) % 4][0] - v[4 + i][0], dy2 = v[4 + (i + 1) % 4][1] - v[4 + i][1];
+    if ((dx1 == 0 && dy1 == 0) || (dx2 == 0 && dy2 == 0)) continue;
+    if (std::fabs(dy1 * dx2 - dx1 * dy2) >= tol) return err = "twisted gentrap faces are not supported", -1;
+  }
+  const int quad[4][4] = {{0, 4, 5, 1}, {1, 5, 6, 2}, {2, 6, 7, 3}, {3, 7, 4, 0}};
+  g.nfacets = 0;
+  for (auto const &q : quad) {
+    const V c[4] = {v[q[0]], v[q[1]], v[q[2]], v[q[3]]};
+    int nvert = 4;
+    for (int i = 0; i < 4; ++i)
+      if (mag2(sub3(c[(i + 1) % 4], c[i])) < tol) nvert--;
+    if (nvert < 3) continue;
+    bool found = false;
+    V nn{0., 0., 0.};
+    for (int i = 0; i < 3 && !found; ++i) {
+      const V e1 = sub3(c[i + 1], c[i]);
+      if (mag2(e1) < tol) continue;
+      for (int j = i + 1; j < 4; ++j) {
+        const V e2 = sub3(c[(j + 1) % 4], c[j]);
+        if (mag2(e2) < tol) continue;
+        nn = cross(e1, e2);
+        if (mag2(nn) < tol) continue;
+        nn = normalized(nn), found = true;
+        break;
+      }
+    }
+    if (!found) continue;
+    double dd = 0.;
+    for (int i = 0; i < 4; ++i)
+      if (!(mag2(sub3(c[(i + 1) % 4], c[i])) < tol)) {
+        dd = -dot(nn, c[i]);
+        break;
+      }
+    for (int k = 0; k < 3; ++k) g.fnormal[g.nfacets][k] = nn[k];
+    g.fdist[g.nfacets++] = dd;
+  }
   return 0;
 }
 
@@ -1057,6 +1174,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeEllipticalTube(params, nparams, r, err);
   case kEllipsoid:
     return MakeEllipsoid(params, nparams, r, err);
+  case kGenTrap:
+    return MakeGenTrap(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -1142,6 +1261,16 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
     // source/UnplacedEllipticalTube.cpp:64-71
     EllipticalTubeData const &e = r.u.eltu;
     lo[0] = -e.dx, lo[1] = -e.dy, lo[2] = -e.dz, hi[0] = e.dx, hi[1] = e.dy, hi[2] = e.dz;
+    return;
+  }
+  case kGenTrap: {
+    // GenTrapStruct::Extent (VecGeom/volumes/GenTrapStruct.h:199-214)
+    GenTrapData const &g = r.u.gtra;
+    lo[0] = hi[0] = g.vx[0], lo[1] = hi[1] = g.vy[0], lo[2] = -g.dz, hi[2] = g.dz;
+    for (int i = 0; i < 8; ++i) {
+      lo[0] = std::min(lo[0], g.vx[i]), hi[0] = std::max(hi[0], g.vx[i]);
+      lo[1] = std::min(lo[1], g.vy[i]), hi[1] = std::max(hi[1], g.vy[i]);
+    }
     return;
   }
   case kEllipsoid: {
