@@ -204,6 +204,18 @@ def run_mix(args):
                         "kernel": "k_batch",
                         "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)},
            "valu": valu_entry("vgmi::k_batch<%d>" % batch.info()["block"])}
+    # the same mix and launch shape in mixed precision (FP32 bulk, FP64 near the surface)
+    from vecgeom_amd.shapes import MIXED
+    mouts = batch.alloc_outputs()
+    batch.capture(P, D, mouts, precision=MIXED)
+    for _ in range(args.warmup):
+        batch.replay()
+    torch.cuda.synchronize()
+    mus = event_time_us(batch.replay, args.steps)
+    res["mixed"] = {"value": round(queries / (mus * 1e-6), 1), "unit": "queries/s", "ms_per_step": round(mus / 1e3, 5),
+                    "vs_fp64": round(us / mus, 3),
+                    "decisions_equal_fp64": bool(torch.equal(mouts["Contains"], outs["Contains"])
+                                                 and torch.equal(mouts["Inside"], outs["Inside"]))}
     if not args.no_cpu:
         from oracle.pyoracle import OracleShape
         orc = [OracleShape(k, prm) for k, prm in shapes]

@@ -147,7 +147,9 @@ VGMI_API vgmi_status vgmi_batch_destroy(vgmi_batch *batch);
 VGMI_API vgmi_status vgmi_batch_set_tuning(vgmi_batch *batch, int tile_points, int block, int blocks_per_cu);
 /* info = nshapes, total points, tiles, tile points, block, blocks per CU */
 VGMI_API vgmi_status vgmi_batch_info(const vgmi_batch *batch, int64_t info[6]);
-/* All six methods for every point of the batch in one persistent launch (NULL outputs are skipped). */
+/* All six methods for every point of the batch in one persistent launch (NULL outputs are skipped).
+ * precision VGMI_FP64, or VGMI_MIXED: each point's answers equal vgmi_shape_query's mixed answers
+ * for its shape bit for bit. */
 VGMI_API vgmi_status vgmi_batch_query_all(vgmi_batch *batch, int precision, const double *px, const double *py,
                                           const double *pz, const double *dx, const double *dy, const double *dz,
                                           uint8_t *contains, int32_t *inside, double *dist_in, double *dist_out,

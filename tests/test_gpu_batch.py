@@ -178,3 +178,60 @@ def test_batch_tessellated_non_convex():
             for m in METHODS:
                 assert np.array_equal(ref[m], out[m][off:off + n].cpu().numpy()), (m, tuning, off)
             off += n
+
+
+def test_batch_mixed_matches_single_shape_mixed():
+    """precision=MIXED: every point's six answers equal the single-shape mixed launches bit for bit
+    (one shape of every type, uneven counts, several launch shapes, a partial output set and a
+    graph replay); decisions equal the FP64 batch's."""
+    from vecgeom_amd import MIXED
+    from tests.test_gpu_shapes import CASES
+    seen, shapes = set(), []
+    for kind, params in CASES:
+        if kind not in seen:
+            seen.add(kind)
+            shapes.append((kind, params))
+    counts = [(1237 * (k + 3)) % 4000 + 1 for k in range(len(shapes))]
+    gpu = [make_shape(k, p) for k, p in shapes]
+    orc = [OracleShape(k, p) for k, p in shapes]
+    exts = [o.extent() for o in orc]
+    p, d = workloads.mix_inputs(exts, counts, [lambda x, o=o: o.query("Contains", x) for o in orc],
+                                [lambda x, y, o=o: o.query("DistanceToOut", x, y) for o in orc])
+    P, D = SOA3D.from_array(p), SOA3D.from_array(d)
+    refs, off = [], 0
+    for s, n in zip(gpu, counts):
+        Ps = SOA3D.from_array(np.ascontiguousarray(p[:, off:off + n]))
+        Ds = SOA3D.from_array(np.ascontiguousarray(d[:, off:off + n]))
+        refs.append({m: (getattr(s, m)(Ps, Ds, precision=MIXED) if m.startswith("Distance")
+                         else getattr(s, m)(Ps, precision=MIXED)) for m in METHODS})
+        off += n
+    b = ShapeBatch(gpu, counts)
+    fp64 = b.query_all(P, D)
+    for tuning in [(64, 64, 1), (256, 256, 4), (1000, 512, 2)]:
+        print("tuning", tuning, flush=True)
+        b.set_tuning(*tuning)
+        out = b.query_all(P, D, precision=MIXED)
+        off = 0
+        for (kind, _), ref, n in zip(shapes, refs, counts):
+            for m in METHODS:
+                assert torch.equal(ref[m], out[m][off:off + n]), (kind, m, tuning)
+            off += n
+        for m in ("Contains", "Inside"):
+            assert torch.equal(out[m], fp64[m]), (m, tuning)
+    part = b.alloc_outputs()
+    for m in ("Inside", "SafetyToIn", "DistanceToOut"):
+        part[m] = None
+    b.query_all(P, D, part, precision=MIXED)
+    for m in METHODS:
+        if part[m] is not None:
+            assert torch.equal(part[m], out[m]), m
+    outs = b.alloc_outputs()
+    b.capture(P, D, outs, precision=MIXED)
+    for rep in range(2):
+        for t in outs.values():
+            t.zero_()
+        torch.cuda.synchronize()
+        b.replay()
+        torch.cuda.synchronize()
+        for m in METHODS:
+            assert torch.equal(outs[m], out[m]), (rep, m)

@@ -11,6 +11,7 @@
 //
 // The launch sequence (counter reset kernel + persistent kernel) can be captured
 // once into a hipGraph and replayed; the reset is a kernel, not a memset node.
+#include "dev_mixed.h"
 #include "dev_tess_tile.h"
 #include "vgmi_internal.h"
 #include <algorithm>
@@ -63,6 +64,32 @@ VGMI_DEV void run_tile(Data const &s, int64_t b, int64_t e, BatchOut const &o, i
   }
 }
 
+// The mixed-precision tile: the per-point answers of the single-shape mixed kernels, with the
+// point's FP32 classification computed once for all methods of the tile.
+template <class Data>
+VGMI_DEV void run_tile_mixed(Data const &s, int64_t b, int64_t e, BatchOut const &o, int nthreads)
+{
+  const bool point = o.contains || o.inside || o.sin || o.sout;
+  const bool dirs  = o.din || o.dout;
+  for (int64_t i = b + threadIdx.x; i < e; i += nthreads) {
+    Vec3<double> p(__builtin_nontemporal_load(o.px + i), __builtin_nontemporal_load(o.py + i),
+                   __builtin_nontemporal_load(o.pz + i));
+    Vec3<double> d(0., 0., 0.);
+    if (dirs)
+      d = Vec3<double>(__builtin_nontemporal_load(o.dx + i), __builtin_nontemporal_load(o.dy + i),
+                       __builtin_nontemporal_load(o.dz + i));
+    MixedState m{};
+    if (point || (dirs && CertifiedF32Dist<Data>::value)) m = mixed_state(s, p);
+    if (o.contains)
+      __builtin_nontemporal_store(uint8_t(mixed_answer<Data, kOpContains>(s, m, p, d) != 0.), o.contains + i);
+    if (o.inside) __builtin_nontemporal_store(int32_t(mixed_answer<Data, kOpInside>(s, m, p, d)), o.inside + i);
+    if (o.sin) __builtin_nontemporal_store(mixed_answer<Data, kOpSafetyIn>(s, m, p, d), o.sin + i);
+    if (o.sout) __builtin_nontemporal_store(mixed_answer<Data, kOpSafetyOut>(s, m, p, d), o.sout + i);
+    if (o.din) __builtin_nontemporal_store(mixed_answer<Data, kOpDistIn>(s, m, p, d), o.din + i);
+    if (o.dout) __builtin_nontemporal_store(mixed_answer<Data, kOpDistOut>(s, m, p, d), o.dout + i);
+  }
+}
+
 // A tessellated tile. When the solid's image (nodes, planes, candidate boxes) fits in the
 // launch's LDS (ldsDoubles), the workgroup copies it there once per solid (consecutive tiles of
 // one solid reuse the copy; `staged` names the solid held) and the walks read it from LDS. The
@@ -88,7 +115,9 @@ __global__ void k_batch_reset(uint32_t *counter) { *counter = 0u; }
 // At least two wavefronts per SIMD: the register budget then counts accumulation registers too
 // (without it the compiler may take 256 vector registers plus one accumulation register, which
 // leaves room for a single wavefront).
-template <int BLOCK>
+// MIX selects the mixed-precision answers; the tessellated solid answers in FP64 either way (its
+// single-shape mixed query does the same).
+template <int BLOCK, bool MIX>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void k_batch(const ShapeRecord *__restrict__ recs, const Tile *__restrict__ tiles,
                                                  int ntiles, uint32_t *counter, BatchOut o, int ldsDoubles)
 {
@@ -117,7 +146,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(2))) void
     switch (rec.type) {
 #define X(E, T, M, NS)                                                                                               \
   case E:                                                                                                            \
-    run_tile<T>(rec.u.M, tile.begin, tile.end, mo, BLOCK);                                                          \
+    if constexpr (MIX)                                                                                               \
+      run_tile_mixed<T>(rec.u.M, tile.begin, tile.end, mo, BLOCK);                                                  \
+    else                                                                                                             \
+      run_tile<T>(rec.u.M, tile.begin, tile.end, mo, BLOCK);                                                        \
     break;
       VGMI_FOR_EACH_SHAPE_BUT_TESSELLATED(X)
 #undef X
@@ -243,7 +275,30 @@ static void batch_drop_graph(vgmi_batch *b)
   b->graph     = nullptr;
 }
 
-static vgmi_status batch_launch(vgmi_batch *b, BatchOut const &o, hipStream_t st)
+template <bool MIX>
+static void launch_k_batch(vgmi_batch const *b, BatchOut const &o, hipStream_t st, int grid, size_t lds, int ldsD)
+{
+  switch (b->block) {
+  case 64:
+    hipLaunchKernelGGL((k_batch<64, MIX>), dim3(grid), dim3(64), lds, st, b->d_recs, b->d_tiles, b->ntiles,
+                       b->d_counter, o, ldsD);
+    break;
+  case 128:
+    hipLaunchKernelGGL((k_batch<128, MIX>), dim3(grid), dim3(128), lds, st, b->d_recs, b->d_tiles, b->ntiles,
+                       b->d_counter, o, ldsD);
+    break;
+  case 512:
+    hipLaunchKernelGGL((k_batch<512, MIX>), dim3(grid), dim3(512), lds, st, b->d_recs, b->d_tiles, b->ntiles,
+                       b->d_counter, o, ldsD);
+    break;
+  default:
+    hipLaunchKernelGGL((k_batch<256, MIX>), dim3(grid), dim3(256), lds, st, b->d_recs, b->d_tiles, b->ntiles,
+                       b->d_counter, o, ldsD);
+    break;
+  }
+}
+
+static vgmi_status batch_launch(vgmi_batch *b, BatchOut const &o, bool mixed, hipStream_t st)
 {
   if (b->ntiles == 0) return VGMI_OK;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(1), 0, st, b->d_counter);
@@ -253,24 +308,10 @@ static vgmi_status batch_launch(vgmi_batch *b, BatchOut const &o, hipStream_t st
   const int grid  = std::max(1, std::min(b->ntiles, b->numCU * perCU));
   const size_t lds = size_t(b->ldsBytes);
   const int ldsD   = b->ldsBytes / int(sizeof(double));
-  switch (b->block) {
-  case 64:
-    hipLaunchKernelGGL(k_batch<64>, dim3(grid), dim3(64), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter, o,
-                       ldsD);
-    break;
-  case 128:
-    hipLaunchKernelGGL(k_batch<128>, dim3(grid), dim3(128), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter,
-                       o, ldsD);
-    break;
-  case 512:
-    hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(512), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter,
-                       o, ldsD);
-    break;
-  default:
-    hipLaunchKernelGGL(k_batch<256>, dim3(grid), dim3(256), lds, st, b->d_recs, b->d_tiles, b->ntiles, b->d_counter,
-                       o, ldsD);
-    break;
-  }
+  if (mixed)
+    launch_k_batch<true>(b, o, st, grid, lds, ldsD);
+  else
+    launch_k_batch<false>(b, o, st, grid, lds, ldsD);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_batch launch: %s", hipGetErrorString(e));
   return VGMI_OK;
@@ -281,7 +322,8 @@ static vgmi_status batch_outputs(vgmi_batch *b, int precision, const double *px,
                                  int32_t *inside, double *din, double *dout, double *sin, double *sout, BatchOut &o)
 {
   if (!b) return set_error(VGMI_ERR_INVALID, "vgmi_batch: null batch");
-  if (precision != VGMI_FP64) return set_error(VGMI_ERR_UNSUPPORTED, "vgmi_batch: only VGMI_FP64 is implemented");
+  if (precision != VGMI_FP64 && precision != VGMI_MIXED)
+    return set_error(VGMI_ERR_INVALID, "vgmi_batch: precision must be VGMI_FP64 or VGMI_MIXED");
   if (b->total > 0 && (!px || !py || !pz)) return set_error(VGMI_ERR_INVALID, "vgmi_batch: null point pointer");
   if ((din || dout) && (!dx || !dy || !dz)) return set_error(VGMI_ERR_INVALID, "vgmi_batch: distances need directions");
   o = BatchOut{px, py, pz, dx, dy, dz, contains, inside, din, dout, sin, sout};
@@ -377,7 +419,7 @@ vgmi_status vgmi_batch_query_all(vgmi_batch *b, int precision, const double *px,
   if (vgmi_status s = batch_outputs(b, precision, px, py, pz, dx, dy, dz, contains, inside, dist_in, dist_out,
                                     safety_in, safety_out, o))
     return s;
-  return batch_launch(b, o, (hipStream_t)stream);
+  return batch_launch(b, o, precision == VGMI_MIXED, (hipStream_t)stream);
 }
 
 vgmi_status vgmi_batch_graph_capture(vgmi_batch *b, int precision, const double *px, const double *py,
@@ -396,7 +438,7 @@ vgmi_status vgmi_batch_graph_capture(vgmi_batch *b, int precision, const double 
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   vgmi_status ls = VGMI_OK;
   if (e == hipSuccess) {
-    ls = batch_launch(b, o, cap);
+    ls = batch_launch(b, o, precision == VGMI_MIXED, cap);
     hipError_t e2 = hipStreamEndCapture(cap, &b->graph);
     if (e == hipSuccess) e = e2;
   }

@@ -60,42 +60,69 @@ VGMI_DEV bool F32PointUndecided<PolyhedronData>(PolyhedronData const &s, Vec3<fl
 constexpr float kMixBandF = float(kMixedBand);
 constexpr float kMixFltBig = 1e30f;
 
-// Contains (uint8), Inside (int32) or a distance/safety (double) of one point.
-template <class Data, int OP>
-VGMI_DEV double mixed_point(Data const &s, Vec3<double> const &p, Vec3<double> const &d)
+// The FP32 classification of one point, shared by every method asked of it.
+struct MixedState {
+  Vec3<float> pf;
+  float sin32, sout32;
+  bool near;
+};
+
+template <class Data>
+VGMI_DEV MixedState mixed_state(Data const &s, Vec3<double> const &p)
 {
   using S = ShapeOps<Data>;
-  if constexpr ((OP == kOpDistIn || OP == kOpDistOut) && !CertifiedF32Dist<Data>::value)
-    return OP == kOpDistIn ? S::template DistIn<double>(s, p, d) : S::template DistOut<double>(s, p, d);
-  const Vec3<float> pf(float(p.x), float(p.y), float(p.z));
-  const float sin32  = S::template SafeIn<float>(s, pf);
-  const float sout32 = S::template SafeOut<float>(s, pf);
+  MixedState m;
+  m.pf     = Vec3<float>(float(p.x), float(p.y), float(p.z));
+  m.sin32  = S::template SafeIn<float>(s, m.pf);
+  m.sout32 = S::template SafeOut<float>(s, m.pf);
   // A consistent FP32 state has safeties of opposite sign (inside: SafetyToIn < 0 < SafetyToOut,
   // outside: the reverse). Equal signs mean the FP32 tolerance bands collapsed onto the surface
   // (a point within FP32 rounding of it is then both inside and outside): treat as near.
   // An FP32 safety that cannot be decided (NaN) sends the point to FP64 as well.
-  const bool near = vabs(sin32) < kMixBandF || vabs(sout32) < kMixBandF || ((sin32 < 0.f) == (sout32 < 0.f)) ||
-                    sin32 != sin32 || sout32 != sout32 || F32PointUndecided(s, pf);
-  if constexpr (OP == kOpContains) {
-    return (near ? S::template Contains<double>(s, p) : S::template Contains<float>(s, pf)) ? 1.0 : 0.0;
+  m.near = vabs(m.sin32) < kMixBandF || vabs(m.sout32) < kMixBandF || ((m.sin32 < 0.f) == (m.sout32 < 0.f)) ||
+           m.sin32 != m.sin32 || m.sout32 != m.sout32 || F32PointUndecided(s, m.pf);
+  return m;
+}
+
+// Contains (uint8), Inside (int32) or a distance/safety (double) of one point whose FP32 state is m.
+// Directed queries of shapes without certified FP32 distances do not use m.
+template <class Data, int OP>
+VGMI_DEV double mixed_answer(Data const &s, MixedState const &m, Vec3<double> const &p, Vec3<double> const &d)
+{
+  using S = ShapeOps<Data>;
+  if constexpr ((OP == kOpDistIn || OP == kOpDistOut) && !CertifiedF32Dist<Data>::value) {
+    return OP == kOpDistIn ? S::template DistIn<double>(s, p, d) : S::template DistOut<double>(s, p, d);
+  } else if constexpr (OP == kOpContains) {
+    return (m.near ? S::template Contains<double>(s, p) : S::template Contains<float>(s, m.pf)) ? 1.0 : 0.0;
   } else if constexpr (OP == kOpInside) {
-    return double(near ? S::template Inside<double>(s, p) : S::template Inside<float>(s, pf));
+    return double(m.near ? S::template Inside<double>(s, p) : S::template Inside<float>(s, m.pf));
   } else if constexpr (OP == kOpSafetyIn) {
-    return near ? S::template SafeIn<double>(s, p) : double(sin32);
+    return m.near ? S::template SafeIn<double>(s, p) : double(m.sin32);
   } else if constexpr (OP == kOpSafetyOut) {
-    return near ? S::template SafeOut<double>(s, p) : double(sout32);
+    return m.near ? S::template SafeOut<double>(s, p) : double(m.sout32);
   } else {
     double r  = 0.;
-    bool fp64 = near;
-    if (!near) {
+    bool fp64 = m.near;
+    if (!m.near) {
       const Vec3<float> df(float(d.x), float(d.y), float(d.z));
-      const float f = OP == kOpDistIn ? S::template DistIn<float>(s, pf, df) : S::template DistOut<float>(s, pf, df);
-      r             = f >= kMixFltBig ? kInfLength : double(f);
-      fp64          = f != f;
+      const float f =
+          OP == kOpDistIn ? S::template DistIn<float>(s, m.pf, df) : S::template DistOut<float>(s, m.pf, df);
+      r    = f >= kMixFltBig ? kInfLength : double(f);
+      fp64 = f != f;
     }
     if (fp64) r = OP == kOpDistIn ? S::template DistIn<double>(s, p, d) : S::template DistOut<double>(s, p, d);
     return r;
   }
+}
+
+// Contains (uint8), Inside (int32) or a distance/safety (double) of one point.
+template <class Data, int OP>
+VGMI_DEV double mixed_point(Data const &s, Vec3<double> const &p, Vec3<double> const &d)
+{
+  if constexpr ((OP == kOpDistIn || OP == kOpDistOut) && !CertifiedF32Dist<Data>::value)
+    return mixed_answer<Data, OP>(s, MixedState{}, p, d);
+  else
+    return mixed_answer<Data, OP>(s, mixed_state(s, p), p, d);
 }
 
 } // namespace vgmi
