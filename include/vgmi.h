@@ -56,7 +56,8 @@ enum vgmi_shape_type {
   VGMI_TET = 14,        /* the four corners p0 p1 p2 p3 (12 values) volumes/UnplacedTet.h:48-49 */
   VGMI_ELLIPTICALTUBE = 15, /* dx dy dz                             volumes/UnplacedEllipticalTube.h:51 */
   VGMI_ELLIPSOID = 16,  /* dx dy dz [zBottomCut zTopCut]           volumes/UnplacedEllipsoid.h:62 */
-  VGMI_GENTRAP = 17     /* x[8] y[8] dz, planar faces only         volumes/UnplacedGenTrap.h:46 */
+  VGMI_GENTRAP = 17,    /* x[8] y[8] dz, planar faces only         volumes/UnplacedGenTrap.h:46 */
+  VGMI_CUTTUBE = 18     /* rmin rmax dz sphi dphi bx by bz tx ty tz volumes/UnplacedCutTube.h:54 */
 };
 
 /* Query methods of VUnplacedVolume / VPlacedVolume (volumes/UnplacedVolume.h:88-223). */

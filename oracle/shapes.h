@@ -347,6 +347,32 @@ public:
   double DistPlane(int i, V3 const &p) const { return p.Dot(fnormal[i]) + fdist[i]; }
 };
 
+// ---------------------------------------------------------------- CutTube
+// VecGeom/volumes/CutTubeStruct.h:36-47 (a tube of infinite half-length and two cut planes through
+// (0, 0, -dz) and (0, 0, dz)), VecGeom/volumes/CutPlanes.h:123-173 with Plane.h:44-191,
+// VecGeom/volumes/kernel/CutTubeImplementation.h:84-370, source/UnplacedCutTube.cpp:128-192 (extent,
+// normal), VecGeom/volumes/UnplacedCutTube.h:36-50 (constructor check).
+class CutTube : public Shape {
+public:
+  Tube tube; // rmin, rmax, kInfLength, sphi, dphi
+  double dz, maxVal, cosPhi1, sinPhi1, cosPhi2, sinPhi2;
+  V3 n[2];        // bottom, top unit normals (outward)
+  double dist[2]; // plane offsets: DistPlane = p.n + dist
+  bool ok = false;
+  CutTube(double rmin, double rmax, double z, double sphi, double dphi, V3 const &bottom, V3 const &top);
+  double DistPlane(int i, V3 const &p) const { return p.Dot(n[i]) + dist[i]; }
+  int CutInside(V3 const &p) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+  double DistanceToInKernel(V3 const &p, V3 const &d, double step_max) const;
+};
+
 // ---------------------------------------------------------------- Cone (UniversalCone)
 // VecGeom/volumes/ConeStruct.h:15-431, VecGeom/volumes/kernel/ConeImplementation.h:95-521 and the scalar
 // ConeHelpers<Precision> specialisation in VecGeom/volumes/ConeUtilities.h:580-820

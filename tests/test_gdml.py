@@ -46,6 +46,9 @@ ONE_SOLID = [
     ("oneTube.gdml", (S.TUBE, [0.0, 100 * CM, 100 * CM / 2, 0.0, 360 * DEG])),
     # semi-axes and cuts as written (processEllipsoid halves nothing)
     ("oneEllipsoid.gdml", (S.ELLIPSOID, [30 * CM, 70 * CM, 100 * CM, 20 * CM, 40 * CM])),
+    # z halved; the cut normals scaled by the length unit like lengths (they are made unit afterwards)
+    ("oneCutTube.gdml", (S.CUTTUBE, [1 * CM, 5 * CM, 20 * CM / 2, 1.0, 4.0, -15 * CM, -15 * CM, -15 * CM,
+                                     10 * CM, 10 * CM, 10 * CM])),
 ]
 
 
@@ -120,10 +123,9 @@ def test_rotation_is_the_passive_xyz_rotation():
 
 
 def test_unsupported_solids_fail(tmp_path):
-    text = BODY % ('<cutTube name="c" z="10" rmin="0" rmax="5" startphi="0" deltaphi="360" aunit="deg" '
-                   'lowX="0" lowY="0" lowZ="-1" highX="0" highY="0" highZ="1"/>',
+    text = BODY % ('<elcone name="c" dx="1" dy="2" zmax="10" zcut="5"/>',
                    "<volume name='W'><solidref ref='c'/></volume>", "W")
-    with pytest.raises(VgmiError, match="cutTube"):
+    with pytest.raises(VgmiError, match="elcone"):
         load_gdml(_write(tmp_path, text))
 
 

@@ -21,7 +21,8 @@ FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "gdml")
 FILES = ["allSolids.gdml", "aRotation.gdml", "aPosition.gdml", "twoBoxes.gdml", "oneBox.gdml", "oneConeSegment.gdml",
          "oneGeneralTrapezoid.gdml", "oneHyperbolicTube.gdml", "oneOrb.gdml", "oneParaboloid.gdml",
          "onePolycone.gdml", "onePolyhedron.gdml", "oneSphere.gdml", "oneTesselatedSolid.gdml", "oneTorus.gdml",
-         "oneTrapezoid.gdml", "oneTube.gdml", "oneEllipsoid.gdml"]
+         "oneTrapezoid.gdml", "oneTube.gdml", "oneEllipsoid.gdml",
+         "oneCutTube.gdml"]
 KIND = {v: k for k, v in __import__("oracle.pyoracle", fromlist=["SHAPE_TYPES"]).SHAPE_TYPES.items()}
 
 
@@ -57,9 +58,9 @@ def _geometries():
 def planar_solids():
     """Parallelepipeds, tets, elliptical tubes, ellipsoids and planar generic trapezoids placed with
     rotations in a box world, one parallelepiped holding a tet, one elliptical tube holding a cut
-    ellipsoid, one generic trapezoid holding a tet."""
+    ellipsoid, one generic trapezoid and one cut tube holding a tet."""
     from tests.test_nav_path import _rot
-    from vecgeom_amd.shapes import BOX, ELLIPSOID, ELLIPTICALTUBE, GENTRAP, PARALLELEPIPED, TET
+    from vecgeom_amd.shapes import BOX, CUTTUBE, ELLIPSOID, ELLIPTICALTUBE, GENTRAP, PARALLELEPIPED, TET
     g = geometry.GeometryDesc()
     w = g.add_logical(g.add_shape(BOX, [300.0, 300.0, 300.0]), "world")
     g.set_world(w)
@@ -85,6 +86,10 @@ def planar_solids():
     g.place(gtra, small, (0.0, 0.0, -10.0) + _rot("z", 0.9))
     g.place(w, gtra, (-200.0, -20.0, 150.0) + _rot("y", 0.7))
     g.place(w, gtra, (220.0, 60.0, -180.0) + _rot("x", -1.0))
+    ctub = g.add_logical(g.add_shape(CUTTUBE, [0.0, 30.0, 40.0, 0.0, 2 * np.pi, 0.0, -0.3, -1.0, 0.2, 0.0, 1.0]),
+                         "cut tube")
+    g.place(ctub, small, (0.0, 5.0, 0.0) + _rot("y", 0.3))
+    g.place(w, ctub, (-20.0, 200.0, 180.0) + _rot("z", 0.8))
     return g
 
 

@@ -86,6 +86,9 @@ CASES = [
     ("gentrap", [-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, -2.0, 2.0, 2.0, -2.0, 5.0]),
     ("gentrap", [-4.0, 5.0, 6.0, -3.0, -1.4, 4.0, 4.6, -0.8, -5.0, -4.0, 4.0, 3.0, -2.5, -1.9, 2.9, 2.3, 8.0]),
     ("gentrap", [-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, 0.0, 0.0, 0.0, 0.0, 6.0]),
+    ("cuttube", [0.0, 10.0, 20.0, 0.0, 2 * math.pi, 0.0, -0.7, -0.71, 0.7, 0.0, 0.71]),
+    ("cuttube", [3.0, 8.0, 15.0, math.radians(30), math.radians(240), 0.3, 0.2, -1.0, -0.4, 0.1, 1.0]),
+    ("cuttube", [1.0, 5.0, 10.0, 1.0, 4.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing then stops at a neighbouring
@@ -198,11 +201,12 @@ def test_tessellated_step_max(case):
         assert (free != ref).sum() > 10, m
 
 
-@pytest.mark.parametrize("case", [i for i, c in enumerate(CASES) if c[0] == "gentrap"])
-def test_gentrap_step_max(case):
-    """A planar GenTrap's DistanceToIn hands step_max to its facet search
-    (GenTrapImplementation.h:471-479 -> TessellatedSection::DistanceToIn with stepMax): facets beyond
-    the limit are not entered. Limits from 0 to beyond the solid, per point; DistanceToOut ignores the limit."""
+@pytest.mark.parametrize("case", [i for i, c in enumerate(CASES) if c[0] in ("gentrap", "cuttube")])
+def test_step_limited_distance_to_in(case):
+    """The GenTrap's and the CutTube's DistanceToIn take step_max: a planar GenTrap hands it to its facet
+    search (GenTrapImplementation.h:471-479 -> TessellatedSection::DistanceToIn), facets beyond it are not
+    entered; the CutTube takes a plane or tube crossing only below it (CutTubeImplementation.h:176,240).
+    Limits from 0 to beyond the solid, per point; DistanceToOut ignores the limit."""
     from vecgeom_amd import make_shape
     import torch
     kind, params = CASES[case]

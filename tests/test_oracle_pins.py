@@ -323,6 +323,23 @@ for _t in ([5.0, 10.0, 20.0], [30.0, 12.0, 8.0]):
     CASES.append(("ellipticaltube", _t, margin_eltube))
 
 
+def margin_cuttube(params, p):
+    """A tube of unbounded length cut by the planes through (0, 0, -dz) and (0, 0, dz) with the given
+    outward normals: the tube margin without its z term, and each plane's signed distance."""
+    rmin, rmax, dz, sphi, dphi = params[:5]
+    m = margin_tube([rmin, rmax, np.inf, sphi, dphi], p)
+    for nrm, z0 in ((params[5:8], -dz), (params[8:11], dz)):
+        u = np.asarray(nrm, float) / np.linalg.norm(nrm)
+        m = np.minimum(m, -(u[0] * p[0] + u[1] * p[1] + u[2] * (p[2] - z0)))
+    return m
+
+
+for _t in ([0.0, 10.0, 20.0, 0.0, TWO_PI, 0.0, -0.7, -0.71, 0.7, 0.0, 0.71],
+           [3.0, 8.0, 15.0, math.radians(30), math.radians(240), 0.3, 0.2, -1.0, -0.4, 0.1, 1.0],
+           [1.0, 5.0, 10.0, 1.0, 4.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0]):
+    CASES.append(("cuttube", _t, margin_cuttube))
+
+
 def gentrap_corners(params):
     """The eight corners of a GenTrap: (x[i], y[i], -dz) for i < 4, (x[i], y[i], +dz) above."""
     x, y, dz = params[:8], params[8:16], params[16]

@@ -182,6 +182,7 @@ static int method_groups(int type, MethodGroup g[3])
   case kEllipticalTube:
   case kEllipsoid:
   case kGenTrap:
+  case kCutTube:
     g[0] = {kMAll, 2};
     return 1;
   case kCone:

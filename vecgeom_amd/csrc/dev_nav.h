@@ -95,7 +95,7 @@ VGMI_DEV void top_matrix(GeomView const &g, uint32_t nav, Mat &m)
 // the bit mask of shape types the kernel compiles in (bit E for ShapeType E).
 // The distances take the step limit the reference's navigator hands the shapes: the mother's
 // DistanceToOut gets the step limit (navigation/VNavigator.h:246), a daughter's DistanceToIn the
-// current step (base/BVH.h:115). Only the tessellated solid's and the GenTrap's DistanceToIn depend on it.
+// current step (base/BVH.h:115). Only the tessellated solid's, the GenTrap's and the CutTube's DistanceToIn depend on it.
 template <unsigned TM>
 VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d, double stepMax)
 {
@@ -103,6 +103,8 @@ VGMI_DEV double dist_in(ShapeRecord const &s, V3 const &p, V3 const &d, double s
     if (s.type == kTessellated) return tess::dist_in_step64(s.u.tess, tess::pool_walk(s.u.tess), p, d, stepMax);
   if constexpr ((TM >> kGenTrap) & 1u)
     if (s.type == kGenTrap) return gtra::DistanceToInStep(s.u.gtra, p, d, stepMax);
+  if constexpr ((TM >> kCutTube) & 1u)
+    if (s.type == kCutTube) return ctub::DistanceToInStep(s.u.ctub, p, d, stepMax);
   return rec_dist_in<TM>(s, p, d);
 }
 template <unsigned TM>

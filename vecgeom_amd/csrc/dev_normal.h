@@ -769,5 +769,19 @@ VGMI_DEV bool Normal(GenTrapData const &s, V3d const &p, V3d &n, bool &set)
   return true;
 }
 
+// CutTubeImplementation::NormalKernel (VecGeom/volumes/kernel/CutTubeImplementation.h:301-322): the
+// unbounded tube's normal where its surface is nearer than the cut planes, else the cut plane of the
+// point's z half
+VGMI_DEV bool Normal(CutTubeData const &s, V3d const &p, V3d &n, bool &set)
+{
+  const double safcut  = ctub::Mn(-ctub::plane(s, 0, p), -ctub::plane(s, 1, p));
+  const double saftube = tube::SafetyToOut(s.tube, p);
+  if (vabs(saftube) < vabs(safcut)) return Normal(s.tube, p, n, set);
+  set = true;
+  n   = p.z < 0 ? V3d(s.n0[0], s.n0[1], s.n0[2]) : V3d(s.n1[0], s.n1[1], s.n1[2]);
+  return true;
+}
+
 } // namespace nrm
+
 } // namespace vgmi

@@ -3,6 +3,7 @@
 #pragma once
 #include "dev_box.h"
 #include "dev_cone.h"
+#include "dev_cuttube.h"
 #include "dev_ellipsoids.h"
 #include "dev_gentrap.h"
 #include "dev_hype.h"
@@ -40,7 +41,8 @@
   X(kTet, TetData, tet, tet)                                                                                         \
   X(kEllipticalTube, EllipticalTubeData, eltu, eltu)                                                                 \
   X(kEllipsoid, EllipsoidData, ells, ells)                                                                           \
-  X(kGenTrap, GenTrapData, gtra, gtra)
+  X(kGenTrap, GenTrapData, gtra, gtra)                                                                               \
+  X(kCutTube, CutTubeData, ctub, ctub)
 
 namespace vgmi {
 

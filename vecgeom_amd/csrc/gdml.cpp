@@ -518,6 +518,11 @@ private:
     } else if (n == "tessellated") {
       type = kTessellated;
       if (!tessellated(e, p)) return false;
+    } else if (n == "cutTube") { // processCutTube (Middleware.cpp:763-794): z halved, the normals in lengths too
+      type = kCutTube;
+      p    = {l * num(e, "rmin"), l * num(e, "rmax"), l * num(e, "z") / 2., a * num(e, "startphi"),
+           a * num(e, "deltaphi"), l * num(e, "lowX"), l * num(e, "lowY"), l * num(e, "lowZ"),
+           l * num(e, "highX"), l * num(e, "highY"), l * num(e, "highZ")};
     } else if (n == "eltube") { // processElTube (Middleware.cpp:747-761): semi-axes, not halved
       type = kEllipticalTube, p = {l * num(e, "dx"), l * num(e, "dy"), l * num(e, "dz")};
     } else if (n == "ellipsoid") { // processEllipsoid (Middleware.cpp:948-965)

@@ -43,6 +43,9 @@ VGMI_DEV double placed_dist(Data const &s, int op, bool mixed, Vec3<double> cons
   } else if constexpr (S::kType == kGenTrap) {
     if (in) return gtra::DistanceToInStep(s, p, d, stepMax); // the step limit bounds the entry search
     return S::template DistOut<double>(s, p, d);
+  } else if constexpr (S::kType == kCutTube) {
+    if (in) return ctub::DistanceToInStep(s, p, d, stepMax); // planes hit beyond the limit are not taken
+    return S::template DistOut<double>(s, p, d);
   } else {
     (void)stepMax; // the reference's other kernels ignore the step limit
     if (mixed) return in ? mixed_point<Data, kOpDistIn>(s, p, d) : mixed_point<Data, kOpDistOut>(s, p, d);

@@ -29,7 +29,8 @@ enum ShapeType : int32_t {
   kEllipticalTube = 15,
   kEllipsoid = 16,
   kGenTrap = 17,
-  kNumShapeTypes = 18
+  kCutTube = 18,
+  kNumShapeTypes = 19
 };
 
 enum QueryOp : int32_t {
@@ -210,6 +211,15 @@ struct GenTrapData {
   int32_t nfacets, pad_;
 };
 
+// CutTube (VecGeom/volumes/CutTubeStruct.h): a tube of infinite half-length (its z fields are
+// kInfLength) cut by two planes through (0, 0, -dz) and (0, 0, dz); n0/n1 are the bottom/top unit
+// outward normals and d0/d1 their offsets (signed distance = p.n + d).
+struct CutTubeData {
+  TubeData tube;
+  double n0[3], d0, n1[3], d1;
+  double dz, maxVal, cosPhi1, sinPhi1, cosPhi2, sinPhi2;
+};
+
 // Tessellated solid of triangular facets (VecGeom/volumes/TessellatedStruct.h, Tile.h). The facets
 // and a threaded BVH over them live in the variable-length pool:
 //   facet k (kFacetDoubles doubles): single-precision box lo[3] hi[3] (3 doubles), normal[3],
@@ -269,6 +279,7 @@ struct alignas(16) ShapeRecord {
     EllipticalTubeData eltu;
     EllipsoidData ells;
     GenTrapData gtra;
+    CutTubeData ctub;
     double raw[64];
   } u;
 };

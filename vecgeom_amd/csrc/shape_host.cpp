@@ -471,6 +471,43 @@ int MakeEllipsoid(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
+// CutTube (VecGeom/volumes/UnplacedCutTube.h:54-66, CutTubeStruct.h:36-47): the tube of infinite
+// half-length, the normals made unit (Vector3D::Unit: / sqrt(mag2 + DBL_MIN)) and set as planes
+// through (0, 0, -dz) and (0, 0, dz) (Plane::Set, Plane.h:44-51: components below kTolerance zeroed,
+// normalised again, offset -n.origin). A bottom normal with z >= 0 or a top one with z <= 0 is refused.
+int MakeCutTube(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 11) return err = "cut tube needs 11 parameters (rmin, rmax, dz, sphi, dphi, bottom normal, top normal)", -1;
+  if (!(a[2] > 0)) return err = "cut tube dz must be positive", -1;
+  if (a[7] >= 0 || a[10] <= 0) return err = "Illegal normal direction for cut planes", -1;
+  const double tp[5] = {a[0], a[1], DBL_MAX, a[3], a[4]}; // kInfLength
+  ShapeRecord t;
+  if (MakeTube(tp, 5, t, err) != 0) return -1;
+  r            = ShapeRecord();
+  r.type       = kCutTube;
+  CutTubeData &c = r.u.ctub;
+  c.tube       = t.u.tube;
+  c.dz         = a[2];
+  for (int i = 0; i < 2; ++i) {
+    const double *v = a + 5 + 3 * i;
+    const double m  = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2] + DBL_MIN);
+    double f[3]     = {v[0] / m, v[1] / m, v[2] / m};
+    for (double &x : f)
+      if (std::fabs(x) < hTol) x = 0.0;
+    const double inv = 1. / std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    double *nn       = i ? c.n1 : c.n0;
+    for (int k = 0; k < 3; ++k) nn[k] = f[k] * inv;
+    const double oz = i ? c.dz : -c.dz;
+    (i ? c.d1 : c.d0) = inv * -(f[0] * 0. + f[1] * 0. + f[2] * oz);
+  }
+  c.cosPhi1 = std::cos(a[3]);
+  c.sinPhi1 = std::sin(a[3]);
+  c.cosPhi2 = std::cos(a[3] + a[4]);
+  c.sinPhi2 = std::sin(a[3] + a[4]);
+  c.maxVal  = std::max(c.tube.rmax, c.dz);
+  return 0;
+}
+
 // GenTrap: UnplacedGenTrap(x[8], y[8], dz) (VecGeom/volumes/UnplacedGenTrap.h:46) and GenTrapStruct::
 // Initialize (VecGeom/volumes/GenTrapStruct.h:57-188): SecondOrderSurfaceShell's face normals from the
 // vertices as given; anticlockwise polygons reverted to clockwise; opposite orientations, crossing opposite
@@ -1176,6 +1213,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeEllipsoid(params, nparams, r, err);
   case kGenTrap:
     return MakeGenTrap(params, nparams, r, err);
+  case kCutTube:
+    return MakeCutTube(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -1271,6 +1310,34 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
       lo[0] = std::min(lo[0], g.vx[i]), hi[0] = std::max(hi[0], g.vx[i]);
       lo[1] = std::min(lo[1], g.vy[i]), hi[1] = std::max(hi[1], g.vy[i]);
     }
+    return;
+  }
+  case kCutTube: {
+    // source/UnplacedCutTube.cpp:128-172 (ZlimitTop/ZlimitBottom: UnplacedCutTube.h:133-147)
+    CutTubeData const &c = r.u.ctub;
+    TubeData const &t    = c.tube;
+    const double rmax = t.rmax, rmin = t.rmin, sphi = t.sphi, dphi = t.dphi;
+    const double *T = c.n1, *B = c.n0;
+    const double dztop    = rmax * std::sqrt(1. - T[2] * T[2]) / T[2];
+    const double dzbottom = -rmax * std::sqrt(1. - B[2] * B[2]) / B[2];
+    lo[0] = -rmax, lo[1] = -rmax, lo[2] = -c.dz - dzbottom;
+    hi[0] = rmax, hi[1] = rmax, hi[2] = c.dz + dztop;
+    if (dphi == 2. * hPi) return;
+    auto zTop = [&](double rr, double ph) { return c.dz - (rr / T[2]) * (T[0] * std::cos(ph) + T[1] * std::sin(ph)); };
+    auto zBot = [&](double rr, double ph) { return -c.dz - (rr / B[2]) * (B[0] * std::cos(ph) + B[1] * std::sin(ph)); };
+    auto in = [&](double x, double y) { return WedgeContainsXY(t.a1x, t.a1y, t.a2x, t.a2y, t.wedgeLtPi, x, y); };
+    if (!in(-T[0], -T[1])) {
+      hi[2] = std::max(zTop(rmax, sphi), zTop(rmax, sphi + dphi));
+      hi[2] = std::max(hi[2], std::max(zTop(rmin, sphi), zTop(rmin, sphi + dphi)));
+    }
+    if (!in(-B[0], -B[1])) {
+      lo[2] = std::min(zBot(rmax, sphi), zBot(rmax, sphi + dphi));
+      lo[2] = std::min(lo[2], std::min(zBot(rmin, sphi), zBot(rmin, sphi + dphi)));
+    }
+    if (!in(1., 0.)) hi[0] = std::max(std::max(rmax * c.cosPhi1, rmax * c.cosPhi2), std::max(rmin * c.cosPhi1, rmin * c.cosPhi2));
+    if (!in(-1., 0.)) lo[0] = std::min(std::min(rmax * c.cosPhi1, rmax * c.cosPhi2), std::min(rmin * c.cosPhi1, rmin * c.cosPhi2));
+    if (!in(0., 1.)) hi[1] = std::max(std::max(rmax * c.sinPhi1, rmax * c.sinPhi2), std::max(rmin * c.sinPhi1, rmin * c.sinPhi2));
+    if (!in(0., -1.)) lo[1] = std::min(std::min(rmax * c.sinPhi1, rmax * c.sinPhi2), std::min(rmin * c.sinPhi1, rmin * c.sinPhi2));
     return;
   }
   case kEllipsoid: {

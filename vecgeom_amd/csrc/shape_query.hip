@@ -133,6 +133,20 @@ __global__ __launch_bounds__(kBlock) void k_gentrap_dist_in_step(GenTrapData s, 
   }
 }
 
+// A CutTube's DistanceToIn with a step limit per point (CutTubeImplementation.h:176-243 compare the
+// plane and tube crossings with stepMax).
+__global__ __launch_bounds__(kBlock) void k_cuttube_dist_in_step(CutTubeData s, QueryArgs a,
+                                                                 const double *__restrict__ step_max)
+{
+  size_t i     = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  size_t strid = size_t(gridDim.x) * kBlock;
+  for (; i < a.n; i += strid) {
+    const Vec3<double> p = load3<double>(a.px, a.py, a.pz, i), d = load3<double>(a.dx, a.dy, a.dz, i);
+    const double r = ctub::DistanceToInStep(s, p, d, __builtin_nontemporal_load(step_max + i));
+    __builtin_nontemporal_store(r, (double *)a.out + i);
+  }
+}
+
 int tess_lds_limit()
 {
   static std::atomic<int> cache[64];
@@ -214,7 +228,8 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
   if (!px || !py || !pz || !out) return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: null point/output pointer");
   if ((op == kOpDistIn || op == kOpDistOut) && (!dx || !dy || !dz))
     return set_error(VGMI_ERR_INVALID, "vgmi_shape_query: DistanceToIn/Out need directions");
-  // step_max: of the supported shapes only the tessellated solid and the GenTrap's DistanceToIn use it
+  // step_max: of the supported shapes only the tessellated solid and the GenTrap's and CutTube's
+  // DistanceToIn use it
   // (the reference's other kernels take the argument and ignore it); they answer every point in FP64
   if (step_max && shape->rec.type == kTessellated && (op == kOpDistIn || op == kOpDistOut)) {
     QueryArgs a{px, py, pz, dx, dy, dz, out, n};
@@ -228,6 +243,14 @@ vgmi_status vgmi_shape_query(const vgmi_shape *shape, int op, int precision, con
     QueryArgs a{px, py, pz, dx, dy, dz, out, n};
     hipLaunchKernelGGL(k_gentrap_dist_in_step, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
                        shape->rec.u.gtra, a, step_max);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_shape_query launch: %s", hipGetErrorString(e));
+    return VGMI_OK;
+  }
+  if (step_max && shape->rec.type == kCutTube && op == kOpDistIn) {
+    QueryArgs a{px, py, pz, dx, dy, dz, out, n};
+    hipLaunchKernelGGL(k_cuttube_dist_in_step, dim3(grid_for(n)), dim3(kBlock), 0, (hipStream_t)stream,
+                       shape->rec.u.ctub, a, step_max);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_error(VGMI_ERR_DEVICE, "vgmi_shape_query launch: %s", hipGetErrorString(e));
     return VGMI_OK;
