@@ -200,10 +200,10 @@ def run_mix(args):
                                          if k not in {s for s, _ in shapes}],
                       "tuning": tuning},
            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("vgmi::k_batch<%d>" % batch.info()["block"], "vgmi::k_batch<"),
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic("vgmi::k_batch<%d, false>" % batch.info()["block"], "vgmi::k_batch<"),
                         "kernel": "k_batch",
                         "bytes_per_launch": BYTES_ALL * total, "avg_launch_us": round(us, 2)},
-           "valu": valu_entry("vgmi::k_batch<%d>" % batch.info()["block"])}
+           "valu": valu_entry("vgmi::k_batch<%d, false>" % batch.info()["block"])}
     # the same mix and launch shape in mixed precision (FP32 bulk, FP64 near the surface)
     from vecgeom_amd.shapes import MIXED
     mouts = batch.alloc_outputs()
@@ -538,7 +538,7 @@ def main():
             "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": mix["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic", "config": mix["config"], "roofline": mix["roofline"],
-            "cpu_baseline": mix.get("cpu_baseline")}
+            "cpu_baseline": mix.get("cpu_baseline"), "valu": mix.get("valu"), "mixed": mix.get("mixed")}
     line.update(extra)
     print(json.dumps(line), flush=True)
 
