@@ -57,7 +57,8 @@ enum vgmi_shape_type {
   VGMI_ELLIPTICALTUBE = 15, /* dx dy dz                             volumes/UnplacedEllipticalTube.h:51 */
   VGMI_ELLIPSOID = 16,  /* dx dy dz [zBottomCut zTopCut]           volumes/UnplacedEllipsoid.h:62 */
   VGMI_GENTRAP = 17,    /* x[8] y[8] dz, planar faces only         volumes/UnplacedGenTrap.h:46 */
-  VGMI_CUTTUBE = 18     /* rmin rmax dz sphi dphi bx by bz tx ty tz volumes/UnplacedCutTube.h:54 */
+  VGMI_CUTTUBE = 18,    /* rmin rmax dz sphi dphi bx by bz tx ty tz volumes/UnplacedCutTube.h:54 */
+  VGMI_ELLIPTICALCONE = 19 /* a b h zcut                          volumes/UnplacedEllipticalCone.h */
 };
 
 /* Query methods of VUnplacedVolume / VPlacedVolume (volumes/UnplacedVolume.h:88-223). */

@@ -17,6 +17,7 @@
 //  16 ellipsoid   dx dy dz [zBottomCut zTopCut]               (volumes/UnplacedEllipsoid.h:62; cuts default 0)
 //  17 gentrap     x[8] y[8] dz (planar ones)                  (volumes/UnplacedGenTrap.h:46)
 //  18 cuttube     rmin rmax dz sphi dphi bx by bz tx ty tz     (volumes/UnplacedCutTube.h:54-58)
+//  19 ellipticalcone a b h zcut                                (volumes/UnplacedEllipticalCone.h)
 //   9 tessellated ntri, then 9 doubles per triangle (v0 v1 v2, anticlockwise seen from outside;
 //                 UnplacedTessellated::AddTriangularFacet, volumes/UnplacedTessellated.h:95-100)
 #include "shapes.h"
@@ -97,6 +98,8 @@ Shape *MakeShape(int type, const double *a, int n)
     }
     return g;
   }
+  case 19:
+    return n >= 4 ? new EllipticalCone(a[0], a[1], a[2], a[3]) : nullptr;
   case 18: { // rmin rmax dz sphi dphi, bottom normal (3), top normal (3)
     if (n < 11) return nullptr;
     auto *c = new CutTube(a[0], a[1], a[2], a[3], a[4], V3(a[5], a[6], a[7]), V3(a[8], a[9], a[10]));

@@ -15,7 +15,7 @@ OPS = {"Contains": 0, "Inside": 1, "DistanceToIn": 2, "DistanceToOut": 3, "Safet
 SHAPE_TYPES = {"box": 0, "tube": 1, "cone": 2, "polycone": 3, "polyhedron": 4, "trapezoid": 5,
                "sphere": 6, "torus2": 7, "orb": 8, "tessellated": 9, "hype": 10,
                "trd": 11, "paraboloid": 12, "parallelepiped": 13, "tet": 14,
-               "ellipticaltube": 15, "ellipsoid": 16, "gentrap": 17, "cuttube": 18}
+               "ellipticaltube": 15, "ellipsoid": 16, "gentrap": 17, "cuttube": 18, "ellipticalcone": 19}
 
 
 def lib():

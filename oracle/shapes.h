@@ -347,6 +347,24 @@ public:
   double DistPlane(int i, V3 const &p) const { return p.Dot(fnormal[i]) + fdist[i]; }
 };
 
+// ---------------------------------------------------------------- EllipticalCone
+// VecGeom/volumes/EllipticalConeStruct.h, source/UnplacedEllipticalCone.cpp:28-75 (parameter clamps,
+// precalculated values, extent), VecGeom/volumes/kernel/EllipticalConeImplementation.h:63-386
+class EllipticalCone : public Shape {
+public:
+  double dx, dy, dz, zCut, rsph, invDx, invDy, cosAxisMin, dApex;
+  EllipticalCone(double a, double b, double h, double zcut);
+  double Safety(V3 const &p) const;
+  bool Contains(V3 const &p) const override;
+  int Inside(V3 const &p) const override;
+  double DistanceToIn(V3 const &p, V3 const &d, double step_max) const override;
+  double DistanceToOut(V3 const &p, V3 const &d, double step_max) const override;
+  double SafetyToIn(V3 const &p) const override;
+  double SafetyToOut(V3 const &p) const override;
+  bool Normal(V3 const &p, V3 &n) const override;
+  void Extent(V3 &lo, V3 &hi) const override;
+};
+
 // ---------------------------------------------------------------- CutTube
 // VecGeom/volumes/CutTubeStruct.h:36-47 (a tube of infinite half-length and two cut planes through
 // (0, 0, -dz) and (0, 0, dz)), VecGeom/volumes/CutPlanes.h:123-173 with Plane.h:44-191,

@@ -870,6 +870,7 @@ SHAPES = {
     "ellipticaltube": ("TestEllipticalTube.cpp", r"EllipticalTube_t", (3,), r"UnplacedEllipticalTube"),
     "ellipsoid": ("TestEllipsoid.cpp", r"Ellipsoid_t", (5,), r"UnplacedEllipsoid"),
     "gentrap": ("TestGenTrap.cpp", r"SimpleGenTrap", (17,), r"UnplacedGenTrap"),
+    "ellipticalcone": ("TestEllipticalCone.cpp", r"EllipticalCone_t", (4,), r"UnplacedEllipticalCone"),
 }
 
 
@@ -877,6 +878,7 @@ SHAPES = {
 # (UnplacedEllipsoid::SetSemiAxes / SetZCuts, UnplacedEllipticalTube::SetParameters / SetDx / SetDy / SetDz).
 SETTERS = {
     "ellipsoid": {"SetSemiAxes": lambda p, a: a[:3] + p[3:], "SetZCuts": lambda p, a: p[:3] + a[:2]},
+    "ellipticalcone": {"SetParameters": lambda p, a: a[:4]},
     "ellipticaltube": {"SetParameters": lambda p, a: a[:3], "SetDx": lambda p, a: [a[0]] + p[1:],
                        "SetDy": lambda p, a: p[:1] + [a[0]] + p[2:], "SetDz": lambda p, a: p[:2] + [a[0]]},
 }

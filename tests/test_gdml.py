@@ -47,6 +47,7 @@ ONE_SOLID = [
     # semi-axes and cuts as written (processEllipsoid halves nothing)
     ("oneEllipsoid.gdml", (S.ELLIPSOID, [30 * CM, 70 * CM, 100 * CM, 20 * CM, 40 * CM])),
     # z halved; the cut normals scaled by the length unit like lengths (they are made unit afterwards)
+    ("oneEllipticalCone.gdml", (S.ELLIPTICALCONE, [70 * CM, 30 * CM, 100 * CM, 15 * CM])),  # all in lengths
     ("oneCutTube.gdml", (S.CUTTUBE, [1 * CM, 5 * CM, 20 * CM / 2, 1.0, 4.0, -15 * CM, -15 * CM, -15 * CM,
                                      10 * CM, 10 * CM, 10 * CM])),
 ]
@@ -123,9 +124,9 @@ def test_rotation_is_the_passive_xyz_rotation():
 
 
 def test_unsupported_solids_fail(tmp_path):
-    text = BODY % ('<elcone name="c" dx="1" dy="2" zmax="10" zcut="5"/>',
-                   "<volume name='W'><solidref ref='c'/></volume>", "W")
-    with pytest.raises(VgmiError, match="elcone"):
+    text = BODY % ('<twistedtubs name="c" twistedangle="30" endinnerrad="10" endouterrad="15" zlen="40" '
+                   'phi="90" aunit="deg"/>', "<volume name='W'><solidref ref='c'/></volume>", "W")
+    with pytest.raises(VgmiError, match="twistedtubs"):
         load_gdml(_write(tmp_path, text))
 
 

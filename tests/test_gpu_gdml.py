@@ -22,7 +22,7 @@ FILES = ["allSolids.gdml", "aRotation.gdml", "aPosition.gdml", "twoBoxes.gdml", 
          "oneGeneralTrapezoid.gdml", "oneHyperbolicTube.gdml", "oneOrb.gdml", "oneParaboloid.gdml",
          "onePolycone.gdml", "onePolyhedron.gdml", "oneSphere.gdml", "oneTesselatedSolid.gdml", "oneTorus.gdml",
          "oneTrapezoid.gdml", "oneTube.gdml", "oneEllipsoid.gdml",
-         "oneCutTube.gdml"]
+         "oneCutTube.gdml", "oneEllipticalCone.gdml"]
 KIND = {v: k for k, v in __import__("oracle.pyoracle", fromlist=["SHAPE_TYPES"]).SHAPE_TYPES.items()}
 
 
@@ -60,7 +60,7 @@ def planar_solids():
     rotations in a box world, one parallelepiped holding a tet, one elliptical tube holding a cut
     ellipsoid, one generic trapezoid and one cut tube holding a tet."""
     from tests.test_nav_path import _rot
-    from vecgeom_amd.shapes import BOX, CUTTUBE, ELLIPSOID, ELLIPTICALTUBE, GENTRAP, PARALLELEPIPED, TET
+    from vecgeom_amd.shapes import BOX, CUTTUBE, ELLIPTICALCONE, ELLIPSOID, ELLIPTICALTUBE, GENTRAP, PARALLELEPIPED, TET
     g = geometry.GeometryDesc()
     w = g.add_logical(g.add_shape(BOX, [300.0, 300.0, 300.0]), "world")
     g.set_world(w)
@@ -90,6 +90,9 @@ def planar_solids():
                          "cut tube")
     g.place(ctub, small, (0.0, 5.0, 0.0) + _rot("y", 0.3))
     g.place(w, ctub, (-20.0, 200.0, 180.0) + _rot("z", 0.8))
+    elco = g.add_logical(g.add_shape(ELLIPTICALCONE, [0.5, 0.4, 60.0, 40.0]), "elliptical cone")
+    g.place(w, elco, (-180.0, 180.0, -180.0) + _rot("x", 0.5))
+    g.place(w, elco, (200.0, -200.0, 200.0) + _rot("y", -0.9))
     return g
 
 

@@ -89,6 +89,9 @@ CASES = [
     ("cuttube", [0.0, 10.0, 20.0, 0.0, 2 * math.pi, 0.0, -0.7, -0.71, 0.7, 0.0, 0.71]),
     ("cuttube", [3.0, 8.0, 15.0, math.radians(30), math.radians(240), 0.3, 0.2, -1.0, -0.4, 0.1, 1.0]),
     ("cuttube", [1.0, 5.0, 10.0, 1.0, 4.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0]),
+    ("ellipticalcone", [0.5, 0.4, 10.0, 5.0]),
+    ("ellipticalcone", [1.0, 2.0, 3.0, 4.0]),
+    ("ellipticalcone", [0.3, 0.9, 20.0, 8.0]),
 ]
 # Torus2 solves its resolvent cubic with cbrt/pow/atan2/cos; the device math library may differ
 # from the host C library in the last bit, and the Newton polishing then stops at a neighbouring
@@ -136,7 +139,8 @@ def test_query_all_matches_single_methods(case):
 
 @pytest.mark.parametrize("shape", ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron",
                                    "torus2", "tessellated", "hype", "trd", "paraboloid", "parallelepiped",
-                                   "tet", "ellipticaltube", "ellipsoid", "gentrap"])
+                                   "tet", "ellipticaltube", "ellipsoid", "gentrap",
+                                   "ellipticalcone"])
 def test_gpu_reference_kats(shape):
     """The reference's known-answer asserts, evaluated by the GPU kernels."""
     from vecgeom_amd import make_shape

@@ -10,7 +10,8 @@ from oracle.pyoracle import OracleShape
 from tests import kat_util
 
 SHAPES = ["box", "tube", "trapezoid", "cone", "orb", "polycone", "sphere", "polyhedron", "torus2", "tessellated",
-          "hype", "trd", "paraboloid", "parallelepiped", "tet", "ellipticaltube", "ellipsoid", "gentrap"]
+          "hype", "trd", "paraboloid", "parallelepiped", "tet", "ellipticaltube", "ellipsoid", "gentrap",
+          "ellipticalcone"]
 
 
 def _oracle_distance(kind):

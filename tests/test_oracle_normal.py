@@ -50,6 +50,8 @@ CASES = [
     # normal by the reverted polygon's edge index from normals made in the order given
     # (GenTrapStruct.h:67-101, GenTrapImplementation.h:654-660), which is not the exit face's
     ("gentrap", [-4.0, -3.0, 6.0, 5.0, -1.4, -0.8, 4.6, 4.0, -5.0, 3.0, 4.0, -4.0, -2.5, 2.3, 2.9, -1.9, 8.0]),
+    ("ellipticalcone", [0.5, 0.4, 10.0, 5.0]),
+    ("ellipticalcone", [0.3, 0.9, 20.0, 8.0]),
     ("cuttube", [0.0, 10.0, 20.0, 0.0, 2 * math.pi, 0.0, -0.7, -0.71, 0.7, 0.0, 0.71]),
     ("cuttube", [3.0, 8.0, 15.0, math.radians(30), math.radians(240), 0.3, 0.2, -1.0, -0.4, 0.1, 1.0]),
 ]

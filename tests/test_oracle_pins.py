@@ -340,6 +340,20 @@ for _t in ([0.0, 10.0, 20.0, 0.0, TWO_PI, 0.0, -0.7, -0.71, 0.7, 0.0, 0.71],
     CASES.append(("cuttube", _t, margin_cuttube))
 
 
+def margin_elcone(params, p):
+    """An elliptical cone sqrt((x/a)^2 + (y/b)^2) + z <= h, |z| <= zcut: the cone term scaled by
+    min(a, b) / sqrt(1 + min(a, b)^2), which makes it 1-Lipschitz (its gradient is at most
+    sqrt(1 + 1 / min(a, b)^2))."""
+    a, b, h, zc = params
+    m = min(a, b)
+    cone = (h - np.sqrt((p[0] / a) ** 2 + (p[1] / b) ** 2) - p[2]) * m / math.sqrt(1 + m * m)
+    return np.minimum(cone, zc - np.abs(p[2]))
+
+
+for _t in ([0.5, 0.4, 10.0, 5.0], [1.0, 2.0, 3.0, 4.0 - 1.0], [0.3, 0.9, 20.0, 8.0]):
+    CASES.append(("ellipticalcone", _t, margin_elcone))
+
+
 def gentrap_corners(params):
     """The eight corners of a GenTrap: (x[i], y[i], -dz) for i < 4, (x[i], y[i], +dz) above."""
     x, y, dz = params[:8], params[8:16], params[16]

@@ -183,6 +183,7 @@ static int method_groups(int type, MethodGroup g[3])
   case kEllipsoid:
   case kGenTrap:
   case kCutTube:
+  case kEllipticalCone:
     g[0] = {kMAll, 2};
     return 1;
   case kCone:

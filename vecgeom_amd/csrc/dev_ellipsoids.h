@@ -220,4 +220,136 @@ VGMI_DEV R SafetyToOut(EllipsoidData const &s, Vec3<R> const &p)
 }
 
 } // namespace ells
+
+// EllipticalCone (VecGeom/volumes/kernel/EllipticalConeImplementation.h:63-304): the cone
+// sqrt((x/dx)^2 + (y/dy)^2) + z = h cut at |z| = zCut; distances solve the scaled quadric.
+namespace elco {
+
+template <class R>
+VGMI_DEV R safety(EllipticalConeData const &s, Vec3<R> const &p)
+{
+  const R px = p.x * R(s.invDx), py = p.y * R(s.invDy), pz = p.z;
+  const R hp = vsqrt(px * px + py * py) + pz;
+  return eltu::Mx((hp - R(s.dz)) * R(s.cosAxisMin), vabs(pz) - R(s.zCut));
+}
+
+template <class R>
+VGMI_DEV bool Contains(EllipticalConeData const &s, Vec3<R> const &p)
+{
+  return !(safety(s, p) > R(kHalfTolerance));
+}
+
+template <class R>
+VGMI_DEV int Inside(EllipticalConeData const &s, Vec3<R> const &p)
+{
+  const R v = safety(s, p);
+  return v <= -R(kHalfTolerance) ? kInside : (v > R(kHalfTolerance) ? kOutside : kSurface);
+}
+
+// :118-205 (far points are first moved to 2 rsph from the origin along the ray)
+template <class R>
+VGMI_DEV R DistanceToIn(EllipticalConeData const &s, Vec3<R> const &point, Vec3<R> const &d)
+{
+  const R kTwoEps = R(2.) * R(DBL_EPSILON); // 2 kEpsilon
+  R distance = InfLen<R>(), offset = R(0.);
+  Vec3<R> p  = point;
+  const R rsph = R(s.rsph);
+  if (p.Mag2() > R(1024.) * rsph * rsph && d.Dot(p) < R(0.)) offset = vsqrt(p.Mag2()) - R(2.) * rsph;
+  p          = p + d * offset;
+  const R px = p.x * R(s.invDx), py = p.y * R(s.invDy), pz = p.z, pz0 = p.z - R(s.dz);
+  const R vx = d.x * R(s.invDx), vy = d.y * R(s.invDy);
+  R vz       = d.z;
+  const R Ar = vx * vx + vy * vy, Br = px * vx + py * vy, Cr = px * px + py * py;
+  const R vzvz     = vz * vz;
+  const bool tinyA = vabs(Ar - vzvz) < kTwoEps * vzvz;
+  if (tinyA) vz = vz + vabs(vz) * kTwoEps;
+  const R Az = vz * vz, Bz = pz0 * vz, Cz = pz0 * pz0;
+  const R A = Ar - Az, B = Br - Bz, B0 = Br - pz0 * d.z, C = Cr - Cz;
+  R D         = B * B - A * C;
+  const R sfz = vabs(pz) - R(s.zCut);
+  R nz        = vsqrt(Cr);
+  const R sfr = (nz + pz0) * R(s.cosAxisMin);
+  if (vabs(p.x) + vabs(p.y) < R(0.1) * R(kHalfTolerance)) nz = R(1.);
+  const R pzA = pz0 + R(s.dApex);
+  bool done   = (sfz >= -R(kHalfTolerance) && pz * vz >= R(0.)) || (sfr >= -R(kHalfTolerance) && Br + nz * vz >= R(0.)) ||
+              (pz0 * R(s.cosAxisMin) > -R(kHalfTolerance) && (Cr - pzA * pzA) <= R(0.) && A >= R(0.));
+  if (sfr <= R(0.) && D < R(0.)) D = R(0.);
+  done = done || (D < R(0.)) || ((D < kTwoEps * B * B) && (A >= R(0.)));
+  const R invz = R(-1.) / NonZero(vz);
+  const R ddz  = vcopysign(R(s.zCut), invz);
+  const R tzin = (pz - ddz) * invz, tzout = (pz + ddz) * invz;
+  R tmp = R(0.), t1 = R(0.), t2 = R(0.);
+  if (!done) tmp = -B - vcopysign(vsqrt(D), B);
+  if (!done) t1 = tmp / A;
+  if (!done && tmp != R(0.)) t2 = C / tmp;
+  if (!done && tinyA && B != R(0.)) t2 = -C / (R(2.) * B0);
+  const R tmin = eltu::Mn(t1, t2), tmax = eltu::Mx(t1, t2);
+  R trin = tmin, trout = tmax;
+  done = done || (A >= R(0.) && pz0 + vz * tmin >= R(0.));
+  if (!done && A < R(0.)) trin = -InfLen<R>(), trout = InfLen<R>();
+  if (!done && A < R(0.) && vz < R(0.)) trin = tmax;
+  if (!done && A < R(0.) && vz > R(0.)) trout = tmin;
+  const R tin = eltu::Mx(tzin, trin), tout = eltu::Mn(tzout, trout);
+  if (!done && (tout - tin) >= R(kHalfTolerance)) distance = tin + offset;
+  return distance;
+}
+
+// :208-270
+template <class R>
+VGMI_DEV R DistanceToOut(EllipticalConeData const &s, Vec3<R> const &p, Vec3<R> const &d)
+{
+  const R kTwoEps = R(2.) * R(DBL_EPSILON); // 2 kEpsilon
+  R distance      = R(0.);
+  const R px = p.x * R(s.invDx), py = p.y * R(s.invDy), pz = p.z, pz0 = pz - R(s.dz);
+  const R hp         = vsqrt(px * px + py * py) + pz;
+  const R sfr        = (hp - R(s.dz)) * R(s.cosAxisMin);
+  const R sfz        = vabs(pz) - R(s.zCut);
+  const bool outside = eltu::Mx(sfr, sfz) > R(kHalfTolerance);
+  if (outside) distance = R(-1.);
+  bool done  = outside;
+  const R vx = d.x * R(s.invDx), vy = d.y * R(s.invDy);
+  R vz       = d.z;
+  const R Ar = vx * vx + vy * vy, Br = px * vx + py * vy, Cr = px * px + py * py;
+  const bool tinyA = vabs(Ar - vz * vz) < kTwoEps * vz * vz;
+  if (tinyA) vz = vz + vabs(vz) * kTwoEps;
+  const R Az = vz * vz, Bz = pz0 * vz, Cz = pz0 * pz0;
+  const R A = Ar - Az, B = Br - Bz, B0 = Br - pz0 * d.z, C = Cr - Cz;
+  R D = B * B - A * C;
+  if (sfr <= R(0.) && D < R(0.)) D = R(0.);
+  done    = done || (D < R(0.)) || (D < kTwoEps * B * B && A >= R(0.));
+  R tzout = sizeof(R) == sizeof(float) ? R(FLT_MAX) : R(DBL_MAX); // kMaximum
+  if (vz != R(0.)) tzout = (vcopysign(R(s.zCut), vz) - pz) / d.z;
+  R tmp = R(0.), t1 = R(0.), t2 = R(0.);
+  if (!done) tmp = -B - vcopysign(vsqrt(D), B);
+  if (!done) t1 = tmp / A;
+  if (!done && tmp != R(0.)) t2 = C / tmp;
+  if (!done && tinyA && B0 != R(0.)) t2 = -C / (R(2.) * B0);
+  const R tmin = eltu::Mn(t1, t2), tmax = eltu::Mx(t1, t2);
+  R trout = tmax;
+  done    = done || ((A >= R(0.) && pz0 + vz * tmax >= R(0.)) || (pz0 >= R(0.) && vz >= R(0.)));
+  if (!done && A < R(0.)) trout = InfLen<R>();
+  if (!done && A < R(0.) && vz > R(0.)) trout = tmin;
+  if (!done) distance = eltu::Mn(tzout, trout);
+  return distance;
+}
+
+// :273-287
+template <class R>
+VGMI_DEV R SafetyToIn(EllipticalConeData const &s, Vec3<R> const &p)
+{
+  const R v = safety(s, p);
+  return vabs(v) <= R(kHalfTolerance) ? R(0.) : v;
+}
+
+// :290-304
+template <class R>
+VGMI_DEV R SafetyToOut(EllipticalConeData const &s, Vec3<R> const &p)
+{
+  const R px = p.x * R(s.invDx), py = p.y * R(s.invDy), pz = p.z;
+  const R hp = vsqrt(px * px + py * py) + pz;
+  const R v  = eltu::Mn((R(s.dz) - hp) * R(s.cosAxisMin), R(s.zCut) - vabs(pz));
+  return vabs(v) <= R(kHalfTolerance) ? R(0.) : v;
+}
+
+} // namespace elco
 } // namespace vgmi

@@ -769,6 +769,31 @@ VGMI_DEV bool Normal(GenTrapData const &s, V3d const &p, V3d &n, bool &set)
   return true;
 }
 
+// EllipticalConeImplementation::NormalKernel (VecGeom/volumes/kernel/EllipticalConeImplementation.h:307-333):
+// the z plane and/or the lateral surface within half a tolerance, summed and made unit; elsewhere
+// invalid, the z direction or the lateral normal, whichever surface is nearer
+VGMI_DEV bool Normal(EllipticalConeData const &s, V3d const &p, V3d &n, bool &set)
+{
+  set = true;
+  V3d normal(0., 0., 0.);
+  bool valid     = true;
+  const double z = vabs(p.z) - s.zCut;
+  if (vabs(z) <= kHalfTolerance) normal.z = vcopysign(1., p.z);
+  const double nx = p.x * s.invDx * s.invDx, ny = p.y * s.invDy * s.invDy;
+  double nz       = vsqrt(p.x * nx + p.y * ny);
+  if ((nx * nx + ny * ny) == 0.) nz = 1.;
+  const V3d nside(nx, ny, nz);
+  const double ds = (nz + p.z - s.dz) * s.cosAxisMin;
+  if (vabs(ds) <= kHalfTolerance) normal = Unit(normal + Unit(nside));
+  if (!(normal.Mag2() > 0.)) {
+    valid    = false;
+    normal.z = vcopysign(1., p.z);
+    if (ds > z) normal = Unit(nside);
+  }
+  n = normal;
+  return valid;
+}
+
 // CutTubeImplementation::NormalKernel (VecGeom/volumes/kernel/CutTubeImplementation.h:301-322): the
 // unbounded tube's normal where its surface is nearer than the cut planes, else the cut plane of the
 // point's z half

@@ -42,7 +42,8 @@
   X(kEllipticalTube, EllipticalTubeData, eltu, eltu)                                                                 \
   X(kEllipsoid, EllipsoidData, ells, ells)                                                                           \
   X(kGenTrap, GenTrapData, gtra, gtra)                                                                               \
-  X(kCutTube, CutTubeData, ctub, ctub)
+  X(kCutTube, CutTubeData, ctub, ctub)                                                                               \
+  X(kEllipticalCone, EllipticalConeData, elco, elco)
 
 namespace vgmi {
 

@@ -518,6 +518,8 @@ private:
     } else if (n == "tessellated") {
       type = kTessellated;
       if (!tessellated(e, p)) return false;
+    } else if (n == "elcone") { // processElCone (Middleware.cpp:818-834): every value in lengths
+      type = kEllipticalCone, p = {l * num(e, "dx"), l * num(e, "dy"), l * num(e, "zmax"), l * num(e, "zcut")};
     } else if (n == "cutTube") { // processCutTube (Middleware.cpp:763-794): z halved, the normals in lengths too
       type = kCutTube;
       p    = {l * num(e, "rmin"), l * num(e, "rmax"), l * num(e, "z") / 2., a * num(e, "startphi"),

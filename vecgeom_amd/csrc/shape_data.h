@@ -30,7 +30,8 @@ enum ShapeType : int32_t {
   kEllipsoid = 16,
   kGenTrap = 17,
   kCutTube = 18,
-  kNumShapeTypes = 19
+  kEllipticalCone = 19,
+  kNumShapeTypes = 20
 };
 
 enum QueryOp : int32_t {
@@ -211,6 +212,12 @@ struct GenTrapData {
   int32_t nfacets, pad_;
 };
 
+// EllipticalCone (VecGeom/volumes/EllipticalConeStruct.h): sqrt((x dx^-1)^2 + (y dy^-1)^2) + z <= dz,
+// |z| <= zCut, with the precalculated bounding radius, inverse slopes and apex offset.
+struct EllipticalConeData {
+  double dx, dy, dz, zCut, rsph, invDx, invDy, cosAxisMin, dApex;
+};
+
 // CutTube (VecGeom/volumes/CutTubeStruct.h): a tube of infinite half-length (its z fields are
 // kInfLength) cut by two planes through (0, 0, -dz) and (0, 0, dz); n0/n1 are the bottom/top unit
 // outward normals and d0/d1 their offsets (signed distance = p.n + d).
@@ -280,6 +287,7 @@ struct alignas(16) ShapeRecord {
     EllipsoidData ells;
     GenTrapData gtra;
     CutTubeData ctub;
+    EllipticalConeData elco;
     double raw[64];
   } u;
 };

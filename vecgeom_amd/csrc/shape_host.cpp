@@ -471,6 +471,29 @@ int MakeEllipsoid(double const *a, int n, ShapeRecord &r, std::string &err)
   return 0;
 }
 
+// EllipticalCone (source/UnplacedEllipticalCone.cpp:28-63): values below 2 kTolerance raised to it, zcut
+// at most h, then the bounding radius, inverse slopes, min axis cosine and apex offset.
+int MakeEllipticalCone(double const *a, int n, ShapeRecord &r, std::string &err)
+{
+  if (n < 4) return err = "elliptical cone needs 4 parameters (a, b, h, zcut)", -1;
+  r                     = ShapeRecord();
+  r.type                = kEllipticalCone;
+  EllipticalConeData &c = r.u.elco;
+  const double tol      = 2. * hTol;
+  c.dx                  = a[0] < tol ? tol : a[0];
+  c.dy                  = a[1] < tol ? tol : a[1];
+  c.dz                  = a[2] < tol ? tol : a[2];
+  c.zCut                = a[3] < tol ? tol : a[3];
+  c.zCut                = std::min(c.dz, c.zCut);
+  const double h2 = c.dz + c.zCut, xmax = c.dx * h2, ymax = c.dy * h2, m = std::min(c.dx, c.dy);
+  c.rsph       = std::sqrt(xmax * xmax + ymax * ymax + c.zCut * c.zCut);
+  c.invDx      = 1. / c.dx;
+  c.invDy      = 1. / c.dy;
+  c.cosAxisMin = m / std::sqrt(1. + m * m);
+  c.dApex      = hHTol / c.cosAxisMin;
+  return 0;
+}
+
 // CutTube (VecGeom/volumes/UnplacedCutTube.h:54-66, CutTubeStruct.h:36-47): the tube of infinite
 // half-length, the normals made unit (Vector3D::Unit: / sqrt(mag2 + DBL_MIN)) and set as planes
 // through (0, 0, -dz) and (0, 0, dz) (Plane::Set, Plane.h:44-51: components below kTolerance zeroed,
@@ -1215,6 +1238,8 @@ int MakeShapeRecord(int type, double const *params, int nparams, ShapeRecord &r,
     return MakeGenTrap(params, nparams, r, err);
   case kCutTube:
     return MakeCutTube(params, nparams, r, err);
+  case kEllipticalCone:
+    return MakeEllipticalCone(params, nparams, r, err);
   default:
     err = "shape type not supported";
     return -1;
@@ -1310,6 +1335,13 @@ void ShapeExtent(ShapeRecord const &r, double lo[3], double hi[3])
       lo[0] = std::min(lo[0], g.vx[i]), hi[0] = std::max(hi[0], g.vx[i]);
       lo[1] = std::min(lo[1], g.vy[i]), hi[1] = std::max(hi[1], g.vy[i]);
     }
+    return;
+  }
+  case kEllipticalCone: {
+    // source/UnplacedEllipticalCone.cpp:67-75
+    EllipticalConeData const &c = r.u.elco;
+    const double xmax = c.dx * (c.dz + c.zCut), ymax = c.dy * (c.dz + c.zCut);
+    lo[0] = -xmax, lo[1] = -ymax, lo[2] = -c.zCut, hi[0] = xmax, hi[1] = ymax, hi[2] = c.zCut;
     return;
   }
   case kCutTube: {

@@ -13,7 +13,7 @@ from ._lib import check, lib
 from .soa import SOA3D, device_array, same_length
 
 (BOX, TUBE, CONE, POLYCONE, POLYHEDRON, TRAPEZOID, SPHERE, TORUS2, ORB, TESSELLATED, HYPE, TRD, PARABOLOID,
- PARALLELEPIPED, TET, ELLIPTICALTUBE, ELLIPSOID, GENTRAP, CUTTUBE) = range(19)
+ PARALLELEPIPED, TET, ELLIPTICALTUBE, ELLIPSOID, GENTRAP, CUTTUBE, ELLIPTICALCONE) = range(20)
 OPS = {"Contains": 0, "Inside": 1, "DistanceToIn": 2, "DistanceToOut": 3, "SafetyToIn": 4, "SafetyToOut": 5}
 FP64, MIXED = 0, 1
 _OUT_DTYPE = {0: torch.uint8, 1: torch.int32}
@@ -133,7 +133,7 @@ class UnplacedTube(UnplacedShape):
 _BY_NAME = {"box": BOX, "tube": TUBE, "cone": CONE, "polycone": POLYCONE, "polyhedron": POLYHEDRON,
             "trapezoid": TRAPEZOID, "sphere": SPHERE, "torus2": TORUS2, "orb": ORB, "tessellated": TESSELLATED,
             "hype": HYPE, "trd": TRD, "paraboloid": PARABOLOID, "parallelepiped": PARALLELEPIPED, "tet": TET,
-            "ellipticaltube": ELLIPTICALTUBE, "ellipsoid": ELLIPSOID, "gentrap": GENTRAP, "cuttube": CUTTUBE}
+            "ellipticaltube": ELLIPTICALTUBE, "ellipsoid": ELLIPSOID, "gentrap": GENTRAP, "cuttube": CUTTUBE, "ellipticalcone": ELLIPTICALCONE}
 
 
 def make_shape(kind, params):
