@@ -63,7 +63,14 @@ def test_extent_matches_oracle():
              ("ellipsoid", [6.0, 9.0, 4.0, 1.0, 4.0]), ("ellipsoid", [6.0, 9.0, 4.0, -4.0, -1.0]),
              ("ellipsoid", [10.0, 7.0, 12.0]),
              ("gentrap", [-3.0, -3.0, 3.0, 3.0, -2.0, -2.0, 2.0, 2.0, -3.0, 3.0, 3.0, -3.0, -2.0, 2.0, 2.0, -2.0, 5.0]),
-             ("gentrap", [-4.0, 5.0, 6.0, -3.0, -1.4, 4.0, 4.6, -0.8, -5.0, -4.0, 4.0, 3.0, -2.5, -1.9, 2.9, 2.3, 8.0])]
+             ("gentrap", [-4.0, 5.0, 6.0, -3.0, -1.4, 4.0, 4.6, -0.8, -5.0, -4.0, 4.0, 3.0, -2.5, -1.9, 2.9, 2.3, 8.0]),
+             # cut tubes: full circle, and phi ranges that reduce x, y and both z limits
+             ("cuttube", [0.0, 10.0, 20.0, 0.0, 2 * math.pi, 0.0, -0.7, -0.71, 0.7, 0.0, 0.71]),
+             ("cuttube", [3.0, 8.0, 15.0, math.radians(30), math.radians(240), 0.3, 0.2, -1.0, -0.4, 0.1, 1.0]),
+             ("cuttube", [1.0, 5.0, 10.0, 1.0, 4.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0]),
+             ("cuttube", [2.0, 6.0, 8.0, math.radians(-40), math.radians(80), 0.5, 0.0, -1.0, -0.5, 0.0, 1.0]),
+             ("ellipticalcone", [0.5, 0.4, 10.0, 5.0]), ("ellipticalcone", [1.0, 2.0, 3.0, 4.0]),
+             ("ellipticalcone", [0.0, 0.3, 5.0, 1.0])]
     # shapes with a variable-length block need a device for creation: extent from the host record
     cases = [c for c in cases if c[0] != "polyhedron"]
     for kind, params in cases:
@@ -135,3 +142,19 @@ def test_device_array_check():
         device_array(torch.zeros(4, dtype=torch.float64), torch.float64, 4, "x")
     with pytest.raises(ValueError, match="torch tensor"):
         device_array([0.0] * 4, torch.float64, 4, "x")
+
+
+def test_cut_tube_and_elliptical_cone_parameter_rules():
+    """A cut tube whose bottom normal points up or top normal points down is refused
+    (UnplacedCutTube.h:54-66); an elliptical cone's values below 2 kTolerance are raised to it and
+    zcut is held to the height (UnplacedEllipticalCone.cpp:28-42), as the oracle does."""
+    import pytest
+    from oracle.pyoracle import OracleShape
+    from vecgeom_amd import VgmiError, make_shape
+    for bad in ([0.0, 5.0, 10.0, 0.0, 6.0, 0.0, 0.0, 1.0, 0.0, 0.0, 1.0],
+                [0.0, 5.0, 10.0, 0.0, 6.0, 0.0, 0.0, -1.0, 0.0, 0.0, -1.0]):
+        with pytest.raises(VgmiError, match="normal"):
+            make_shape("cuttube", bad)
+    lo, hi = make_shape("ellipticalcone", [0.5, 0.4, 10.0, 50.0]).Extent()
+    olo, ohi = OracleShape("ellipticalcone", [0.5, 0.4, 10.0, 50.0]).extent()
+    assert hi[2] == 10.0 and list(lo) == list(olo) and list(hi) == list(ohi)
